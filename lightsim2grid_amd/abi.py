@@ -1,0 +1,300 @@
+"""ctypes front-end of the pfgpu C ABI (include/pfgpu.h).
+
+This is a thin, typed wrapper: every call goes to libpfgpu.so (HIP kernels).
+There is no CPU path behind it; if the library is missing or no GPU is
+present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpfgpu.so")
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+FP64 = 0
+FP32 = 1
+
+ERROR_NAMES = {0: "NoError", 1: "SingularMatrix", 2: "TooManyIterations", 3: "InifiniteValue", 4: "SolverAnalyze",
+               5: "SolverFactor", 6: "SolverReFactor", 7: "SolverSolve", 8: "NotInitError", 9: "LicenseError"}
+
+
+class GridInfo(ctypes.Structure):
+    _fields_ = [("n_bus", ctypes.c_int), ("n_jac", ctypes.c_int), ("nnz_ybus", ctypes.c_int),
+                ("nnz_jac", ctypes.c_int), ("nnz_lu", ctypes.c_int64), ("n_slots", ctypes.c_int),
+                ("n_levels", ctypes.c_int), ("n_back_levels", ctypes.c_int), ("n_pairs", ctypes.c_int64),
+                ("flops", ctypes.c_double), ("single_slack", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total_s", ctypes.c_double), ("h2d_s", ctypes.c_double), ("kernel_s", ctypes.c_double),
+                ("d2h_s", ctypes.c_double), ("instances_per_wg", ctypes.c_int), ("lds_slots", ctypes.c_int),
+                ("workgroups", ctypes.c_int), ("fallback_fp64", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# exported symbols and their signatures (restype, argtypes)
+SIGNATURES = {
+    "pfgpu_last_error": (ctypes.c_char_p, []),
+    "pfgpu_abi_version": (ctypes.c_int, []),
+    "pfgpu_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "pfgpu_ctx_destroy": (ctypes.c_int, [_vp]),
+    "pfgpu_ctx_device_info": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_size_t)]),
+    "pfgpu_grid_load": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, _f64p, _i32p,
+                                       ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "pfgpu_grid_destroy": (ctypes.c_int, [_vp]),
+    "pfgpu_grid_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(GridInfo)]),
+    "pfgpu_grid_get_J_pattern": (ctypes.c_int, [_vp, _i32p, _i32p]),
+    "pfgpu_solve_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _f64p, ctypes.c_int, _i32p, _i32p, _f64p, _f64p,
+                                         ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, ctypes.POINTER(Stats)]),
+    "pfgpu_solve_batch_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                                ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.POINTER(Stats)]),
+    "pfgpu_dev_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "pfgpu_dev_free": (ctypes.c_int, [_vp, _vp]),
+    "pfgpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "pfgpu_memcpy_d2h": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+}
+
+_lib = None
+
+
+class PfgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pfgpu error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpfgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m lightsim2grid_amd.build` "
+                              "(the engine has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PfgpuError(rc, lib().pfgpu_last_error().decode(errors="replace"))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_i32p)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_f64p)
+
+
+def _c(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a, a.ctypes.data_as(_f64p)
+
+
+class Context:
+    """One GPU + one HIP stream + an HBM budget (pfgpu_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._L = lib()
+        h = _vp()
+        _check(self._L.pfgpu_ctx_create(int(device), int(hbm_budget), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pfgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        cu = ctypes.c_int()
+        lds = ctypes.c_int()
+        fr = ctypes.c_size_t()
+        _check(self._L.pfgpu_ctx_device_info(self.h, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(fr)))
+        return {"n_cu": cu.value, "lds_bytes": lds.value, "free_bytes": fr.value}
+
+    # device buffers -----------------------------------------------------
+    def to_device(self, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = DeviceArray(self, arr.nbytes, arr.dtype, arr.shape)
+        _check(self._L.pfgpu_memcpy_h2d(self.h, d.ptr, arr.ctypes.data_as(_vp), arr.nbytes))
+        return d
+
+    def empty(self, shape, dtype) -> "DeviceArray":
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        return DeviceArray(self, nbytes, dt, shape)
+
+
+class DeviceArray:
+    def __init__(self, ctx: Context, nbytes: int, dtype, shape):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
+        p = _vp()
+        _check(ctx._L.pfgpu_dev_alloc(ctx.h, max(int(nbytes), 1), ctypes.byref(p)))
+        self.ptr = p
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(self.ctx._L.pfgpu_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        _check(self.ctx._L.pfgpu_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data_as(_vp), arr.nbytes))
+
+    def free(self):
+        if self.ptr:
+            self.ctx._L.pfgpu_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h is not None:
+                self.free()
+        except Exception:
+            pass
+
+
+def _csc(Ybus):
+    Y = sp.csc_matrix(Ybus)
+    Y.sum_duplicates()
+    Y.sort_indices()
+    return Y
+
+
+class Grid:
+    """A loaded grid (pfgpu_grid): J pattern, ordering, symbolic LU and schedule on the device."""
+
+    def __init__(self, ctx: Context, Ybus, slack_ids, slack_weights, pv, pq, single_slack: bool = False):
+        self.ctx = ctx
+        Y = _csc(Ybus)
+        self.n = Y.shape[0]
+        self.Yp, pYp = _i(Y.indptr)
+        self.Yi, pYi = _i(Y.indices)
+        sl, psl = _i(np.atleast_1d(slack_ids))
+        w, pw = _d(slack_weights)
+        pv_, ppv = _i(pv)
+        pq_, ppq = _i(pq)
+        h = _vp()
+        _check(ctx._L.pfgpu_grid_load(ctx.h, self.n, pYp, pYi, psl, len(sl), pw, ppv, len(pv_), ppq, len(pq_),
+                                      1 if single_slack else 0, ctypes.byref(h)))
+        self.h = h
+        self.single_slack = single_slack
+        info = GridInfo()
+        _check(ctx._L.pfgpu_grid_get_info(self.h, ctypes.byref(info)))
+        self.info = info.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._L.pfgpu_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def J_pattern(self):
+        N, nnz = self.info["n_jac"], self.info["nnz_jac"]
+        Jp = np.zeros(N + 1, np.int32)
+        Ji = np.zeros(nnz, np.int32)
+        _check(self.ctx._L.pfgpu_grid_get_J_pattern(self.h, Jp.ctypes.data_as(_i32p), Ji.ctypes.data_as(_i32p)))
+        return Jp, Ji
+
+    def ybus_values(self, Ybus) -> np.ndarray:
+        """Values of Ybus in this grid's CSC order (pattern must match)."""
+        Y = _csc(Ybus)
+        if not (np.array_equal(Y.indptr, self.Yp) and np.array_equal(Y.indices, self.Yi)):
+            raise ValueError("Ybus pattern differs from the loaded grid")
+        return np.ascontiguousarray(Y.data, np.complex128)
+
+    def solve(self, Yx, Sbus, V0, max_iter=10, tol=1e-8, precision=FP64, deltas=None, instances_per_wg=0,
+              want_J=False):
+        """Batched solve with host arrays.
+
+        Yx: (nnz,) shared values or (ninst, nnz) per instance; Sbus: (n,) or (ninst, n);
+        V0: (n,) or (ninst, n); deltas: optional (dptr, dpos, dval) Ybus edits per instance.
+        Returns dict(V, iters, status, stats[, Jx])."""
+        Yx = np.ascontiguousarray(Yx, np.complex128)
+        Sbus = np.ascontiguousarray(Sbus, np.complex128)
+        V0 = np.ascontiguousarray(V0, np.complex128)
+        ninst = 1
+        for arr in (Yx if Yx.ndim == 2 else None, Sbus if Sbus.ndim == 2 else None, V0 if V0.ndim == 2 else None):
+            if arr is not None:
+                ninst = max(ninst, arr.shape[0])
+        if deltas is not None:
+            ninst = max(ninst, len(deltas[0]) - 1)
+        n = self.n
+        V = np.empty((ninst, n), np.complex128)
+        if V0.ndim == 2:
+            V[:] = V0
+            v0s = n
+        else:
+            V[0] = V0
+            v0s = 0
+        iters = np.zeros(ninst, np.int32)
+        status = np.zeros(ninst, np.int32)
+        Jx = np.zeros(self.info["nnz_jac"]) if want_J else None
+        st = Stats()
+        dp = dq = dv = None
+        if deltas is not None:
+            dp, _ = _i(deltas[0])
+            dq, _ = _i(deltas[1])
+            dv, _ = _c(deltas[2])
+        _check(self.ctx._L.pfgpu_solve_batch(
+            self.ctx.h, self.h, ninst, Yx.ctypes.data_as(_f64p), self.info["nnz_ybus"] if Yx.ndim == 2 else 0,
+            dp.ctypes.data_as(_i32p) if dp is not None else None, dq.ctypes.data_as(_i32p) if dq is not None else None,
+            dv.ctypes.data_as(_f64p) if dv is not None else None, Sbus.ctypes.data_as(_f64p),
+            n if Sbus.ndim == 2 else 0, V.ctypes.data_as(_f64p), v0s, int(max_iter), float(tol), int(precision),
+            int(instances_per_wg), iters.ctypes.data_as(_i32p), status.ctypes.data_as(_i32p),
+            Jx.ctypes.data_as(_f64p) if Jx is not None else None, ctypes.byref(st)))
+        out = {"V": V, "iters": iters, "status": status, "stats": st.as_dict()}
+        if want_J:
+            Jp, Ji = self.J_pattern()
+            N = self.info["n_jac"]
+            out["J"] = sp.csc_matrix((Jx, Ji, Jp), shape=(N, N))
+        return out
+
+    def solve_device(self, d_Yx, yx_stride, d_S, s_stride, d_V0, v0_stride, d_V, ninst, d_iters, d_status,
+                     max_iter=10, tol=1e-8, precision=FP64, deltas=None, instances_per_wg=0):
+        """Batched solve on device-resident arrays (DeviceArray); returns the stats dict."""
+        st = Stats()
+        dp, dq, dv = deltas if deltas is not None else (None, None, None)
+        _check(self.ctx._L.pfgpu_solve_batch_device(
+            self.ctx.h, self.h, int(ninst), d_Yx.ptr, int(yx_stride), dp.ptr if dp else None, dq.ptr if dq else None,
+            dv.ptr if dv else None, d_S.ptr, int(s_stride), d_V0.ptr, int(v0_stride), d_V.ptr, int(max_iter),
+            float(tol), int(precision), int(instances_per_wg), d_iters.ptr, d_status.ptr, ctypes.byref(st)))
+        return st.as_dict()

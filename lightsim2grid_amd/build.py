@@ -1,0 +1,67 @@
+"""In-tree build of the native parts (hipcc for gfx950, g++ for host-only code).
+
+Outputs (git-ignored, shipped with the tree):
+  lightsim2grid_amd/libpfgpu.so            C-ABI library: HIP kernels + host analysis
+  lightsim2grid_amd/lightsim2grid_amd_cpp*.so  pybind11 module mirroring lightsim2grid_cpp
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpfgpu.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PFGPU_ARCH", "gfx950")
+
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             # packed FP32 ops stay off (hipcc's host pass warns that it does not know the feature)
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, **kw):
+    print("[build]", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, **kw)
+
+
+def build_lib() -> str:
+    srcs = [os.path.join(CSRC, "pfgpu.hip"), os.path.join(CSRC, "analysis.cpp")]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs, "-Wl,-soname,libpfgpu.so"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def module_path() -> str:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "lightsim2grid_amd_cpp" + suffix)
+
+
+def build_module() -> str:
+    import pybind11
+    src = os.path.join(CSRC, "module.cpp")
+    if not os.path.exists(src):
+        return ""
+    out = module_path()
+    inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{ROOT}/include", f"-I{CSRC}"]
+    srcs = [src] + [os.path.join(CSRC, f) for f in ("gridmodel.cpp",) if os.path.exists(os.path.join(CSRC, f))]
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", *inc, *srcs, "-o", tmp,
+          f"-L{HERE}", "-lpfgpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_lib()
+    build_module()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,512 @@
+// Host-side analysis for the batched GPU Newton-Raphson engine (see analysis.hpp).
+#include "analysis.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <queue>
+#include <sstream>
+#include <stdexcept>
+
+namespace pfgpu {
+
+// ---------------------------------------------------------------------------
+// Jacobian pattern + scatter map.
+// Layout follows the reference exactly so that get_J() matches it:
+//   distributed slack (BaseNRAlgo.tpp:396-555): row/col 0 = reference-slack
+//   active-power equation / slack_absorbed unknown; columns 1.. = Va of pvpq
+//   then Vm of pq; rows 1.. = P of pvpq then Q of pq. Column 0 holds the
+//   slack weights (w[slack0] at row 0, w[pvpq[k]] at row k+1 when |w|>1e-7).
+//   single slack (BaseNRSingleSlackAlgo.tpp:247-...): same without row/col 0.
+// The value of J entry t is Re/Im of dS_dVa or dS_dVm at Ybus entry src[t]
+// (BaseNRAlgo.tpp:562-635: J11 = dS_dVa.real, J21 = dS_dVa.imag,
+// J12 = dS_dVm.real, J22 = dS_dVm.imag, slack row = real parts).
+// ---------------------------------------------------------------------------
+JPattern build_jacobian_pattern(const YbusPattern& Y, const int* slack_ids, int nslack, const double* slack_w,
+                                const int* pv, int npv, const int* pq, int npq, bool single_slack) {
+    if (nslack < 1) throw std::runtime_error("pfgpu: at least one slack bus is required");
+    JPattern J;
+    const int n = Y.n;
+    J.n_bus = n;
+    J.off = single_slack ? 0 : 1;
+    J.slack0 = slack_ids[0];
+    if (!single_slack)
+        for (int s = 1; s < nslack; ++s) J.my_pv.push_back(slack_ids[s]);  // BaseAlgo.hpp:201-216
+    J.my_pv.insert(J.my_pv.end(), pv, pv + npv);
+    J.pq.assign(pq, pq + npq);
+    J.pvpq = J.my_pv;
+    J.pvpq.insert(J.pvpq.end(), pq, pq + npq);
+    const int npvpq = (int)J.pvpq.size();
+    std::vector<int> pvpq_inv(n, -1), pq_inv(n, -1);
+    for (int k = 0; k < npvpq; ++k) {
+        const int b = J.pvpq[k];
+        if (b < 0 || b >= n) throw std::runtime_error("pfgpu: pv/pq bus index out of range");
+        if (pvpq_inv[b] >= 0) throw std::runtime_error("pfgpu: a bus is listed twice in pv/pq/slack");
+        pvpq_inv[b] = k;
+    }
+    for (int k = 0; k < npq; ++k) pq_inv[J.pq[k]] = k;
+    if (J.slack0 < 0 || J.slack0 >= n) throw std::runtime_error("pfgpu: slack bus index out of range");
+    if (pvpq_inv[J.slack0] >= 0) throw std::runtime_error("pfgpu: the reference slack bus is also pv/pq");
+    const int off = J.off;
+    J.N = npvpq + npq + off;
+
+    struct E { int r; int src; uint8_t sel; double c; };
+    std::vector<std::vector<E>> cols(J.N);
+    auto add_bus_column = [&](int jc, int bus, uint8_t sel_re, uint8_t sel_im) {
+        for (int q = Y.p[bus]; q < Y.p[bus + 1]; ++q) {
+            const int r = Y.i[q];
+            if (pvpq_inv[r] >= 0) cols[jc].push_back({pvpq_inv[r] + off, q, sel_re, 0.0});
+            if (pq_inv[r] >= 0) cols[jc].push_back({pq_inv[r] + npvpq + off, q, sel_im, 0.0});
+            if (off && r == J.slack0) cols[jc].push_back({0, q, sel_re, 0.0});
+        }
+    };
+    for (int k = 0; k < npvpq; ++k) add_bus_column(k + off, J.pvpq[k], SEL_DVA_RE, SEL_DVA_IM);
+    for (int k = 0; k < npq; ++k) add_bus_column(k + npvpq + off, J.pq[k], SEL_DVM_RE, SEL_DVM_IM);
+    if (off) {
+        cols[0].push_back({0, -1, SEL_CONST, slack_w[J.slack0]});
+        for (int k = 0; k < npvpq; ++k) {
+            const double w = slack_w[J.pvpq[k]];
+            if (std::fabs(w) > 1e-7) cols[0].push_back({k + 1, -1, SEL_CONST, w});  // _tol_equal_float
+        }
+    }
+    J.Jp.assign(J.N + 1, 0);
+    for (int c = 0; c < J.N; ++c) {
+        auto& v = cols[c];
+        std::sort(v.begin(), v.end(), [](const E& a, const E& b) { return a.r < b.r; });
+        for (size_t t = 1; t < v.size(); ++t)
+            if (v[t].r == v[t - 1].r) throw std::runtime_error("pfgpu: duplicate Ybus entry");
+        J.Jp[c + 1] = J.Jp[c] + (int)v.size();
+        for (const E& e : v) {
+            J.Ji.push_back(e.r);
+            J.src.push_back(e.src);
+            J.sel.push_back(e.sel);
+            J.cval.push_back(e.c);
+        }
+    }
+    return J;
+}
+
+// ---------------------------------------------------------------------------
+// Ordering: minimum degree on the bus graph. Each bus with unknowns is one
+// block (Va, then Vm for pq buses; the slack_absorbed unknown rides with the
+// reference slack bus). The degree of a block is the number of unknowns in its
+// neighbouring blocks of the current elimination graph; ties go to the lowest
+// bus id. Blocks are expanded to unknowns in elimination order, so the factor
+// keeps the 2x2 bus structure of the power-flow Jacobian.
+// ---------------------------------------------------------------------------
+static std::vector<int> bus_block_order(const JPattern& J) {
+    const int N = J.N, n = J.n_bus, off = J.off;
+    const int npvpq = (int)J.pvpq.size(), npq = (int)J.pq.size();
+    std::vector<int> blk(N);
+    if (off) blk[0] = J.slack0;
+    for (int k = 0; k < npvpq; ++k) blk[k + off] = J.pvpq[k];
+    for (int k = 0; k < npq; ++k) blk[k + npvpq + off] = J.pq[k];
+    std::vector<std::vector<int>> members(n);
+    for (int u = 0; u < N; ++u) members[blk[u]].push_back(u);  // ascending J index: Va before Vm
+    std::vector<int> w(n, 0);
+    for (int b = 0; b < n; ++b) w[b] = (int)members[b].size();
+    std::vector<std::vector<int>> adj(n);
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            const int a = blk[J.Ji[q]], b = blk[c];
+            if (a == b) continue;
+            adj[a].push_back(b);
+            adj[b].push_back(a);
+        }
+    for (auto& a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<long> deg(n, 0);
+    using QE = std::pair<long, int>;
+    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+    for (int b = 0; b < n; ++b) {
+        if (!w[b]) continue;
+        for (int u : adj[b]) deg[b] += w[u];
+        pq.push({deg[b], b});
+    }
+    std::vector<char> done(n, 0);
+    std::vector<int> order;
+    std::vector<int> merged;
+    while (!pq.empty()) {
+        auto [d, b] = pq.top();
+        pq.pop();
+        if (done[b] || d != deg[b]) continue;  // stale entry
+        done[b] = 1;
+        for (int u : members[b]) order.push_back(u);
+        const std::vector<int> nb = adj[b];
+        for (int u : nb) {
+            merged.clear();
+            std::set_union(adj[u].begin(), adj[u].end(), nb.begin(), nb.end(), std::back_inserter(merged));
+            auto& au = adj[u];
+            au.clear();
+            long du = 0;
+            for (int x : merged)
+                if (x != u && x != b) { au.push_back(x); du += w[x]; }
+            if (du != deg[u]) { deg[u] = du; pq.push({du, u}); }
+        }
+        adj[b].clear();
+        adj[b].shrink_to_fit();
+    }
+    if ((int)order.size() != N) throw std::runtime_error("pfgpu: ordering lost unknowns");
+    return order;
+}
+
+static inline int find_sorted(const std::vector<int>& v, int b, int e, int x) {
+    auto it = std::lower_bound(v.begin() + b, v.begin() + e, x);
+    if (it == v.begin() + e || *it != x) return -1;
+    return (int)(it - v.begin());
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic LU on the symmetrised pattern with static diagonal pivots:
+// S_k = struct(A(k+1:,k)) U  union over etree children c of (S_c \ {k}),
+// parent(k) = min S_k (George & Liu's symbolic Cholesky on A + A^T).
+// ---------------------------------------------------------------------------
+SymbolicLU symbolic_lu(const JPattern& J) {
+    SymbolicLU S;
+    const int N = J.N;
+    S.N = N;
+    S.perm = bus_block_order(J);
+    S.pinv.assign(N, -1);
+    for (int k = 0; k < N; ++k) S.pinv[S.perm[k]] = k;
+    std::vector<std::vector<int>> low(N);
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            const int a = S.pinv[J.Ji[q]], b = S.pinv[c];
+            if (a == b) continue;
+            low[std::min(a, b)].push_back(std::max(a, b));
+        }
+    std::vector<std::vector<int>> children(N);
+    S.parent.assign(N, -1);
+    S.cp.assign(N + 1, 0);
+    std::vector<std::vector<int>> arms(N);
+    std::vector<int> mark(N, -1);
+    for (int k = 0; k < N; ++k) {
+        std::vector<int>& s = arms[k];
+        mark[k] = k;
+        for (int r : low[k]) if (mark[r] != k) { mark[r] = k; s.push_back(r); }
+        for (int c : children[k])
+            for (int r : arms[c]) if (mark[r] != k) { mark[r] = k; s.push_back(r); }
+        std::sort(s.begin(), s.end());
+        if (!s.empty()) { S.parent[k] = s[0]; children[s[0]].push_back(k); }
+        S.cp[k + 1] = S.cp[k] + (int)s.size();
+    }
+    S.ci.reserve(S.cp[N]);
+    for (int k = 0; k < N; ++k) S.ci.insert(S.ci.end(), arms[k].begin(), arms[k].end());
+    S.nnzL = S.cp[N];
+    S.off_L = N;
+    S.off_U = S.off_L + S.nnzL;
+    S.off_y = S.off_U + S.nnzL;
+    S.nvals = S.off_y + N;
+    // J entry -> value slot
+    S.jslot.resize(J.Ji.size());
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            const int a = S.pinv[J.Ji[q]], b = S.pinv[c];
+            int slot;
+            if (a == b) slot = a;
+            else if (a > b) slot = (int)S.off_L + find_sorted(S.ci, S.cp[b], S.cp[b + 1], a);
+            else slot = (int)S.off_U + find_sorted(S.ci, S.cp[a], S.cp[a + 1], b);
+            S.jslot[q] = slot;
+        }
+    double fl = 0;
+    for (int k = 0; k < N; ++k) {
+        const double c = S.cp[k + 1] - S.cp[k];
+        fl += 2.0 * c * c + c + 1;  // rank-1 update of the arm block, scaling, reciprocal
+    }
+    S.flops = fl;
+    return S;
+}
+
+// ---------------------------------------------------------------------------
+// Schedule. Value slot of the target (a, b) of a pair through pivot k:
+//   a == b -> diag a ; a > b -> L(a, b) in arm b ; a < b -> U(a, b) in arm a.
+// Tasks are entries; a task's level is 1 + the max level of what it reads.
+// ---------------------------------------------------------------------------
+Schedule build_schedule(const SymbolicLU& S) {
+    Schedule sch;
+    const int N = S.N;
+    const int64_t nv = S.nvals;
+    auto slot_of = [&](int a, int b) -> int {
+        if (a == b) return a;
+        if (a > b) return (int)S.off_L + find_sorted(S.ci, S.cp[b], S.cp[b + 1], a);
+        return (int)S.off_U + find_sorted(S.ci, S.cp[a], S.cp[a + 1], b);
+    };
+    // count pairs per target
+    std::vector<int> cnt(nv + 1, 0);
+    for (int k = 0; k < N; ++k) {
+        const int b0 = S.cp[k], b1 = S.cp[k + 1];
+        for (int qa = b0; qa < b1; ++qa) {
+            const int a = S.ci[qa];
+            for (int qb = b0; qb < b1; ++qb) cnt[slot_of(a, S.ci[qb])]++;
+            cnt[S.off_y + a]++;
+        }
+    }
+    std::vector<int> start(nv + 1, 0);
+    for (int64_t t = 0; t < nv; ++t) start[t + 1] = start[t] + cnt[t];
+    const int64_t npairs = start[nv];
+    std::vector<int> pl(npairs), pu(npairs), pd(npairs);
+    std::vector<int> fillp(start.begin(), start.end() - 1);
+    for (int k = 0; k < N; ++k) {
+        const int b0 = S.cp[k], b1 = S.cp[k + 1];
+        for (int qa = b0; qa < b1; ++qa) {
+            const int a = S.ci[qa];
+            const int lslot = (int)S.off_L + qa;
+            for (int qb = b0; qb < b1; ++qb) {
+                const int t = slot_of(a, S.ci[qb]);
+                const int p = fillp[t]++;
+                pl[p] = lslot; pu[p] = (int)S.off_U + qb; pd[p] = k;
+            }
+            const int t = (int)S.off_y + a;
+            const int p = fillp[t]++;
+            pl[p] = lslot; pu[p] = (int)S.off_y + k; pd[p] = k;
+        }
+    }
+    // levels in a topological order: column by column, then y of that column
+    std::vector<int> level(nv, 0);
+    auto eval_level = [&](int t) {
+        int lv = 0;
+        for (int p = start[t]; p < start[t + 1]; ++p)
+            lv = std::max({lv, level[pl[p]] + 1, level[pu[p]] + 1, level[pd[p]] + 1});
+        level[t] = lv;
+    };
+    for (int j = 0; j < N; ++j) {
+        eval_level(j);
+        for (int q = S.cp[j]; q < S.cp[j + 1]; ++q) {
+            eval_level((int)S.off_L + q);
+            eval_level((int)S.off_U + q);
+        }
+        eval_level((int)S.off_y + j);
+    }
+    int nlev = 0;
+    for (int64_t t = 0; t < nv; ++t) nlev = std::max(nlev, level[t] + 1);
+    // bucket tasks by level (stable: slot order inside a level)
+    std::vector<int> lc(nlev + 1, 0);
+    for (int64_t t = 0; t < nv; ++t) lc[level[t] + 1]++;
+    for (int l = 0; l < nlev; ++l) lc[l + 1] += lc[l];
+    sch.lvl_ptr = lc;
+    std::vector<int> order(nv);
+    {
+        std::vector<int> f(lc.begin(), lc.end() - 1);
+        for (int64_t t = 0; t < nv; ++t) order[f[level[t]]++] = (int)t;
+    }
+    sch.task_out.resize(nv);
+    sch.task_pbeg.resize(nv);
+    sch.task_pend.resize(nv);
+    sch.task_diag.resize(nv);
+    sch.pair_l.resize(npairs);
+    sch.pair_u.resize(npairs);
+    sch.pair_d.resize(npairs);
+    int64_t pp = 0;
+    for (int64_t i = 0; i < nv; ++i) {
+        const int t = order[i];
+        sch.task_out[i] = t;
+        sch.task_diag[i] = (t < N) ? 1 : 0;
+        sch.task_pbeg[i] = (int)pp;
+        for (int p = start[t]; p < start[t + 1]; ++p, ++pp) {
+            sch.pair_l[pp] = pl[p];
+            sch.pair_u[pp] = pu[p];
+            sch.pair_d[pp] = pd[p];
+        }
+        sch.task_pend[i] = (int)pp;
+        sch.max_pairs_per_task = std::max(sch.max_pairs_per_task, start[t + 1] - start[t]);
+    }
+    sch.npairs = npairs;
+    // backward levels
+    std::vector<int> bl(N, 0);
+    int nb = 0;
+    for (int j = N - 1; j >= 0; --j) {
+        int lv = 0;
+        for (int q = S.cp[j]; q < S.cp[j + 1]; ++q) lv = std::max(lv, bl[S.ci[q]] + 1);
+        bl[j] = lv;
+        nb = std::max(nb, lv + 1);
+    }
+    std::vector<int> bc(nb + 1, 0);
+    for (int j = 0; j < N; ++j) bc[bl[j] + 1]++;
+    for (int l = 0; l < nb; ++l) bc[l + 1] += bc[l];
+    sch.blvl_ptr = bc;
+    sch.bcol.resize(N);
+    {
+        std::vector<int> f(bc.begin(), bc.end() - 1);
+        for (int j = 0; j < N; ++j) sch.bcol[f[bl[j]]++] = j;
+    }
+    return sch;
+}
+
+std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch) {
+    std::ostringstream os;
+    const int nlev = (int)sch.lvl_ptr.size() - 1;
+    int maxw = 0;
+    for (int l = 0; l < nlev; ++l) maxw = std::max(maxw, sch.lvl_ptr[l + 1] - sch.lvl_ptr[l]);
+    int etree_h = 0;
+    {
+        std::vector<int> h(S.N, 0);
+        for (int k = 0; k < S.N; ++k)
+            if (S.parent[k] >= 0) h[S.parent[k]] = std::max(h[S.parent[k]], h[k] + 1);
+        for (int k = 0; k < S.N; ++k) etree_h = std::max(etree_h, h[k] + 1);
+    }
+    os << "N=" << J.N << " nnzJ=" << J.Ji.size() << " nnzL=" << S.nnzL << " nnzLU=" << (S.N + 2 * S.nnzL)
+       << " slots=" << S.nvals << " flops=" << S.flops << " etree_h=" << etree_h << " levels=" << nlev
+       << " max_level_width=" << maxw << " pairs=" << sch.npairs << " max_pairs=" << sch.max_pairs_per_task
+       << " back_levels=" << (sch.blvl_ptr.size() - 1);
+    return os.str();
+}
+
+}  // namespace pfgpu
+
+namespace pfgpu {
+
+DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const SymbolicLU& S, const Schedule& sch) {
+    DevicePlan P;
+    const int n = Y.n, N = J.N;
+    P.n = n; P.N = N; P.off = J.off; P.nnzY = Y.p[n]; P.nJ = (int)J.Ji.size(); P.slack0 = J.slack0;
+    P.nvals = (int)S.nvals;
+    P.flops = S.flops;
+    P.nnzLU = S.N + 2 * S.nnzL;
+    const int nlev = (int)sch.lvl_ptr.size() - 1;
+    const int nblev = (int)sch.blvl_ptr.size() - 1;
+    P.nlev = nlev; P.nblev = nblev;
+
+    // --- slot hotness and renumbering ----------------------------------
+    std::vector<int> hot(S.nvals, -1);
+    for (int l = 0; l < nlev; ++l)
+        for (int t = sch.lvl_ptr[l]; t < sch.lvl_ptr[l + 1]; ++t) {
+            hot[sch.task_out[t]] = std::max(hot[sch.task_out[t]], l);
+            for (int p = sch.task_pbeg[t]; p < sch.task_pend[t]; ++p) {
+                hot[sch.pair_l[p]] = std::max(hot[sch.pair_l[p]], l);
+                hot[sch.pair_u[p]] = std::max(hot[sch.pair_u[p]], l);
+                hot[sch.pair_d[p]] = std::max(hot[sch.pair_d[p]], l);
+            }
+        }
+    // backward reads U, x, 1/d; make it count as later than every factor level
+    for (int l = 0; l < nblev; ++l)
+        for (int q = sch.blvl_ptr[l]; q < sch.blvl_ptr[l + 1]; ++q) {
+            const int j = sch.bcol[q];
+            const int lv = nlev + l;
+            hot[S.off_y + j] = std::max(hot[S.off_y + j], lv);
+            hot[j] = std::max(hot[j], lv);
+            for (int a = S.cp[j]; a < S.cp[j + 1]; ++a) hot[S.off_U + a] = std::max(hot[S.off_U + a], lv);
+        }
+    std::vector<int> ord(S.nvals);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hot[a] > hot[b]; });
+    std::vector<int> nw(S.nvals);
+    for (int i = 0; i < (int)S.nvals; ++i) nw[ord[i]] = i;
+    auto ys = [&](int k) { return nw[S.off_y + k]; };  // slot of y/x for permuted index k
+
+    // --- Ybus row view ----------------------------------------------------
+    P.yr_ptr.assign(n + 1, 0);
+    for (int c = 0; c < n; ++c)
+        for (int q = Y.p[c]; q < Y.p[c + 1]; ++q) P.yr_ptr[Y.i[q] + 1]++;
+    for (int r = 0; r < n; ++r) P.yr_ptr[r + 1] += P.yr_ptr[r];
+    P.yr_col.resize(P.nnzY); P.yr_pos.resize(P.nnzY);
+    {
+        std::vector<int> f(P.yr_ptr.begin(), P.yr_ptr.end() - 1);
+        for (int c = 0; c < n; ++c)
+            for (int q = Y.p[c]; q < Y.p[c + 1]; ++q) {
+                const int k = f[Y.i[q]]++;
+                P.yr_col[k] = c; P.yr_pos[k] = q;
+            }
+    }
+    // --- F rows and unknowns --------------------------------------------
+    const int npvpq = (int)J.pvpq.size(), npq = (int)J.pq.size(), off = J.off;
+    P.fslot_p.assign(n, -1); P.fslot_q.assign(n, -1);
+    if (off) P.fslot_p[J.slack0] = ys(S.pinv[0]);
+    for (int k = 0; k < npvpq; ++k) P.fslot_p[J.pvpq[k]] = ys(S.pinv[k + off]);
+    for (int k = 0; k < npq; ++k) P.fslot_q[J.pq[k]] = ys(S.pinv[k + npvpq + off]);
+    P.ux_bus.resize(N); P.ux_kind.resize(N); P.ux_slot.resize(N);
+    if (off) { P.ux_bus[0] = J.slack0; P.ux_kind[0] = 0; P.ux_slot[0] = ys(S.pinv[0]); }
+    for (int k = 0; k < npvpq; ++k) { P.ux_bus[k + off] = J.pvpq[k]; P.ux_kind[k + off] = 1; P.ux_slot[k + off] = ys(S.pinv[k + off]); }
+    for (int k = 0; k < npq; ++k) {
+        const int j = k + npvpq + off;
+        P.ux_bus[j] = J.pq[k]; P.ux_kind[j] = 2; P.ux_slot[j] = ys(S.pinv[j]);
+    }
+    // --- J entries ----------------------------------------------------------
+    // bus of a J row / column: rows/cols 'off..off+npvpq' -> pvpq, then pq, 0 -> slack0
+    auto jbus = [&](int jr) -> int {
+        if (off && jr == 0) return J.slack0;
+        const int k = jr - off;
+        return k < npvpq ? J.pvpq[k] : J.pq[k - npvpq];
+    };
+    P.j_ypos.resize(P.nJ); P.j_rbus.resize(P.nJ); P.j_cbus.resize(P.nJ); P.j_sel.resize(P.nJ); P.j_cval.resize(P.nJ);
+    std::vector<int> slot_jsrc(S.nvals, -1);
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            P.j_ypos[q] = J.src[q];
+            P.j_sel[q] = J.sel[q];
+            P.j_cval[q] = J.cval[q];
+            if (J.src[q] >= 0) {
+                // Ybus entry (r, c) in CSC: find its row bus
+                P.j_rbus[q] = Y.i[J.src[q]];
+                int cb = 0;
+                // column bus of the Ybus entry: the bus whose CSC column holds src
+                cb = (int)(std::upper_bound(Y.p.begin(), Y.p.end(), J.src[q]) - Y.p.begin()) - 1;
+                P.j_cbus[q] = cb;
+            } else {
+                P.j_rbus[q] = jbus(J.Ji[q]);
+                P.j_cbus[q] = jbus(c);
+            }
+            slot_jsrc[S.jslot[q]] = q;
+        }
+    // --- factor tasks ----------------------------------------------------------
+    P.f_lvl = sch.lvl_ptr;
+    P.f_maxp.assign(nlev, 0);
+    const int ntask = sch.lvl_ptr[nlev];
+    P.t_out.resize(ntask); P.t_pbeg.resize(ntask + 1); P.t_jsrc.resize(ntask); P.t_kind.resize(ntask);
+    P.p_l.resize(sch.npairs); P.p_u.resize(sch.npairs); P.p_d.resize(sch.npairs);
+    for (int l = 0; l < nlev; ++l)
+        for (int t = sch.lvl_ptr[l]; t < sch.lvl_ptr[l + 1]; ++t) {
+            P.t_out[t] = nw[sch.task_out[t]];
+            P.t_pbeg[t] = sch.task_pbeg[t];
+            P.t_jsrc[t] = slot_jsrc[sch.task_out[t]];
+            P.t_kind[t] = sch.task_diag[t] ? 1 : (sch.task_out[t] >= S.off_y ? 2 : 0);
+            P.f_maxp[l] = std::max(P.f_maxp[l], sch.task_pend[t] - sch.task_pbeg[t]);
+        }
+    P.t_pbeg[ntask] = (int)sch.npairs;
+    for (int64_t p = 0; p < sch.npairs; ++p) {
+        P.p_l[p] = nw[sch.pair_l[p]];
+        P.p_u[p] = nw[sch.pair_u[p]];
+        P.p_d[p] = nw[sch.pair_d[p]];
+    }
+    // --- backward tasks ---------------------------------------------------------
+    P.b_lvl = sch.blvl_ptr;
+    P.b_maxp.assign(nblev, 0);
+    P.bt_out.resize(N); P.bt_pbeg.resize(N + 1); P.bt_diag.resize(N);
+    int bp = 0;
+    for (int l = 0; l < nblev; ++l)
+        for (int q = sch.blvl_ptr[l]; q < sch.blvl_ptr[l + 1]; ++q) {
+            const int j = sch.bcol[q];
+            P.bt_out[q] = ys(j);
+            P.bt_diag[q] = nw[j];
+            P.bt_pbeg[q] = bp;
+            for (int a = S.cp[j]; a < S.cp[j + 1]; ++a) {
+                P.bp_u.push_back(nw[S.off_U + a]);
+                P.bp_x.push_back(ys(S.ci[a]));
+                ++bp;
+            }
+            P.b_maxp[l] = std::max(P.b_maxp[l], S.cp[j + 1] - S.cp[j]);
+        }
+    P.bt_pbeg[N] = bp;
+    return P;
+}
+
+std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<int>& maxp, int G, int nslot) {
+    const int nlev = (int)lvl_ptr.size() - 1;
+    std::vector<int> team(nlev, 1);
+    const int tmax = std::max(1, 64 / G);
+    for (int l = 0; l < nlev; ++l) {
+        const int ntask = lvl_ptr[l + 1] - lvl_ptr[l];
+        const int mp = std::max(1, maxp[l]);
+        double best = 1e300;
+        for (int t = 1; t <= tmax && t <= nslot; t *= 2) {
+            const int conc = std::max(1, nslot / t);                 // tasks in flight
+            const double rounds = std::ceil((double)ntask / conc);
+            const double cost = rounds * (std::ceil((double)mp / t) * 3.0 + 2.0 * std::log2((double)t) + 4.0);
+            if (cost < best - 1e-9) { best = cost; team[l] = t; }
+        }
+    }
+    return team;
+}
+
+}  // namespace pfgpu

@@ -1,0 +1,134 @@
+// Host-side analysis for the batched GPU Newton-Raphson engine.
+//
+// Everything here is computed once per (grid pattern, slack/pv/pq set) and is
+// shared by every instance of a batch:
+//   * the Jacobian pattern in the reference's layout and the scatter map from
+//     each J entry to a Ybus entry + {dVa,dVm}x{Re,Im} selector
+//     (reference: src/powerflow_algorithm/BaseNRAlgo.tpp:396-635, single slack
+//     BaseNRSingleSlackAlgo.tpp:247-...);
+//   * a fill-reducing symmetric ordering of J (minimum degree on the bus graph,
+//     each bus carrying its 1 or 2 unknowns as one block);
+//   * the symbolic LU with static diagonal pivots on the symmetric pattern
+//     (this plays the role of klu_analyze, src/linear_solvers/KLUSolver.cpp:31);
+//   * the level schedule that the HIP kernels execute: Crout-style dot-product
+//     tasks for the numeric refactorisation fused with the forward solve, and
+//     the backward-solve levels.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pfgpu {
+
+using cplx = std::complex<double>;
+
+// Jacobian entry sources: J value = Re/Im of dS_dVa or dS_dVm at a Ybus entry.
+enum JSel : uint8_t { SEL_DVA_RE = 0, SEL_DVA_IM = 1, SEL_DVM_RE = 2, SEL_DVM_IM = 3, SEL_CONST = 4 };
+
+struct JPattern {
+    int n_bus = 0;
+    int N = 0;       // J dimension
+    int off = 0;     // 1 for the distributed-slack layout (row/col 0 = slack), 0 for single slack
+    int slack0 = 0;  // reference slack bus
+    std::vector<int> my_pv, pq, pvpq;
+    std::vector<int> Jp, Ji;     // CSC, rows sorted (reference layout)
+    std::vector<int> src;        // Ybus CSC value index per J entry (-1 for constants)
+    std::vector<uint8_t> sel;    // JSel per J entry
+    std::vector<double> cval;    // constant value per J entry (slack-weight column)
+};
+
+// Ybus in CSC (rows sorted per column), as the reference takes it.
+struct YbusPattern {
+    int n = 0;
+    std::vector<int> p, i;
+};
+
+JPattern build_jacobian_pattern(const YbusPattern& Y, const int* slack_ids, int nslack, const double* slack_w,
+                                const int* pv, int npv, const int* pq, int npq, bool single_slack);
+
+// One Crout task computes one entry of the factor (or of the forward-solved
+// right-hand side) as  v = a - sum_p  X[lp] * X[up] * X[dp]  over its pair range,
+// where X is the per-instance value array, X[dp] holds 1/d_k for the pivot k of
+// the pair (L is kept unscaled). Diagonal tasks store 1/v instead of v.
+struct SymbolicLU {
+    int N = 0;
+    std::vector<int> perm;      // perm[new] = old J index
+    std::vector<int> pinv;      // pinv[old] = new
+    std::vector<int> parent;    // elimination tree (new indices), -1 = root
+    // column "arms": for new column k, sorted rows R_k (> k) of L(:,k) == cols of U(k,:)
+    std::vector<int> cp, ci;    // CSC of strictly-lower L pattern (new indices)
+    // value array layout (per instance): [ diag(0..N-1) | Lvals(nnzL) | Uvals(nnzL) | y(0..N-1) ]
+    int64_t nnzL = 0;
+    int64_t nvals = 0;          // total value slots per instance
+    int64_t off_L = 0, off_U = 0, off_y = 0;
+    // J entry t -> value slot
+    std::vector<int> jslot;
+    // flops of one refactorisation (multiply-adds counted as 2, divisions 1)
+    double flops = 0;
+};
+
+SymbolicLU symbolic_lu(const JPattern& J);
+
+struct Schedule {
+    // Factor + forward-solve tasks, grouped by level.
+    std::vector<int> lvl_ptr;          // level l covers tasks [lvl_ptr[l], lvl_ptr[l+1])
+    std::vector<int> task_out;         // value slot written
+    std::vector<int> task_pbeg;        // pair range begin (pair ranges are contiguous: end = next begin)
+    std::vector<int> task_pend;
+    std::vector<uint8_t> task_diag;    // 1 => store reciprocal
+    std::vector<int> pair_l, pair_u, pair_d;  // value slots of the unscaled L entry, the U/y entry, and 1/d
+    // Backward solve: x_j = (y_j - sum_i u_ji x_i) / d_j, levels from the root down.
+    std::vector<int> blvl_ptr;
+    std::vector<int> bcol;             // column j per backward task
+    // statistics
+    int max_pairs_per_task = 0;
+    int64_t npairs = 0;
+};
+
+Schedule build_schedule(const SymbolicLU& S);
+
+// Flattened arrays uploaded to the device. Value slots are renumbered by
+// "hotness" (the highest level at which a slot is read or written comes
+// first), so that a launch can keep the first `nlds` slots of every instance
+// in LDS and spill the rest to a per-instance global buffer.
+struct DevicePlan {
+    int n = 0, N = 0, off = 0, nnzY = 0, nJ = 0, nvals = 0, slack0 = 0;
+    int nlev = 0, nblev = 0;
+    // Ybus row view for Ibus = Y V: row r covers [yr_ptr[r], yr_ptr[r+1]) with column yr_col and CSC value pos yr_pos
+    std::vector<int> yr_ptr, yr_col, yr_pos;
+    // per bus: value slot of the P-row / Q-row of F (-1 if the bus has no such equation)
+    std::vector<int> fslot_p, fslot_q;
+    // per J unknown (original J column order): bus, kind (0 slack_absorbed, 1 Va, 2 Vm), value slot of dx
+    std::vector<int> ux_bus, ux_slot;
+    std::vector<int8_t> ux_kind;
+    // J entries (reference CSC order): Ybus CSC pos, row bus, col bus, selector, constant
+    std::vector<int> j_ypos, j_rbus, j_cbus;
+    std::vector<uint8_t> j_sel;
+    std::vector<double> j_cval;
+    // factor + forward tasks (level-major)
+    std::vector<int> f_lvl;                 // nlev + 1
+    std::vector<int> f_maxp;                // per level max pair count
+    std::vector<int> t_out, t_pbeg, t_jsrc; // per task; pair range [t_pbeg[i], t_pbeg[i+1])
+    // task kind: 0 = L/U entry (initial value: its J entry or 0 for fill),
+    // 1 = diagonal (initial value: its J entry; stores the reciprocal),
+    // 2 = forward-solve entry y_k (initial value: the F slot already in place)
+    std::vector<uint8_t> t_kind;
+    std::vector<int> p_l, p_u, p_d;         // per pair
+    // backward tasks (level-major): x slot (in/out), pair range, slot of 1/d
+    std::vector<int> b_lvl, b_maxp, bt_out, bt_pbeg, bt_diag;
+    std::vector<int> bp_u, bp_x;
+    double flops = 0;
+    int64_t nnzLU = 0;
+};
+
+DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const SymbolicLU& S, const Schedule& sch);
+
+// Per-level team sizes (lanes per task, power of two, team * G <= 64) for a
+// launch with G interleaved instances and `nslot` lanes per instance.
+std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<int>& maxp, int G, int nslot);
+
+std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch);
+
+}  // namespace pfgpu

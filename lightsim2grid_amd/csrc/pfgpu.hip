@@ -1,0 +1,963 @@
+// MI355X batched Newton-Raphson power flow: HIP kernels + the C ABI (include/pfgpu.h).
+//
+// One workgroup (256 lanes) runs the whole NR loop for G interleaved
+// instances (a "persistent" NR kernel: no host round trip per iteration).
+// Lane t serves instance g = t % G as slot s = t / G of nslot = 256 / G.
+// Per iteration, all phases run inside the kernel, separated by workgroup
+// barriers:
+//   M  mismatch   Ibus = Ybus V (row view), F = V.conj(Ibus) - Sbus (+ slack
+//                 share), scattered to the forward-solve slots, inf-norm
+//                 (BaseAlgo.cpp:56-118, :120-129)
+//   F  factor     level-scheduled Crout tasks: each entry of the static-pivot
+//                 LU (and of the forward-solved right-hand side) is
+//                 a - sum X[l]*X[u]*X[1/d]; the J value a is computed in the
+//                 task from Ybus, V and Ibus through the scatter map
+//                 (_dSbus_dV + known-pattern fill, BaseNRAlgo.tpp:202-260,
+//                 :637-688; klu_refactor, KLUSolver.cpp:47-59)
+//   B  backward   level-scheduled x_j = (y_j - sum u_ji x_i) / d_j (klu_solve)
+//   U  update     slack_absorbed, Va, Vm, V (BaseNRAlgo.tpp:138-158)
+// Per level, a team of lanes (power of two) shares each task's dot product
+// and reduces it with cross-lane shuffles.
+// The value array X of an instance lives in LDS for its first `nlds` slots
+// (the hottest, by construction of the plan) and in a global spill buffer for
+// the rest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pfgpu.h"
+#include "analysis.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxG = 32;
+
+thread_local std::string g_last_error;
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_OK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw ApiError(PFGPU_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const ApiError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return PFGPU_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PFGPU_E_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return PFGPU_E_INTERNAL;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel arguments
+// ---------------------------------------------------------------------------
+struct KArgs {
+    int n, N, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter;
+    double tol;
+    const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
+    const int8_t* ux_kind;
+    const int *j_ypos, *j_rbus, *j_cbus;
+    const uint8_t* j_sel;
+    const double* j_cval;
+    const int *f_lvl, *f_team, *t_out, *t_pbeg, *t_jsrc;
+    const uint8_t* t_kind;
+    const int *p_l, *p_u, *p_d;
+    const int *b_lvl, *b_team, *bt_out, *bt_pbeg, *bt_diag, *bp_u, *bp_x;
+    const double* w;
+    const double* Yx;
+    long long y_stride;  // doubles between instances (0 = shared)
+    const double* S;
+    long long s_stride;  // doubles between instances (0 = shared)
+    double* V;           // [ninst][n] complex
+    double* Va;          // [ninst][n]
+    double* Vm;          // [ninst][n]
+    double* Ib;          // [ninst][n] complex
+    void* Xg;            // spill [nblocks][nspill][G]
+    int* iters;
+    int* status;
+    double* Jx_last;     // optional (instance 0)
+};
+
+struct c64 {
+    double re, im;
+};
+__device__ __forceinline__ c64 cmul(c64 a, c64 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ c64 cconj(c64 a) { return {a.re, -a.im}; }
+__device__ __forceinline__ c64 csub(c64 a, c64 b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ c64 cadd(c64 a, c64 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ c64 ld_c(const double* p, int i) { return {p[2 * i], p[2 * i + 1]}; }
+
+// Reduce over the nslot lanes of one instance: shuffles inside the wave
+// (lanes g, g+G, ... of a wave), then the kWaves partials through LDS.
+template <int G, bool IS_MAX>
+__device__ __forceinline__ double inst_reduce(double v, double (*part)[kMaxG]) {
+#pragma unroll
+    for (int o = G; o < 64; o <<= 1) {
+        const double u = __shfl_xor(v, o);
+        v = IS_MAX ? fmax(v, u) : fmin(v, u);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane < G) part[wave][lane] = v;
+    __syncthreads();
+    const int g = threadIdx.x % G;
+    double r = part[0][g];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) r = IS_MAX ? fmax(r, part[k][g]) : fmin(r, part[k][g]);
+    return r;
+}
+
+__device__ __forceinline__ double inf_if_nonfinite(double a) { return (a <= 1.7976931348623157e308) ? a : INFINITY; }
+
+template <typename FT, int G>
+__global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    FT* Xl = reinterpret_cast<FT*>(lds_raw);
+    __shared__ double s_part[kWaves][kMaxG];
+    __shared__ double s_slack[kMaxG];
+    __shared__ int s_done[kMaxG], s_it[kMaxG], s_status[kMaxG], s_piv[kMaxG];
+    __shared__ int s_ndone;
+
+    const int tid = threadIdx.x;
+    const int g = tid % G, s = tid / G;
+    constexpr int nslot = kBlock / G;
+    const long long inst = (long long)blockIdx.x * G + g;
+    const bool valid = inst < a.ninst;
+    const long long ii = valid ? inst : 0;
+    const int n = a.n;
+    FT* Xgb = reinterpret_cast<FT*>(a.Xg) + (size_t)blockIdx.x * (size_t)a.nspill * G;
+    const int nlds = a.nlds;
+    auto XL = [&](int slot) -> FT {
+        return slot < nlds ? Xl[slot * G + g] : Xgb[(size_t)(slot - nlds) * G + g];
+    };
+    auto XS = [&](int slot, FT v) {
+        if (slot < nlds) Xl[slot * G + g] = v;
+        else Xgb[(size_t)(slot - nlds) * G + g] = v;
+    };
+    const double* Y = a.Yx + ii * a.y_stride;
+    const double* S = a.S + ii * a.s_stride;
+    double* V = a.V + ii * 2 * (long long)n;
+    double* Va = a.Va + ii * (long long)n;
+    double* Vm = a.Vm + ii * (long long)n;
+    double* Ib = a.Ib + ii * 2 * (long long)n;
+
+    // ---- init: Va/Vm from V, slack_absorbed = Re(sum Sbus) (BaseNRAlgo.tpp:60,77-79)
+    double ssum = 0.0;
+    if (valid)
+        for (int b = s; b < n; b += nslot) {
+            const double vr = V[2 * b], vi = V[2 * b + 1];
+            Va[b] = atan2(vi, vr);
+            Vm[b] = sqrt(vr * vr + vi * vi);
+            ssum += S[2 * b];
+        }
+    // sum-reduce (as max of a single value is not a sum, do a dedicated pass)
+#pragma unroll
+    for (int o = G; o < 64; o <<= 1) ssum += __shfl_xor(ssum, o);
+    {
+        const int lane = tid & 63, wave = tid >> 6;
+        if (lane < G) s_part[wave][lane] = ssum;
+        __syncthreads();
+        if (s == 0) {
+            double t = 0;
+            for (int k = 0; k < kWaves; ++k) t += s_part[k][g];
+            s_slack[g] = t;
+            s_it[g] = 0;
+            s_status[g] = 0;
+            s_piv[g] = 0;
+            s_done[g] = valid ? 0 : 1;
+        }
+        __syncthreads();
+    }
+
+    // ---- mismatch phase ------------------------------------------------------
+    auto mismatch = [&]() -> double {
+        double fm = 0.0;
+        if (valid && !s_done[g]) {
+            const double sa = s_slack[g];
+            for (int r = s; r < n; r += nslot) {
+                c64 acc = {0.0, 0.0};
+                for (int q = a.yr_ptr[r]; q < a.yr_ptr[r + 1]; ++q)
+                    acc = cadd(acc, cmul(ld_c(Y, a.yr_pos[q]), ld_c(V, a.yr_col[q])));
+                Ib[2 * r] = acc.re;
+                Ib[2 * r + 1] = acc.im;
+                c64 mis = csub(cmul(ld_c(V, r), cconj(acc)), ld_c(S, r));
+                if (!a.single_slack) mis.re += sa * a.w[r];
+                const int fp = a.fslot_p[r], fq = a.fslot_q[r];
+                if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
+                if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
+            }
+        }
+        return inst_reduce<G, true>(fm, s_part);
+    };
+
+    double fmx = mismatch();
+    if (s == 0 && valid && !s_done[g]) {
+        if (fmx < a.tol) s_done[g] = 1;  // converged at iteration 0
+        else if (a.max_iter <= 0) { s_status[g] = 2; s_done[g] = 1; }
+    }
+    __syncthreads();
+
+    // J value of J entry q at the current V / Ibus (dS_dVa / dS_dVm, BaseNRAlgo.tpp:202-260)
+    auto jvalue = [&](int q) -> double {
+        const int sel = a.j_sel[q];
+        if (sel == 4) return a.j_cval[q];
+        const int r = a.j_rbus[q], c = a.j_cbus[q];
+        const c64 y = ld_c(Y, a.j_ypos[q]);
+        const c64 vr = ld_c(V, r), vc = ld_c(V, c);
+        if (sel >= 2) {
+            const double ic = 1.0 / sqrt(vc.re * vc.re + vc.im * vc.im);
+            c64 d = cmul(cconj(cmul(y, {vc.re * ic, vc.im * ic})), vr);
+            if (r == c) {
+                const double ir = 1.0 / sqrt(vr.re * vr.re + vr.im * vr.im);
+                d = cadd(d, cmul(cconj(ld_c(Ib, r)), {vr.re * ir, vr.im * ir}));
+            }
+            return sel == 2 ? d.re : d.im;
+        }
+        c64 t = cmul(y, vc);
+        if (r == c) t = csub(t, ld_c(Ib, r));
+        const c64 d = cmul(cconj({-t.re, -t.im}), {-vr.im, vr.re});  // * (j V_r)
+        return sel == 0 ? d.re : d.im;
+    };
+
+    for (;;) {
+        if (tid == 0) {
+            int nd = 0;
+            for (int k = 0; k < G; ++k) nd += s_done[k];
+            s_ndone = nd;
+        }
+        __syncthreads();
+        if (s_ndone == G) break;
+        bool act = valid && !s_done[g];
+        if (s == 0 && act) s_it[g] += 1;
+
+        // ---- factor + forward solve --------------------------------------------
+        for (int l = 0; l < a.nlev; ++l) {
+            const int team = a.f_team[l];
+            const int tpos = s & (team - 1);
+            const int nteams = nslot / team;
+            const int hi = a.f_lvl[l + 1];
+            if (act) {
+                for (int t = a.f_lvl[l] + s / team; t < hi; t += nteams) {
+                    const int pb = a.t_pbeg[t], pe = a.t_pbeg[t + 1];
+                    FT acc = (FT)0;
+                    for (int p = pb + tpos; p < pe; p += team) acc += XL(a.p_l[p]) * XL(a.p_u[p]) * XL(a.p_d[p]);
+                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
+                    if (tpos == 0) {
+                        const int kind = a.t_kind[t], o = a.t_out[t], js = a.t_jsrc[t];
+                        FT av = (FT)0;
+                        if (kind == 2) {
+                            av = XL(o);
+                        } else if (js >= 0) {
+                            const double jv = jvalue(js);
+                            if (a.Jx_last != nullptr && inst == 0) a.Jx_last[js] = jv;
+                            av = (FT)jv;
+                        }
+                        FT v = av - acc;
+                        if (kind == 1) {
+                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                            v = (FT)1 / v;
+                        }
+                        XS(o, v);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (s == 0 && act && s_piv[g]) {  // zero / non-finite pivot: klu_refactor failure
+            s_status[g] = 6;
+            s_done[g] = 1;
+        }
+        __syncthreads();
+        act = valid && !s_done[g];
+
+        // ---- backward solve ---------------------------------------------------------
+        for (int l = 0; l < a.nblev; ++l) {
+            const int team = a.b_team[l];
+            const int tpos = s & (team - 1);
+            const int nteams = nslot / team;
+            const int hi = a.b_lvl[l + 1];
+            if (act) {
+                for (int t = a.b_lvl[l] + s / team; t < hi; t += nteams) {
+                    const int pb = a.bt_pbeg[t], pe = a.bt_pbeg[t + 1];
+                    FT acc = (FT)0;
+                    for (int p = pb + tpos; p < pe; p += team) acc += XL(a.bp_u[p]) * XL(a.bp_x[p]);
+                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
+                    if (tpos == 0) {
+                        const int o = a.bt_out[t];
+                        XS(o, (XL(o) - acc) * XL(a.bt_diag[t]));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- voltage update ------------------------------------------------------------
+        if (act && a.single_slack)  // BaseNRSingleSlackAlgo.tpp:131-132
+            for (int b = s; b < n; b += nslot) {
+                const double vr = V[2 * b], vi = V[2 * b + 1];
+                Va[b] = atan2(vi, vr);
+                Vm[b] = sqrt(vr * vr + vi * vi);
+            }
+        __syncthreads();
+        if (act)
+            for (int j = s; j < a.N; j += nslot) {
+                const double dx = (double)XL(a.ux_slot[j]);
+                const int kind = a.ux_kind[j], b = a.ux_bus[j];
+                if (kind == 0) s_slack[g] -= dx;
+                else if (kind == 1) Va[b] -= dx;
+                else Vm[b] -= dx;
+            }
+        __syncthreads();
+        double vmin = INFINITY;
+        if (act)
+            for (int b = s; b < n; b += nslot) {
+                double sn, cs;
+                sincos(Va[b], &sn, &cs);
+                const double m = Vm[b];
+                V[2 * b] = m * cs;
+                V[2 * b + 1] = m * sn;
+                vmin = fmin(vmin, m);
+            }
+        vmin = inst_reduce<G, false>(vmin, s_part);
+        if (act && vmin < 0.0)
+            for (int b = s; b < n; b += nslot) {
+                const double vr = V[2 * b], vi = V[2 * b + 1];
+                Va[b] = atan2(vi, vr);
+                Vm[b] = sqrt(vr * vr + vi * vi);
+            }
+        __syncthreads();
+
+        // ---- mismatch + convergence ----------------------------------------------------
+        fmx = mismatch();
+        if (s == 0 && act) {
+            if (!(fmx <= 1.7976931348623157e308)) { s_status[g] = 3; s_done[g] = 1; }
+            else if (fmx < a.tol) { s_status[g] = 0; s_done[g] = 1; }
+            else if (s_it[g] >= a.max_iter) { s_status[g] = 2; s_done[g] = 1; }
+        }
+        __syncthreads();
+    }
+    if (s == 0 && valid) {
+        a.iters[inst] = s_it[g];
+        a.status[inst] = s_status[g];
+    }
+}
+
+// Per-instance setup: Ybus values with deltas and broadcast of V0.
+__global__ void k_apply_deltas(const double* Ybase, double* Yout, int nnzY, int ninst,
+                               const int* dptr, const int* dpos, const double* dval) {
+    const long long total = (long long)ninst * nnzY;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(k % nnzY);
+        Yout[2 * k] = Ybase[2 * q];
+        Yout[2 * k + 1] = Ybase[2 * q + 1];
+    }
+}
+__global__ void k_apply_deltas2(double* Yout, int nnzY, int ninst, const int* dptr, const int* dpos,
+                                const double* dval) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    double* Y = Yout + 2 * (long long)i * nnzY;
+    for (int k = dptr[i]; k < dptr[i + 1]; ++k) {
+        const int q = dpos[k];
+        Y[2 * q] -= dval[2 * k];
+        Y[2 * q + 1] -= dval[2 * k + 1];
+    }
+}
+__global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n, int ninst) {
+    const long long total = (long long)ninst * n;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long i = k / n;
+        const int b = (int)(k - i * n);
+        V[2 * k] = V0[i * v0_stride + 2 * b];
+        V[2 * k + 1] = V0[i * v0_stride + 2 * b + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void upload(const std::vector<T>& v) {
+        release();
+        n = v.size();
+        if (n == 0) return;
+        HIP_OK(hipMalloc(&p, n * sizeof(T)));
+        HIP_OK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct pfgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    int n_cu = 0;
+    int lds_max = 0;
+    // grow-only workspace
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+};
+
+struct pfgpu_grid {
+    pfgpu_ctx* ctx = nullptr;
+    pfgpu::YbusPattern Y;
+    pfgpu::JPattern J;
+    pfgpu::DevicePlan P;
+    pfgpu_grid_info info{};
+    DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, f_lvl, t_out, t_pbeg,
+        t_jsrc, p_l, p_u, p_d, b_lvl, bt_out, bt_pbeg, bt_diag, bp_u, bp_x;
+    DevBuf<int8_t> ux_kind;
+    DevBuf<uint8_t> j_sel, t_kind;
+    DevBuf<double> j_cval, w;
+    // per-G team tables
+    struct Teams {
+        DevBuf<int> f, b;
+    };
+    std::vector<std::unique_ptr<Teams>> teams;  // index log2(G)
+};
+
+namespace {
+
+template <typename FT, int G>
+void launch_nr(const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
+    auto kfn = k_nr<FT, G>;
+    if (lds > 65536) HIP_OK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(kBlock), lds, st, a);
+    HIP_OK(hipGetLastError());
+}
+
+template <typename FT>
+void launch_nr_g(int G, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
+    switch (G) {
+        case 1: launch_nr<FT, 1>(a, nblocks, lds, st); break;
+        case 2: launch_nr<FT, 2>(a, nblocks, lds, st); break;
+        case 4: launch_nr<FT, 4>(a, nblocks, lds, st); break;
+        case 8: launch_nr<FT, 8>(a, nblocks, lds, st); break;
+        case 16: launch_nr<FT, 16>(a, nblocks, lds, st); break;
+        case 32: launch_nr<FT, 32>(a, nblocks, lds, st); break;
+        default: throw ApiError(PFGPU_E_INVALID, "instances_per_wg must be 1, 2, 4, 8, 16 or 32");
+    }
+}
+
+int log2i(int g) {
+    int l = 0;
+    while ((1 << l) < g) ++l;
+    return l;
+}
+
+struct LaunchChoice {
+    int G, nlds, nspill, nblocks;
+    size_t lds;
+};
+
+// Launch heuristic: the largest G whose value arrays fit entirely in LDS while
+// keeping at least two workgroups per CU; otherwise G = 1 with the hottest
+// slots in LDS and the rest spilled to global memory.
+LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req) {
+    const size_t lds_avail = (size_t)std::max(0, ctx->lds_max - 2048);  // keep room for static LDS
+    LaunchChoice c{};
+    if (G_req > 0) {
+        c.G = G_req;
+    } else {
+        c.G = 1;
+        for (int G = kMaxG; G >= 2; G >>= 1) {
+            const size_t need = (size_t)P.nvals * G * fsize;
+            const long long blocks = (ninst + G - 1) / G;
+            if (need <= lds_avail && blocks >= 2LL * std::max(1, ctx->n_cu)) { c.G = G; break; }
+        }
+        if (c.G == 1) {
+            // small batches: still allow G>1 if it fits and there is more than one instance
+            for (int G = std::min(kMaxG, 8); G >= 2; G >>= 1) {
+                const size_t need = (size_t)P.nvals * G * fsize;
+                if (need <= lds_avail && ninst >= 4 * G * std::max(1, ctx->n_cu)) { c.G = G; break; }
+            }
+        }
+    }
+    const size_t cap = lds_avail / ((size_t)c.G * fsize);
+    c.nlds = (int)std::min<size_t>((size_t)P.nvals, cap);
+    c.nspill = P.nvals - c.nlds;
+    c.nblocks = (ninst + c.G - 1) / c.G;
+    c.lds = (size_t)c.nlds * c.G * fsize;
+    return c;
+}
+
+void ensure_teams(pfgpu_grid* gr, int G) {
+    const int lg = log2i(G);
+    if ((int)gr->teams.size() <= lg) gr->teams.resize(lg + 1);
+    if (gr->teams[lg]) return;
+    auto t = std::make_unique<pfgpu_grid::Teams>();
+    t->f.upload(pfgpu::level_teams(gr->P.f_lvl, gr->P.f_maxp, G, kBlock / G));
+    t->b.upload(pfgpu::level_teams(gr->P.b_lvl, gr->P.b_maxp, G, kBlock / G));
+    gr->teams[lg] = std::move(t);
+}
+
+void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
+    if (bytes > ctx->budget) throw ApiError(PFGPU_E_BUDGET, "batch workspace exceeds the context's HBM budget");
+    if (bytes > ctx->ws_bytes) {
+        if (ctx->ws) HIP_OK(hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+        HIP_OK(hipMalloc(&ctx->ws, bytes));
+        ctx->ws_bytes = bytes;
+    }
+    return ctx->ws;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Core solve on device buffers. Workspace layout: [Yinst?][Va][Vm][Ib][Xg].
+void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
+                  const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
+                  int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
+                  double* d_Jx_last, pfgpu_stats* st) {
+    const pfgpu::DevicePlan& P = gr->P;
+    const int n = P.n;
+    if (ninst <= 0) return;
+    if (precision != PFGPU_FP64 && precision != PFGPU_FP32) throw ApiError(PFGPU_E_INVALID, "bad precision");
+    if (Yx_stride != 0 && Yx_stride != P.nnzY) throw ApiError(PFGPU_E_INVALID, "Yx_stride must be 0 or nnz_ybus");
+    if (sbus_stride != 0 && sbus_stride != n) throw ApiError(PFGPU_E_INVALID, "sbus_stride must be 0 or n");
+    if (v0_stride != 0 && v0_stride != n) throw ApiError(PFGPU_E_INVALID, "v0_stride must be 0 or n");
+    if (max_iter < 0) throw ApiError(PFGPU_E_INVALID, "max_iter must be >= 0");
+    const int fsize = precision == PFGPU_FP32 ? 4 : 8;
+    LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req);
+    ensure_teams(gr, lc.G);
+    const bool deltas = d_dptr != nullptr;
+    size_t off = 0;
+    const size_t o_y = off;
+    if (deltas) off += align_up((size_t)ninst * P.nnzY * 16);
+    const size_t o_va = off; off += align_up((size_t)ninst * n * 8);
+    const size_t o_vm = off; off += align_up((size_t)ninst * n * 8);
+    const size_t o_ib = off; off += align_up((size_t)ninst * n * 16);
+    const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
+    char* ws = static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
+    hipStream_t stream = ctx->stream;
+    const double* Ypt = d_Yx;
+    long long ystride = (long long)Yx_stride * 2;
+    if (deltas) {
+        double* Yi = reinterpret_cast<double*>(ws + o_y);
+        if (Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
+        hipLaunchKernelGGL(k_apply_deltas, dim3(1024), dim3(256), 0, stream, d_Yx, Yi, P.nnzY, ninst, d_dptr, d_dpos,
+                           d_dval);
+        hipLaunchKernelGGL(k_apply_deltas2, dim3((ninst + 255) / 256), dim3(256), 0, stream, Yi, P.nnzY, ninst, d_dptr,
+                           d_dpos, d_dval);
+        HIP_OK(hipGetLastError());
+        Ypt = Yi;
+        ystride = (long long)P.nnzY * 2;
+    }
+    if (d_V0 != d_V || v0_stride == 0) {
+        // d_V0 == d_V with a shared row is allowed: row 0 is only read by the threads writing row 0 itself
+        hipLaunchKernelGGL(k_init_v, dim3(1024), dim3(256), 0, stream, d_V0, (long long)v0_stride * 2, d_V, n, ninst);
+        HIP_OK(hipGetLastError());
+    }
+    KArgs a{};
+    a.n = n; a.N = P.N; a.nlev = P.nlev; a.nblev = P.nblev; a.nlds = lc.nlds; a.nspill = lc.nspill;
+    a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
+    a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
+    a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
+    a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
+    a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
+    auto& tm = *gr->teams[log2i(lc.G)];
+    a.f_lvl = gr->f_lvl.p; a.f_team = tm.f.p; a.t_out = gr->t_out.p; a.t_pbeg = gr->t_pbeg.p; a.t_jsrc = gr->t_jsrc.p;
+    a.t_kind = gr->t_kind.p; a.p_l = gr->p_l.p; a.p_u = gr->p_u.p; a.p_d = gr->p_d.p;
+    a.b_lvl = gr->b_lvl.p; a.b_team = tm.b.p; a.bt_out = gr->bt_out.p; a.bt_pbeg = gr->bt_pbeg.p; a.bt_diag = gr->bt_diag.p;
+    a.bp_u = gr->bp_u.p; a.bp_x = gr->bp_x.p; a.w = gr->w.p;
+    a.Yx = Ypt; a.y_stride = ystride;
+    a.S = d_S; a.s_stride = (long long)sbus_stride * 2;
+    a.V = d_V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
+    a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
+    a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
+    HIP_OK(hipEventRecord(ctx->ev0, stream));
+    if (precision == PFGPU_FP32) launch_nr_g<float>(lc.G, a, lc.nblocks, lc.lds, stream);
+    else launch_nr_g<double>(lc.G, a, lc.nblocks, lc.lds, stream);
+    HIP_OK(hipEventRecord(ctx->ev1, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    if (st) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        st->kernel_s += ms * 1e-3;
+        st->instances_per_wg = lc.G;
+        st->lds_slots = lc.nlds;
+        st->workgroups = lc.nblocks;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pfgpu_last_error(void) { return g_last_error.c_str(); }
+int pfgpu_abi_version(void) { return PFGPU_ABI_VERSION; }
+
+int pfgpu_ctx_create(int device, size_t hbm_budget_bytes, pfgpu_ctx** out) {
+    return guarded([&]() -> int {
+        if (!out) throw ApiError(PFGPU_E_INVALID, "out is NULL");
+        *out = nullptr;
+        int ndev = 0;
+        HIP_OK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw ApiError(PFGPU_E_INVALID, "no such device");
+        HIP_OK(hipSetDevice(device));
+        auto c = std::make_unique<pfgpu_ctx>();
+        c->device = device;
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+        HIP_OK(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        size_t fr = 0, tot = 0;
+        HIP_OK(hipMemGetInfo(&fr, &tot));
+        c->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * (double)fr);
+        HIP_OK(hipEventCreate(&c->ev0));
+        HIP_OK(hipEventCreate(&c->ev1));
+        *out = c.release();
+        return 0;
+    });
+}
+
+int pfgpu_ctx_destroy(pfgpu_ctx* ctx) {
+    return guarded([&]() -> int {
+        if (!ctx) return 0;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->ws) (void)hipFree(ctx->ws);
+        if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return 0;
+    });
+}
+
+int pfgpu_ctx_device_info(pfgpu_ctx* ctx, int* n_cu, int* lds_bytes, size_t* free_bytes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ApiError(PFGPU_E_INVALID, "ctx is NULL");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (n_cu) *n_cu = ctx->n_cu;
+        if (lds_bytes) *lds_bytes = ctx->lds_max;
+        if (free_bytes) {
+            size_t fr = 0, tot = 0;
+            HIP_OK(hipMemGetInfo(&fr, &tot));
+            *free_bytes = fr;
+        }
+        return 0;
+    });
+}
+
+int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+                    const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack,
+                    pfgpu_grid** out) {
+    return guarded([&]() -> int {
+        if (!ctx || !out || !Yp || !Yi || !slack_ids || !slack_weights || n <= 0 || nslack < 1 || npv < 0 || npq < 0 ||
+            (npv > 0 && !pv) || (npq > 0 && !pq))
+            throw ApiError(PFGPU_E_INVALID, "invalid grid arguments");
+        *out = nullptr;
+        HIP_OK(hipSetDevice(ctx->device));
+        auto g = std::make_unique<pfgpu_grid>();
+        g->ctx = ctx;
+        g->Y.n = n;
+        g->Y.p.assign(Yp, Yp + n + 1);
+        if (g->Y.p[0] != 0) throw ApiError(PFGPU_E_INVALID, "Yp[0] must be 0");
+        for (int c = 0; c < n; ++c)
+            if (g->Y.p[c + 1] < g->Y.p[c]) throw ApiError(PFGPU_E_INVALID, "Yp must be non-decreasing");
+        g->Y.i.assign(Yi, Yi + Yp[n]);
+        for (int c = 0; c < n; ++c)
+            for (int q = Yp[c]; q < Yp[c + 1]; ++q) {
+                if (Yi[q] < 0 || Yi[q] >= n) throw ApiError(PFGPU_E_INVALID, "Ybus row index out of range");
+                if (q > Yp[c] && Yi[q] <= Yi[q - 1]) throw ApiError(PFGPU_E_INVALID, "Ybus rows must be sorted and unique");
+            }
+        for (int c = 0; c < n; ++c) {
+            bool diag = false;
+            for (int q = Yp[c]; q < Yp[c + 1]; ++q) diag |= (Yi[q] == c);
+            if (!diag) throw ApiError(PFGPU_E_INVALID, "Ybus must hold every diagonal entry");
+        }
+        for (int k = 0; k < nslack; ++k)
+            if (slack_ids[k] < 0 || slack_ids[k] >= n) throw ApiError(PFGPU_E_INVALID, "slack bus out of range");
+        for (int k = 0; k < npq; ++k)
+            if (pq[k] < 0 || pq[k] >= n) throw ApiError(PFGPU_E_INVALID, "pq bus out of range");
+        for (int k = 0; k < npv; ++k)
+            if (pv[k] < 0 || pv[k] >= n) throw ApiError(PFGPU_E_INVALID, "pv bus out of range");
+        g->J = pfgpu::build_jacobian_pattern(g->Y, slack_ids, nslack, slack_weights, pv, npv, pq, npq, single_slack != 0);
+        pfgpu::SymbolicLU S = pfgpu::symbolic_lu(g->J);
+        pfgpu::Schedule sch = pfgpu::build_schedule(S);
+        g->P = pfgpu::build_device_plan(g->Y, g->J, S, sch);
+        const auto& P = g->P;
+        g->info.n_bus = n;
+        g->info.n_jac = P.N;
+        g->info.nnz_ybus = P.nnzY;
+        g->info.nnz_jac = P.nJ;
+        g->info.nnz_lu = P.nnzLU;
+        g->info.n_slots = P.nvals;
+        g->info.n_levels = P.nlev;
+        g->info.n_back_levels = P.nblev;
+        g->info.n_pairs = (int64_t)P.p_l.size();
+        g->info.flops = P.flops;
+        g->info.single_slack = single_slack ? 1 : 0;
+        g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
+        g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
+        g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
+        g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
+        g->j_cval.upload(P.j_cval);
+        g->f_lvl.upload(P.f_lvl); g->t_out.upload(P.t_out); g->t_pbeg.upload(P.t_pbeg); g->t_jsrc.upload(P.t_jsrc);
+        g->t_kind.upload(P.t_kind);
+        g->p_l.upload(P.p_l); g->p_u.upload(P.p_u); g->p_d.upload(P.p_d);
+        g->b_lvl.upload(P.b_lvl); g->bt_out.upload(P.bt_out); g->bt_pbeg.upload(P.bt_pbeg); g->bt_diag.upload(P.bt_diag);
+        g->bp_u.upload(P.bp_u); g->bp_x.upload(P.bp_x);
+        g->w.upload(std::vector<double>(slack_weights, slack_weights + n));
+        // buffers that may be empty still need a valid pointer for the kernel
+        if (!g->p_l.p) { std::vector<int> z(1, 0); g->p_l.upload(z); g->p_u.upload(z); g->p_d.upload(z); }
+        if (!g->bp_u.p) { std::vector<int> z(1, 0); g->bp_u.upload(z); g->bp_x.upload(z); }
+        *out = g.release();
+        return 0;
+    });
+}
+
+int pfgpu_grid_destroy(pfgpu_grid* grid) {
+    return guarded([&]() -> int {
+        if (!grid) return 0;
+        (void)hipSetDevice(grid->ctx->device);
+        delete grid;
+        return 0;
+    });
+}
+
+int pfgpu_grid_get_info(const pfgpu_grid* grid, pfgpu_grid_info* info) {
+    return guarded([&]() -> int {
+        if (!grid || !info) throw ApiError(PFGPU_E_INVALID, "NULL argument");
+        *info = grid->info;
+        return 0;
+    });
+}
+
+int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji) {
+    return guarded([&]() -> int {
+        if (!grid || !Jp || !Ji) throw ApiError(PFGPU_E_INVALID, "NULL argument");
+        std::copy(grid->J.Jp.begin(), grid->J.Jp.end(), Jp);
+        std::copy(grid->J.Ji.begin(), grid->J.Ji.end(), Ji);
+        return 0;
+    });
+}
+
+int pfgpu_dev_alloc(pfgpu_ctx* ctx, size_t bytes, void** dptr) {
+    return guarded([&]() -> int {
+        if (!ctx || !dptr) throw ApiError(PFGPU_E_INVALID, "NULL argument");
+        HIP_OK(hipSetDevice(ctx->device));
+        *dptr = nullptr;
+        HIP_OK(hipMalloc(dptr, std::max<size_t>(bytes, 1)));
+        return 0;
+    });
+}
+int pfgpu_dev_free(pfgpu_ctx* ctx, void* dptr) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ApiError(PFGPU_E_INVALID, "NULL ctx");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (dptr) HIP_OK(hipFree(dptr));
+        return 0;
+    });
+}
+int pfgpu_memcpy_h2d(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ApiError(PFGPU_E_INVALID, "NULL ctx");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+int pfgpu_memcpy_d2h(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ApiError(PFGPU_E_INVALID, "NULL ctx");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
+                             const int* d_dptr, const int* d_dpos, const double* d_dval, const double* d_Sbus,
+                             int sbus_stride, const double* d_V0, int v0_stride, double* d_V, int max_iter, double tol,
+                             int precision, int instances_per_wg, int* d_iters, int* d_status, pfgpu_stats* stats) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !d_Yx || !d_Sbus || !d_V0 || !d_V || !d_iters || !d_status || ninst < 0)
+            throw ApiError(PFGPU_E_INVALID, "invalid solve arguments");
+        if ((d_dptr == nullptr) != (d_dpos == nullptr) || (d_dptr == nullptr) != (d_dval == nullptr))
+            throw ApiError(PFGPU_E_INVALID, "dptr/dpos/dval must be all set or all NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        if (stats) *stats = pfgpu_stats{};
+        solve_device(ctx, grid, ninst, d_Yx, Yx_stride, d_dptr, d_dpos, d_dval, d_Sbus, sbus_stride, d_V0, v0_stride,
+                     d_V, max_iter, tol, precision, instances_per_wg, d_iters, d_status, nullptr, stats);
+        if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    });
+}
+
+int pfgpu_solve_batch(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* Yx, int Yx_stride, const int* dptr,
+                      const int* dpos, const double* dval, const double* Sbus, int sbus_stride, double* V, int v0_stride,
+                      int max_iter, double tol, int precision, int instances_per_wg, int* iters, int* status,
+                      double* Jx_last, pfgpu_stats* stats) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !Yx || !Sbus || !V || !iters || !status || ninst < 0)
+            throw ApiError(PFGPU_E_INVALID, "invalid solve arguments");
+        if ((dptr == nullptr) != (dpos == nullptr) || (dptr == nullptr) != (dval == nullptr))
+            throw ApiError(PFGPU_E_INVALID, "dptr/dpos/dval must be all set or all NULL");
+        if (ninst == 0) return 0;
+        std::unique_lock<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        using clk = std::chrono::steady_clock;
+        auto t0 = clk::now();
+        pfgpu_stats st{};
+        const pfgpu::DevicePlan& P = grid->P;
+        const int n = P.n;
+        if (dptr) {
+            if (dptr[0] != 0) throw ApiError(PFGPU_E_INVALID, "dptr[0] must be 0");
+            for (int i = 0; i < ninst; ++i)
+                if (dptr[i + 1] < dptr[i]) throw ApiError(PFGPU_E_INVALID, "dptr must be non-decreasing");
+            for (int k = 0; k < dptr[ninst]; ++k)
+                if (dpos[k] < 0 || dpos[k] >= P.nnzY) throw ApiError(PFGPU_E_INVALID, "delta position out of range");
+        }
+        const size_t ny = (size_t)(Yx_stride ? ninst : 1) * P.nnzY * 16;
+        const size_t ns = (size_t)(sbus_stride ? ninst : 1) * n * 16;
+        const size_t nv = (size_t)ninst * n * 16;
+        const int nd = dptr ? dptr[ninst] : 0;
+        std::vector<double> v0_copy;
+        if (precision == PFGPU_FP32) v0_copy.assign(V, V + (v0_stride ? (size_t)ninst : 1) * n * 2);
+        struct Tmp {
+            std::vector<void*> ptrs;
+            ~Tmp() { for (void* p : ptrs) (void)hipFree(p); }
+            void* get(size_t b) { void* p = nullptr; HIP_OK(hipMalloc(&p, std::max<size_t>(b, 1))); ptrs.push_back(p); return p; }
+        } tmp;
+        double* dY = static_cast<double*>(tmp.get(ny));
+        double* dS = static_cast<double*>(tmp.get(ns));
+        double* dV = static_cast<double*>(tmp.get(nv));
+        int* dIt = static_cast<int*>(tmp.get((size_t)ninst * 4));
+        int* dSt = static_cast<int*>(tmp.get((size_t)ninst * 4));
+        double* dJ = Jx_last ? static_cast<double*>(tmp.get((size_t)P.nJ * 8)) : nullptr;
+        int *ddp = nullptr, *ddq = nullptr;
+        double* ddv = nullptr;
+        if (dptr) {
+            ddp = static_cast<int*>(tmp.get((size_t)(ninst + 1) * 4));
+            ddq = static_cast<int*>(tmp.get((size_t)std::max(nd, 1) * 4));
+            ddv = static_cast<double*>(tmp.get((size_t)std::max(nd, 1) * 16));
+        }
+        hipStream_t s = ctx->stream;
+        HIP_OK(hipMemcpyAsync(dY, Yx, ny, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(dS, Sbus, ns, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(dV, V, v0_stride ? nv : (size_t)n * 16, hipMemcpyHostToDevice, s));
+        if (dptr) {
+            HIP_OK(hipMemcpyAsync(ddp, dptr, (size_t)(ninst + 1) * 4, hipMemcpyHostToDevice, s));
+            if (nd) {
+                HIP_OK(hipMemcpyAsync(ddq, dpos, (size_t)nd * 4, hipMemcpyHostToDevice, s));
+                HIP_OK(hipMemcpyAsync(ddv, dval, (size_t)nd * 16, hipMemcpyHostToDevice, s));
+            }
+        }
+        HIP_OK(hipStreamSynchronize(s));
+        auto t1 = clk::now();
+        st.h2d_s = std::chrono::duration<double>(t1 - t0).count();
+        solve_device(ctx, grid, ninst, dY, Yx_stride, ddp, ddq, ddv, dS, sbus_stride, dV, v0_stride, dV, max_iter, tol,
+                     precision, instances_per_wg, dIt, dSt, dJ, &st);
+        auto t2 = clk::now();
+        HIP_OK(hipMemcpyAsync(V, dV, nv, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(iters, dIt, (size_t)ninst * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(status, dSt, (size_t)ninst * 4, hipMemcpyDeviceToHost, s));
+        if (Jx_last) HIP_OK(hipMemcpyAsync(Jx_last, dJ, (size_t)P.nJ * 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        st.d2h_s = std::chrono::duration<double>(clk::now() - t2).count();
+        if (precision == PFGPU_FP32) {
+            // Instances that did not converge with FP32 factors are re-solved with
+            // FP64 factors from their original initial voltages.
+            std::vector<int> bad;
+            for (int i = 0; i < ninst; ++i)
+                if (status[i] != 0) bad.push_back(i);
+            if (!bad.empty()) {
+                const int nb = (int)bad.size();
+                std::vector<double> subY, subS, subV((size_t)nb * n * 2), subdv;
+                std::vector<int> subdp, subdq, subIt(nb), subSt(nb);
+                if (Yx_stride) {
+                    subY.resize((size_t)nb * P.nnzY * 2);
+                    for (int k = 0; k < nb; ++k)
+                        std::memcpy(&subY[(size_t)k * P.nnzY * 2], Yx + (size_t)bad[k] * P.nnzY * 2, (size_t)P.nnzY * 16);
+                }
+                if (sbus_stride) {
+                    subS.resize((size_t)nb * n * 2);
+                    for (int k = 0; k < nb; ++k)
+                        std::memcpy(&subS[(size_t)k * n * 2], Sbus + (size_t)bad[k] * n * 2, (size_t)n * 16);
+                }
+                for (int k = 0; k < nb; ++k)
+                    std::memcpy(&subV[(size_t)k * n * 2], v0_copy.data() + (v0_stride ? (size_t)bad[k] * n * 2 : 0),
+                                (size_t)n * 16);
+                if (dptr) {
+                    subdp.push_back(0);
+                    for (int k = 0; k < nb; ++k) {
+                        for (int q = dptr[bad[k]]; q < dptr[bad[k] + 1]; ++q) {
+                            subdq.push_back(dpos[q]);
+                            subdv.push_back(dval[2 * q]);
+                            subdv.push_back(dval[2 * q + 1]);
+                        }
+                        subdp.push_back((int)subdq.size());
+                    }
+                    if (subdq.empty()) { subdq.push_back(0); subdv.push_back(0); subdv.push_back(0); }
+                }
+                pfgpu_stats st2{};
+                lk.unlock();
+                const int rc = pfgpu_solve_batch(ctx, grid, nb, Yx_stride ? subY.data() : Yx, Yx_stride,
+                                                 dptr ? subdp.data() : nullptr, dptr ? subdq.data() : nullptr,
+                                                 dptr ? subdv.data() : nullptr, sbus_stride ? subS.data() : Sbus,
+                                                 sbus_stride, subV.data(), n, max_iter, tol, PFGPU_FP64,
+                                                 instances_per_wg, subIt.data(), subSt.data(), nullptr, &st2);
+                if (rc != 0) return rc;
+                for (int k = 0; k < nb; ++k) {
+                    std::memcpy(V + (size_t)bad[k] * n * 2, &subV[(size_t)k * n * 2], (size_t)n * 16);
+                    iters[bad[k]] = subIt[k];
+                    status[bad[k]] = subSt[k];
+                }
+                st.fallback_fp64 = nb;
+                st.kernel_s += st2.kernel_s;
+            }
+        }
+        st.total_s = std::chrono::duration<double>(clk::now() - t0).count();
+        if (stats) *stats = st;
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+"""Synthetic grid generator for the benchmark and test workloads.
+
+A grid is a spanning tree plus extra edges on buses placed uniformly at random
+in the unit square. The tree attaches each new bus (random order) to one of its
+3 nearest already-placed buses; each extra edge joins a random bus to one of its
+6 nearest neighbours. This locality gives the sparsity of real transmission
+grids: at 2000 buses the factor fill matches the reference's case1888 fixture
+(nnz(L+U) / nnz(J) ~ 1.5), where uniformly random chords would give ~6x fill
+and ~100x the refactorisation flops.
+
+Parameters (per bus / branch):
+  r ~ U(0.01, 0.08) pu, x = r * U(2, 8), line charging b ~ U(0, 0.05) pu;
+  1 reference slack bus (|V| = 1.0), PV buses with Vset ~ U(1.00, 1.06), PQ
+  buses with P ~ U(0.05, 0.4) pu and Q = P * U(0.2, 0.4). PV active generation
+  is split evenly and sized to cover ``gen_share`` of the load (0.9 by default).
+
+Balancing power. On the 14- and 118-bus grids the single slack bus supplies
+the rest of the load plus losses. On 2000+ buses these line parameters give
+losses of tens of pu that no single bus can deliver, so those grids use the
+reference's distributed-slack model (BaseNRAlgo with several slack buses,
+BaseAlgo.hpp:201-216): the reference slack bus and every PV generator share
+the balancing power with equal weights. Even then full loading collapses on
+long radial feeders, so those grids also carry a global loading factor
+(``loading``: 0.35 at 2000 buses, 0.25 at 10000 buses) on loads and dispatch;
+at 2000 buses 2111 of the 2128 non-bridge N-1 outages then converge.
+
+Branch stamps follow the reference pi-model (LineContainer.cpp:13-34 splits h
+evenly; TwoSidesContainer_rxh_A.hpp:921-940): ys = 1/(r + jx),
+yff = ytt = ys + j*b/2, yft = ytf = -ys.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import scipy.sparse as sp
+
+
+@dataclasses.dataclass
+class Grid:
+    n: int
+    f: np.ndarray           # branch from bus
+    t: np.ndarray           # branch to bus
+    r: np.ndarray
+    x: np.ndarray
+    b: np.ndarray           # total line charging susceptance (pu)
+    slack: int
+    pv: np.ndarray
+    pq: np.ndarray
+    vset: np.ndarray        # |V| setpoint per bus (1.0 for PQ)
+    load_p: np.ndarray      # per bus (pu), 0 on non-PQ buses
+    load_q: np.ndarray
+    gen_p: np.ndarray       # per bus (pu) PV dispatch, 0 elsewhere
+    pos: np.ndarray         # bus coordinates (for reproducibility / plots)
+    distributed_slack: bool = False
+    loading: float = 1.0    # global factor applied to loads and dispatch in sbus()
+
+    @property
+    def nbr(self) -> int:
+        return len(self.f)
+
+    def branch_admittances(self):
+        ys = 1.0 / (self.r + 1j * self.x)
+        yff = ys + 0.5j * self.b
+        return yff, yff.copy(), -ys, -ys  # yff, ytt, yft, ytf
+
+    def ybus(self) -> sp.csc_matrix:
+        """Complex Ybus (CSC, sorted indices, no duplicates)."""
+        yff, ytt, yft, ytf = self.branch_admittances()
+        rows = np.concatenate([self.f, self.t, self.f, self.t])
+        cols = np.concatenate([self.f, self.t, self.t, self.f])
+        vals = np.concatenate([yff, ytt, yft, ytf])
+        Y = sp.coo_matrix((vals, (rows, cols)), shape=(self.n, self.n)).tocsc()
+        Y.sum_duplicates()
+        Y.sort_indices()
+        return Y
+
+    def sbus(self, load_scale: np.ndarray | float = 1.0, gen_scale: np.ndarray | float = 1.0) -> np.ndarray:
+        return self.loading * (self.gen_p * gen_scale - (self.load_p + 1j * self.load_q) * load_scale)
+
+    def v0(self) -> np.ndarray:
+        """Flat start with PV/slack magnitudes at their setpoints."""
+        return self.vset.astype(np.complex128)
+
+    @property
+    def slack_ids(self) -> np.ndarray:
+        """Slack buses in the reference's convention: the reference slack first, then
+        the other participating buses (treated as PV buses by the NR)."""
+        if self.distributed_slack:
+            return np.r_[self.slack, self.pv].astype(np.int32)
+        return np.array([self.slack], np.int32)
+
+    def slack_weights(self) -> np.ndarray:
+        w = np.zeros(self.n)
+        ids = self.slack_ids
+        w[ids] = 1.0 / len(ids)
+        return w
+
+    @property
+    def pv_solver(self) -> np.ndarray:
+        """PV buses passed to the solver (buses already in slack_ids are excluded)."""
+        if self.distributed_slack:
+            return np.zeros(0, np.int32)
+        return self.pv
+
+    def solver_inputs(self, sbus=None, v0=None) -> dict:
+        """Arguments of compute_pf(Ybus, V, Sbus, slack_ids, slack_weights, pv, pq, ...)."""
+        return dict(Ybus=self.ybus(), V0=self.v0() if v0 is None else v0,
+                    Sbus=self.sbus() if sbus is None else sbus, slack_ids=self.slack_ids,
+                    slack_weights=self.slack_weights(), pv=self.pv_solver, pq=self.pq)
+
+    def ybus_deltas(self, branches) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Per-contingency Ybus value deltas on the base CSC pattern (explicit zeros kept),
+        as ContingencyAnalysis::init_li_coeffs builds them (ContingencyAnalysis.cpp:47-110):
+        4 coefficients per outaged branch (ff, tt, ft, tf). ``branches`` is a list of
+        branch-id lists. Returns (dptr, dpos, dval) with Ybus.data[dpos] -= dval."""
+        Y = self.ybus()
+        yff, ytt, yft, ytf = self.branch_admittances()
+
+        def pos(r, c):
+            s, e = Y.indptr[c], Y.indptr[c + 1]
+            k = s + np.searchsorted(Y.indices[s:e], r)
+            assert Y.indices[k] == r
+            return k
+
+        dptr = [0]
+        dpos, dval = [], []
+        for cont in branches:
+            for br in cont:
+                f, t = int(self.f[br]), int(self.t[br])
+                for (rr, cc, v) in ((f, f, yff[br]), (t, t, ytt[br]), (f, t, yft[br]), (t, f, ytf[br])):
+                    dpos.append(pos(rr, cc))
+                    dval.append(v)
+            dptr.append(len(dpos))
+        return (np.asarray(dptr, np.int32), np.asarray(dpos, np.int32), np.asarray(dval, np.complex128))
+
+
+def generate(n: int, nbr: int, npv: int, seed: int, gen_share: float = 0.9, distributed_slack: bool = False,
+             loading: float = 1.0) -> Grid:
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 2))
+    order = rng.permutation(n)
+    edges = set()
+    # spanning tree with locality: attach to one of the 3 nearest placed buses
+    grid_cells = {}
+    cell = max(1, int(np.sqrt(n) / 2))
+
+    def cell_of(p):
+        return (min(cell - 1, int(p[0] * cell)), min(cell - 1, int(p[1] * cell)))
+
+    placed = []
+    for k, a in enumerate(order):
+        if k > 0:
+            # search growing rings of cells until at least 3 candidates
+            cx, cy = cell_of(pos[a])
+            ring, cand = 0, []
+            while len(cand) < min(3, k) or ring == 0:
+                cand = []
+                for ix in range(cx - ring, cx + ring + 1):
+                    for iy in range(cy - ring, cy + ring + 1):
+                        cand.extend(grid_cells.get((ix, iy), ()))
+                ring += 1
+                if ring > cell + 1:
+                    break
+            cand = np.asarray(cand)
+            d = np.sum((pos[cand] - pos[a]) ** 2, axis=1)
+            kk = min(3, len(cand))
+            near = cand[np.argsort(d, kind="stable")[:kk]]
+            b = int(near[rng.integers(0, kk)])
+            edges.add((min(a, b), max(a, b)))
+        grid_cells.setdefault(cell_of(pos[a]), []).append(int(a))
+        placed.append(int(a))
+    tree = cKDTree(pos)
+    kq = min(7, n)
+    guard = 0
+    while len(edges) < nbr:
+        a = int(rng.integers(0, n))
+        _, nb = tree.query(pos[a], k=kq)
+        b = int(nb[rng.integers(1, kq)])
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+        guard += 1
+        if guard > 100 * nbr:
+            raise RuntimeError("gridgen: cannot place the requested number of distinct branches")
+    E = np.array(sorted(edges), dtype=np.int64)
+    E = E[rng.permutation(len(E))]
+    m = len(E)
+    r = rng.uniform(0.01, 0.08, m)
+    x = r * rng.uniform(2.0, 8.0, m)
+    bch = rng.uniform(0.0, 0.05, m)
+    slack = int(rng.integers(0, n))
+    others = np.setdiff1d(np.arange(n), [slack])
+    pv = np.sort(rng.choice(others, npv, replace=False))
+    pq = np.setdiff1d(others, pv)
+    vset = np.ones(n)
+    vset[pv] = rng.uniform(1.00, 1.06, npv)
+    load_p = np.zeros(n)
+    load_q = np.zeros(n)
+    load_p[pq] = rng.uniform(0.05, 0.4, len(pq))
+    load_q[pq] = load_p[pq] * rng.uniform(0.2, 0.4, len(pq))
+    gen_p = np.zeros(n)
+    if npv:
+        gen_p[pv] = gen_share * load_p.sum() / npv
+    return Grid(n=n, f=E[:, 0].astype(np.int32), t=E[:, 1].astype(np.int32), r=r, x=x, b=bch, slack=slack,
+                pv=pv.astype(np.int32), pq=pq.astype(np.int32), vset=vset, load_p=load_p, load_q=load_q,
+                gen_p=gen_p, pos=pos, distributed_slack=distributed_slack, loading=loading)
+
+
+def bridges(grid: Grid) -> np.ndarray:
+    """Boolean mask of bridge branches (their outage islands the grid), Tarjan's low-link."""
+    n = grid.n
+    adj = [[] for _ in range(n)]
+    for k, (a, b) in enumerate(zip(grid.f, grid.t)):
+        adj[a].append((b, k))
+        adj[b].append((a, k))
+    disc = [-1] * n
+    low = [0] * n
+    is_bridge = np.zeros(grid.nbr, bool)
+    timer = 0
+    for root in range(n):
+        if disc[root] >= 0:
+            continue
+        stack = [(root, -1, 0)]
+        disc[root] = low[root] = timer
+        timer += 1
+        while stack:
+            v, pe, i = stack[-1]
+            if i < len(adj[v]):
+                stack[-1] = (v, pe, i + 1)
+                w, k = adj[v][i]
+                if k == pe:
+                    continue
+                if disc[w] < 0:
+                    disc[w] = low[w] = timer
+                    timer += 1
+                    stack.append((w, k, 0))
+                else:
+                    low[v] = min(low[v], disc[w])
+            else:
+                stack.pop()
+                if stack:
+                    u = stack[-1][0]
+                    low[u] = min(low[u], low[v])
+                    if low[v] > disc[u]:
+                        is_bridge[pe] = True
+    return is_bridge
+
+
+# Workloads named by what they are (bus count, branch count, PV count, seed).
+CONFIGS = {
+    "case14_synth": dict(n=14, nbr=20, npv=4, seed=14),
+    "case118_synth": dict(n=118, nbr=186, npv=53, seed=118),
+    "case2000_synth": dict(n=2000, nbr=2700, npv=400, seed=2000, distributed_slack=True, loading=0.35),
+    "case10000_synth": dict(n=10000, nbr=13000, npv=2500, seed=10000, distributed_slack=True, loading=0.25),
+}
+
+
+def make(name: str, **kw) -> Grid:
+    cfg = dict(CONFIGS[name])
+    cfg.update(kw)
+    return generate(**cfg)
+
+
+def time_series_sbus(grid: Grid, T: int = 8760, seed: int = 1) -> np.ndarray:
+    """Hourly injections: base loads x daily sinusoid (+-25%) x seasonal sinusoid
+    (+-15%) x lognormal noise (sigma 0.05); PV dispatch rescaled per step so that
+    it covers the same share of the step's load as in the base case."""
+    rng = np.random.default_rng(seed)
+    h = np.arange(T)
+    daily = 1.0 + 0.25 * np.sin(2 * np.pi * h / 24.0)
+    season = 1.0 + 0.15 * np.sin(2 * np.pi * h / 8760.0)
+    noise = rng.lognormal(0.0, 0.05, (T, grid.n))
+    scale = (daily * season)[:, None] * noise
+    P = grid.load_p[None, :] * scale
+    Q = grid.load_q[None, :] * scale
+    base_load = grid.load_p.sum()
+    gen_scale = P.sum(axis=1) / base_load if base_load > 0 else np.ones(T)
+    return grid.loading * (grid.gen_p[None, :] * gen_scale[:, None] - (P + 1j * Q))

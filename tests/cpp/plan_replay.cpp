@@ -1,0 +1,138 @@
+// Test-only: serial CPU replay of the device plan built by the product's host
+// analysis (lightsim2grid_amd/csrc/analysis.cpp). It executes the same phases,
+// slots, tasks and pairs as the HIP kernel, one instance at a time, so that
+// the schedule can be checked against the oracle on machines without a GPU.
+// It is compiled by tests/conftest.py into tests/cpp/_build/ and is never
+// part of the product library.
+#include "../../lightsim2grid_amd/csrc/analysis.hpp"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <set>
+#include <vector>
+
+using namespace pfgpu;
+
+extern "C" {
+
+// Returns iterations (>= 0) or a negative value on analysis failure. status_out: 0 ok, 2 max iter, 3 non-finite.
+// Also reports write-write / read-before-write conflicts found in the level schedule (conflicts_out).
+int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const double* Sbus,
+                 const int* slack_ids, int nslack, const double* w, const int* pv, int npv, const int* pq, int npq,
+                 int single_slack, double* V, int max_iter, double tol, int use_fp32,
+                 int* status_out, int* conflicts_out, double* Jx_out) {
+    YbusPattern Y;
+    Y.n = n;
+    Y.p.assign(Yp, Yp + n + 1);
+    Y.i.assign(Yi, Yi + Yp[n]);
+    JPattern J = build_jacobian_pattern(Y, slack_ids, nslack, w, pv, npv, pq, npq, single_slack != 0);
+    SymbolicLU S = symbolic_lu(J);
+    Schedule sch = build_schedule(S);
+    DevicePlan P = build_device_plan(Y, J, S, sch);
+    const cplx* Yc = reinterpret_cast<const cplx*>(Yx);
+    const cplx* Sc = reinterpret_cast<const cplx*>(Sbus);
+    std::vector<cplx> Vc(reinterpret_cast<cplx*>(V), reinterpret_cast<cplx*>(V) + n);
+    std::vector<double> Va(n), Vm(n);
+    for (int b = 0; b < n; ++b) { Va[b] = std::arg(Vc[b]); Vm[b] = std::abs(Vc[b]); }
+    std::vector<cplx> Ib(n);
+    std::vector<double> X(P.nvals, 0.0);
+    double slack_abs = 0;
+    for (int b = 0; b < n; ++b) slack_abs += Sc[b].real();
+    auto rnd = [&](double v) { return use_fp32 ? (double)(float)v : v; };
+
+    // schedule conflict check: every slot read by a level must have been written at an earlier level
+    // (or be a y slot / J value read by the task itself), and no slot written twice
+    {
+        std::vector<int> wlev(P.nvals, -1);
+        int conflicts = 0;
+        for (int l = 0; l < P.nlev; ++l)
+            for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
+                if (wlev[P.t_out[t]] >= 0) ++conflicts;
+                wlev[P.t_out[t]] = l;
+            }
+        for (int l = 0; l < P.nlev; ++l)
+            for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t)
+                for (int p = P.t_pbeg[t]; p < P.t_pbeg[t + 1]; ++p)
+                    if (wlev[P.p_l[p]] >= l || wlev[P.p_u[p]] >= l || wlev[P.p_d[p]] >= l) ++conflicts;
+        *conflicts_out = conflicts;
+    }
+
+    auto mismatch = [&](double& fmax) {
+        fmax = 0;
+        for (int r = 0; r < n; ++r) {
+            cplx acc(0, 0);
+            for (int q = P.yr_ptr[r]; q < P.yr_ptr[r + 1]; ++q) acc += Yc[P.yr_pos[q]] * Vc[P.yr_col[q]];
+            Ib[r] = acc;
+            cplx mis = Vc[r] * std::conj(acc) - Sc[r];
+            if (!single_slack) mis += slack_abs * w[r];
+            if (P.fslot_p[r] >= 0) { X[P.fslot_p[r]] = rnd(mis.real()); fmax = std::max(fmax, std::fabs(mis.real())); if (!std::isfinite(mis.real())) fmax = NAN; }
+            if (P.fslot_q[r] >= 0) { X[P.fslot_q[r]] = rnd(mis.imag()); fmax = std::max(fmax, std::fabs(mis.imag())); if (!std::isfinite(mis.imag())) fmax = NAN; }
+        }
+    };
+    auto jval = [&](int q) -> double {
+        const int sel = P.j_sel[q];
+        if (sel == SEL_CONST) return P.j_cval[q];
+        const int r = P.j_rbus[q], c = P.j_cbus[q];
+        const cplx y = Yc[P.j_ypos[q]];
+        if (sel >= SEL_DVM_RE) {
+            cplx vn = Vc[c] / std::abs(Vc[c]);
+            cplx d = std::conj(y * vn) * Vc[r];
+            if (r == c) d += std::conj(Ib[r]) * (Vc[r] / std::abs(Vc[r]));
+            return sel == SEL_DVM_RE ? d.real() : d.imag();
+        }
+        cplx t = y * Vc[c];
+        if (r == c) t -= Ib[r];
+        cplx d = std::conj(-t) * (cplx(0, 1) * Vc[r]);
+        return sel == SEL_DVA_RE ? d.real() : d.imag();
+    };
+    double fmax;
+    mismatch(fmax);
+    bool conv = fmax < tol;
+    int it = 0, status = 0;
+    while (!conv && it < max_iter) {
+        ++it;
+        for (int l = 0; l < P.nlev; ++l)
+            for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
+                const int o = P.t_out[t];
+                const int kind = P.t_kind[t];
+                double a = 0.0;
+                if (kind == 2) a = X[o];  // F already in place
+                else if (P.t_jsrc[t] >= 0) {
+                    a = rnd(jval(P.t_jsrc[t]));
+                    if (Jx_out) Jx_out[P.t_jsrc[t]] = jval(P.t_jsrc[t]);
+                }
+                double acc = 0;
+                for (int p = P.t_pbeg[t]; p < P.t_pbeg[t + 1]; ++p) acc += X[P.p_l[p]] * X[P.p_u[p]] * X[P.p_d[p]];
+                double v = rnd(a - acc);
+                X[o] = kind == 1 ? rnd(1.0 / v) : v;
+            }
+        for (int l = 0; l < P.nblev; ++l)
+            for (int t = P.b_lvl[l]; t < P.b_lvl[l + 1]; ++t) {
+                double acc = 0;
+                for (int p = P.bt_pbeg[t]; p < P.bt_pbeg[t + 1]; ++p) acc += X[P.bp_u[p]] * X[P.bp_x[p]];
+                X[P.bt_out[t]] = rnd((X[P.bt_out[t]] - acc) * X[P.bt_diag[t]]);
+            }
+        if (single_slack)
+            for (int b = 0; b < n; ++b) { Va[b] = std::arg(Vc[b]); Vm[b] = std::abs(Vc[b]); }
+        for (int j = 0; j < P.N; ++j) {
+            const double dx = X[P.ux_slot[j]];
+            const int b = P.ux_bus[j];
+            if (P.ux_kind[j] == 0) slack_abs -= dx;
+            else if (P.ux_kind[j] == 1) Va[b] -= dx;
+            else Vm[b] -= dx;
+        }
+        double vmin = 1e300;
+        for (int b = 0; b < n; ++b) { Vc[b] = Vm[b] * cplx(std::cos(Va[b]), std::sin(Va[b])); vmin = std::min(vmin, Vm[b]); }
+        if (vmin < 0) for (int b = 0; b < n; ++b) { Vm[b] = std::abs(Vc[b]); Va[b] = std::arg(Vc[b]); }
+        mismatch(fmax);
+        if (!std::isfinite(fmax)) { status = 3; break; }
+        conv = fmax < tol;
+    }
+    if (!conv && status == 0) status = 2;
+    for (int b = 0; b < n; ++b) { V[2 * b] = Vc[b].real(); V[2 * b + 1] = Vc[b].imag(); }
+    *status_out = status;
+    return it;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle.
+
+Bar (per instance): same converged flag, NR iterations within +-1 of the
+oracle (FP64 form: equal), max |V - V_ref| <= 1e-8 pu. The FP32-factor form
+keeps the mismatch, update and convergence test in FP64 and must meet the
+same V bound (its converged V satisfies the same 1e-8 mismatch test).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+V_TOL = 1e-8
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _data(name):
+    if name.endswith("_synth"):
+        from lightsim2grid_amd import gridgen
+        return gridgen.make(name).solver_inputs()
+    return load_fixture(name)
+
+
+@pytest.mark.parametrize("name", list(FIXTURES) + ["case14_synth", "case118_synth", "case2000_synth"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_single_solve_matches_oracle(gpu_ctx, oracle_mod, name, prec):
+    from lightsim2grid_amd import abi
+    d = _data(name)
+    singles = [False, True] if len(np.atleast_1d(d["slack_ids"])) == 1 else [False]
+    for single in singles:
+        _say(name, "single" if single else "dist", "fp32" if prec else "fp64")
+        o = oracle_mod.OracleNR(single)
+        conv = o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"],
+                            10, 1e-8)
+        g = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], single_slack=single)
+        r = g.solve(g.ybus_values(d["Ybus"]), d["Sbus"], d["V0"], 10, 1e-8, precision=prec, want_J=True)
+        assert (r["status"][0] == 0) == conv
+        if prec == 0:
+            assert r["iters"][0] == o.get_nb_iter()
+        else:
+            assert abs(int(r["iters"][0]) - o.get_nb_iter()) <= 1
+        assert np.max(np.abs(r["V"][0] - o.get_V())) <= V_TOL
+        if prec == 0 and name in FIXTURES:
+            # the last J (reference layout) equals the oracle's to rounding, and the fixture's J to 1e-4 L1
+            Jg = r["J"].toarray()
+            Jo = o.get_J().toarray()
+            assert np.max(np.abs(Jg - Jo)) <= 1e-9 * max(1.0, np.abs(Jo).max())
+            Jc = Jg if single else Jg[1:, 1:]
+            assert np.abs(Jc - d["J"]).sum() <= 1e-4
+
+
+def _n1(grid_name, max_cont=None, seed=0):
+    from lightsim2grid_amd import gridgen
+    g = gridgen.make(grid_name)
+    nb = np.nonzero(~gridgen.bridges(g))[0]
+    if max_cont is not None and len(nb) > max_cont:
+        nb = np.sort(np.random.default_rng(seed).choice(nb, max_cont, replace=False))
+    conts = [[int(k)] for k in nb]
+    return g, conts, g.ybus_deltas(conts)
+
+
+@pytest.mark.parametrize("grid_name,max_cont", [("case118_synth", None), ("case2000_synth", 256)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_contingency_batch_matches_oracle(gpu_ctx, oracle_mod, grid_name, max_cont, prec):
+    from lightsim2grid_amd import abi
+    g, conts, deltas = _n1(grid_name, max_cont)
+    d = g.solver_inputs()
+    _say("N-1", grid_name, len(conts), "fp32" if prec else "fp64")
+    # base case, then every contingency warm-started from the base V (init_from_n_powerflow)
+    o = oracle_mod.OracleNR(False)
+    assert o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    Vb = o.get_V()
+    Vr, itr, errr, convr, conn, _ = oracle_mod.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"],
+                                                           d["pv"], d["pq"], Vb, deltas, 10, 1e-8, False, False, 4)
+    assert conn.all()
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    r = gr.solve(gr.ybus_values(d["Ybus"]), d["Sbus"], Vb, 10, 1e-8, precision=prec, deltas=deltas)
+    convg = r["status"] == 0
+    assert np.array_equal(convg, convr)
+    tol_it = 0 if prec == 0 else 1
+    assert np.max(np.abs(r["iters"][convr] - itr[convr])) <= tol_it
+    assert np.max(np.abs(r["V"][convr] - Vr[convr])) <= V_TOL
+
+
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16, 32])
+def test_instances_per_workgroup_do_not_change_results(gpu_ctx, G):
+    from lightsim2grid_amd import abi
+    g, conts, deltas = _n1("case118_synth")
+    d = g.solver_inputs()
+    _say("G sweep", G)
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    ref = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8, deltas=deltas, instances_per_wg=1)
+    r = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8, deltas=deltas, instances_per_wg=G)
+    assert r["stats"]["instances_per_wg"] == G
+    assert np.array_equal(r["status"], ref["status"])
+    assert np.array_equal(r["iters"], ref["iters"])
+    assert np.max(np.abs(r["V"] - ref["V"])) <= 1e-12
+
+
+def test_lds_spill_path(gpu_ctx, oracle_mod):
+    """The 2000-bus FP64 value array (~42.5k slots, 340 KB) exceeds LDS: the
+    coldest slots go to the global spill buffer."""
+    from lightsim2grid_amd import abi
+    from lightsim2grid_amd import gridgen
+    d = gridgen.make("case2000_synth").solver_inputs()
+    _say("spill case2000 fp64")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    r = gr.solve(gr.ybus_values(d["Ybus"]), d["Sbus"], d["V0"], 10, 1e-8, precision=0, instances_per_wg=1)
+    assert r["stats"]["lds_slots"] < gr.info["n_slots"]
+    o = oracle_mod.OracleNR(False)
+    o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    assert r["status"][0] == 0 and r["iters"][0] == o.get_nb_iter()
+    assert np.max(np.abs(r["V"][0] - o.get_V())) <= V_TOL
+
+
+def test_error_statuses(gpu_ctx):
+    from lightsim2grid_amd import abi
+    d = load_fixture("case14")
+    _say("error statuses")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    r = gr.solve(Yx, d["Sbus"], d["V0"], 1, 1e-8)
+    assert r["status"][0] == 2 and r["iters"][0] == 1  # TooManyIterations
+    S = d["Sbus"].copy()
+    S[3] = np.nan
+    r = gr.solve(Yx, S, d["V0"], 10, 1e-8)
+    assert r["status"][0] == 3  # InifiniteValue
+    r = gr.solve(Yx, d["Sbus"], d["V0"], 0, 1e-8)
+    assert r["status"][0] == 2 and r["iters"][0] == 0
+    # already converged start: zero iterations
+    ok = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8)
+    r = gr.solve(Yx, d["Sbus"], ok["V"][0], 10, 1e-8)
+    assert r["status"][0] == 0 and r["iters"][0] == 0

@@ -1,0 +1,128 @@
+"""The oracle pinned on the reference's golden vectors and on closed-form checks.
+
+Reference test this mirrors: lightsim2grid/tests/test_KLUSolver.py:76-114 and
+test_NewtonPF.py:67-119 (max_it=10, tol=1e-8, J blocks and Va/Vm compared in
+L1 at 1e-4, iteration count = the k of J_k.npy / V_k.npy).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import FIXTURES, load_fixture, solver_args
+
+TOL_TEST = 1e-4
+
+
+def _blocks_l1(J, Jpp, npvpq):
+    d = np.abs(J - Jpp)
+    return (d[:npvpq, :npvpq].sum(), d[npvpq:, :npvpq].sum(), d[:npvpq, npvpq:].sum(), d[npvpq:, npvpq:].sum())
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+@pytest.mark.parametrize("single", [False, True])
+def test_oracle_matches_reference_fixture(oracle_mod, name, single):
+    d = load_fixture(name)
+    s = oracle_mod.OracleNR(single_slack=single)
+    assert s.compute_pf(*solver_args(d), 10, 1e-8)
+    assert s.get_nb_iter() == d["iters"] == FIXTURES[name]
+    J = s.get_J().toarray()
+    Jc = J if single else J[1:, 1:]
+    npvpq = len(d["pv"]) + len(d["pq"])
+    for blk in _blocks_l1(Jc, d["J"], npvpq):
+        assert blk <= TOL_TEST
+    assert np.abs(s.get_Va() - np.angle(d["V"])).sum() <= TOL_TEST
+    assert np.abs(s.get_Vm() - np.abs(d["V"])).sum() <= TOL_TEST
+    assert s.get_error() == 0
+
+
+def _F(Y, V, S, pv, pq):
+    mis = V * np.conj(Y @ V) - S
+    return np.r_[mis.real[pv], mis.real[pq], mis.imag[pq]]
+
+
+@pytest.mark.parametrize("name", ["case14", "case30"])
+def test_oracle_jacobian_is_derivative_of_mismatch(oracle_mod, name):
+    """Single-slack J equals the central finite-difference Jacobian of F(Va, Vm)."""
+    d = load_fixture(name)
+    s = oracle_mod.OracleNR(single_slack=True)
+    s.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 1, 1e-30)
+    J = s.get_J().toarray()  # built at V0 (first iteration)
+    Y = d["Ybus"].toarray()
+    pv, pq = d["pv"], d["pq"]
+    pvpq = np.r_[pv, pq]
+    Va0, Vm0 = np.angle(d["V0"]), np.abs(d["V0"])
+    h = 1e-6
+    cols = []
+    for k, b in enumerate(np.r_[pvpq, pq]):
+        is_va = k < len(pvpq)
+        Fp, Fm = [], []
+        for sgn in (1, -1):
+            Va, Vm = Va0.copy(), Vm0.copy()
+            if is_va:
+                Va[b] += sgn * h
+            else:
+                Vm[b] += sgn * h
+            (Fp if sgn == 1 else Fm).append(_F(Y, Vm * np.exp(1j * Va), d["Sbus"], pv, pq))
+        cols.append((Fp[0] - Fm[0]) / (2 * h))
+    Jfd = np.array(cols).T
+    assert np.max(np.abs(J - Jfd)) < 1e-5 * max(1.0, np.abs(J).max())
+
+
+@pytest.mark.parametrize("name", ["case14", "case118"])
+def test_distributed_J_is_single_J_bordered(oracle_mod, name):
+    """With one slack of weight 1, J_dist = [[w0, row0], [e0-column, J_single]]."""
+    d = load_fixture(name)
+    sd = oracle_mod.OracleNR(False)
+    ss = oracle_mod.OracleNR(True)
+    sd.compute_pf(*solver_args(d), 1, 1e-30)
+    ss.compute_pf(*solver_args(d), 1, 1e-30)
+    Jd = sd.get_J().toarray()
+    Js = ss.get_J().toarray()
+    assert np.array_equal(Jd[1:, 1:], Js)
+    assert Jd[0, 0] == 1.0 and np.all(Jd[1:, 0] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["case14", "case30", "case300"])
+def test_oracle_lu_reconstructs_and_solves(oracle_mod, name):
+    d = load_fixture(name)
+    s = oracle_mod.OracleNR(single_slack=True)
+    s.compute_pf(*solver_args(d), 2, 1e-30)  # iteration 1 = factor, iteration 2 = refactor
+    J = s.get_J().toarray()
+    P, Q, L, U = s.get_lu_dense()
+    # P * J(:, Q) = L U  with P the row pivot sequence
+    assert np.max(np.abs(J[P][:, Q] - L @ U)) < 1e-10 * np.abs(J).max()
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(J.shape[0])
+    x, err = oracle_mod.lu_solve(sp.csc_matrix(J), b)
+    assert err == 0
+    xs = spla.spsolve(sp.csc_matrix(J), b)
+    assert np.max(np.abs(x - xs)) < 1e-9 * max(1.0, np.abs(xs).max())
+
+
+def test_oracle_lu_brute_force_tiny():
+    """Tiny dense systems against numpy.linalg.solve (brute force)."""
+    from oracle import oracle
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 5, 8):
+        A = rng.standard_normal((n, n)) + n * np.eye(n)
+        A[rng.random((n, n)) < 0.3] = 0.0
+        np.fill_diagonal(A, np.diag(A) + 0.5)
+        b = rng.standard_normal(n)
+        x, err = oracle.lu_solve(sp.csc_matrix(A), b)
+        assert err == 0
+        assert np.allclose(x, np.linalg.solve(A, b), atol=1e-12, rtol=1e-10)
+
+
+def test_oracle_error_paths(oracle_mod):
+    d = load_fixture("case14")
+    s = oracle_mod.OracleNR(False)
+    assert not s.compute_pf(*solver_args(d), 1, 1e-8)
+    assert s.get_error() == 2  # TooManyIterations
+    S = d["Sbus"].copy()
+    S[3] = np.nan
+    s2 = oracle_mod.OracleNR(False)
+    args = list(solver_args(d))
+    args[2] = S
+    assert not s2.compute_pf(*args, 10, 1e-8)
+    assert s2.get_error() == 3  # InifiniteValue

@@ -1,0 +1,61 @@
+"""CPU replay of the device plan (the exact slots, tasks and pairs the HIP
+kernel executes) against the oracle. Pins the host analysis (J pattern,
+scatter map, ordering, symbolic LU, level schedule) without a GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import FIXTURES, build_replay, load_fixture, solver_args
+
+_i32 = ctypes.POINTER(ctypes.c_int)
+_f64 = ctypes.POINTER(ctypes.c_double)
+
+
+def replay(d, single, fp32, max_iter=10, tol=1e-8, want_J=False):
+    L = ctypes.CDLL(build_replay())
+    Y = sp.csc_matrix(d["Ybus"])
+    Y.sort_indices()
+    n = Y.shape[0]
+    a = lambda x, t: np.ascontiguousarray(x, dtype=t)
+    Yp, Yi, Yx = a(Y.indptr, np.int32), a(Y.indices, np.int32), a(Y.data, np.complex128)
+    S = a(d["Sbus"], np.complex128)
+    sl = a(np.atleast_1d(d["slack_ids"]), np.int32)
+    w = a(d["slack_weights"], np.float64)
+    pv, pq = a(d["pv"], np.int32), a(d["pq"], np.int32)
+    V = a(d["V0"], np.complex128).copy()
+    st, cf = ctypes.c_int(), ctypes.c_int()
+    L.replay_solve.restype = ctypes.c_int
+    it = L.replay_solve(n, Yp.ctypes.data_as(_i32), Yi.ctypes.data_as(_i32), Yx.ctypes.data_as(_f64),
+                        S.ctypes.data_as(_f64), sl.ctypes.data_as(_i32), len(sl), w.ctypes.data_as(_f64),
+                        pv.ctypes.data_as(_i32), len(pv), pq.ctypes.data_as(_i32), len(pq), int(single),
+                        V.ctypes.data_as(_f64), int(max_iter), ctypes.c_double(tol), int(fp32), ctypes.byref(st),
+                        ctypes.byref(cf), None)
+    return it, st.value, cf.value, V
+
+
+CASES = list(FIXTURES) + ["case14_synth", "case118_synth", "case2000_synth"]
+
+
+def _data(name):
+    if name.endswith("_synth"):
+        from lightsim2grid_amd import gridgen
+        return gridgen.make(name).solver_inputs()
+    return load_fixture(name)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_replay_matches_oracle(oracle_mod, name, fp32):
+    d = _data(name)
+    singles = [False, True] if len(np.atleast_1d(d["slack_ids"])) == 1 else [False]
+    for single in singles:
+        o = oracle_mod.OracleNR(single)
+        conv = o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"],
+                            10, 1e-8)
+        it, st, conflicts, V = replay(d, single, fp32)
+        assert conflicts == 0
+        assert (st == 0) == conv
+        assert abs(it - o.get_nb_iter()) <= (0 if not fp32 else 1)
+        assert np.max(np.abs(V - o.get_V())) <= 1e-8
