@@ -1,0 +1,702 @@
+// Host-side mirror of lightsim2grid's solver surface (see host.hpp).
+#include "host.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <queue>
+#include <sstream>
+
+namespace ls2g {
+
+namespace {
+using clk = std::chrono::steady_clock;
+double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+constexpr double kTolEqualFloat = 1e-7;  // BaseConstants.cpp:22
+}  // namespace
+
+std::string solver_type_name(SolverType t) {
+    static const char* names[] = {
+        "SparseLU", "KLU", "GaussSeidel", "DC", "GaussSeidelSynch", "NICSLU", "SparseLUSingleSlack",
+        "KLUSingleSlack", "NICSLUSingleSlack", "KLUDC", "NICSLUDC", "CKTSO", "CKTSOSingleSlack", "CKTSODC",
+        "FDPF_XB_SparseLU", "FDPF_BX_SparseLU", "FDPF_XB_KLU", "FDPF_BX_KLU", "FDPF_XB_NICSLU", "FDPF_BX_NICSLU",
+        "FDPF_XB_CKTSO", "FDPF_BX_CKTSO", "AMDGPU", "AMDGPUSingleSlack", "AMDGPUMixed", "AMDGPUMixedSingleSlack"};
+    const int k = (int)t;
+    return (k >= 0 && k < (int)(sizeof(names) / sizeof(names[0]))) ? names[k] : "Unknown";
+}
+
+bool is_gpu_type(SolverType t) {
+    return t == SolverType::AMDGPU || t == SolverType::AMDGPUSingleSlack || t == SolverType::AMDGPUMixed ||
+           t == SolverType::AMDGPUMixedSingleSlack;
+}
+
+void check_rc(int rc, const char* what) {
+    if (rc != 0) {
+        std::ostringstream os;
+        os << what << ": pfgpu error " << rc << " (" << pfgpu_last_error() << ")";
+        throw std::runtime_error(os.str());
+    }
+}
+
+pfgpu_ctx* engine_ctx() {
+    static std::once_flag once;
+    static pfgpu_ctx* ctx = nullptr;
+    static std::string err;
+    std::call_once(once, [] {
+        if (pfgpu_ctx_create(0, 0, &ctx) != 0) {
+            err = pfgpu_last_error();
+            ctx = nullptr;
+        }
+    });
+    if (!ctx) throw std::runtime_error("AMDGPU solver: no usable GPU (" + err + ")");
+    return ctx;
+}
+
+// ---------------------------------------------------------------------------
+// GpuNRSolver
+// ---------------------------------------------------------------------------
+GpuNRSolver::~GpuNRSolver() {
+    if (grid_) pfgpu_grid_destroy(grid_);
+}
+
+void GpuNRSolver::reset() {
+    if (grid_) pfgpu_grid_destroy(grid_);
+    grid_ = nullptr;
+    key_p_.clear();
+    V_.clear();
+    Jx_.clear();
+    nr_iter_ = 0;
+    err_ = ErrorType::NotInitError;
+}
+
+RealVect GpuNRSolver::get_Va() const {
+    RealVect r(V_.size());
+    for (size_t b = 0; b < V_.size(); ++b) r[b] = std::arg(V_[b]);
+    return r;
+}
+RealVect GpuNRSolver::get_Vm() const {
+    RealVect r(V_.size());
+    for (size_t b = 0; b < V_.size(); ++b) r[b] = std::abs(V_[b]);
+    return r;
+}
+
+void GpuNRSolver::ensure_grid(const CscC& Y, const IntVect& slack_ids, const RealVect& w, const IntVect& pv,
+                              const IntVect& pq) {
+    if (grid_ && Y.p == key_p_ && Y.i == key_i_ && slack_ids == key_slack_ && pv == key_pv_ && pq == key_pq_ &&
+        w == key_w_)
+        return;
+    auto t0 = clk::now();
+    if (grid_) pfgpu_grid_destroy(grid_);
+    grid_ = nullptr;
+    pfgpu_grid* g = nullptr;
+    check_rc(pfgpu_grid_load(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(), (int)slack_ids.size(),
+                             w.data(), pv.data(), (int)pv.size(), pq.data(), (int)pq.size(), single_ ? 1 : 0, &g),
+             "pfgpu_grid_load");
+    grid_ = g;
+    check_rc(pfgpu_grid_get_info(grid_, &info_), "pfgpu_grid_get_info");
+    key_p_ = Y.p; key_i_ = Y.i; key_slack_ = slack_ids; key_pv_ = pv; key_pq_ = pq; key_w_ = w;
+    timer_initialize_ += since(t0);
+}
+
+static void check_sizes(const CscC& Y, size_t nV, size_t nS, const char* who) {
+    if (Y.nrow != Y.ncol || (int)nS != Y.nrow || (int)nS != Y.ncol) {
+        std::ostringstream os;
+        os << who << ": Size of the Sbus should be the same as the size of Ybus. Currently: Sbus  (" << nS
+           << ") and Ybus (" << Y.nrow << ", " << Y.ncol << ").";
+        throw std::runtime_error(os.str());
+    }
+    if ((int)nV != Y.nrow) {
+        std::ostringstream os;
+        os << who << ": Size of V (init voltages) should be the same as the size of Ybus. Currently: V  (" << nV
+           << ") and Ybus (" << Y.nrow << ", " << Y.ncol << ").";
+        throw std::runtime_error(os.str());
+    }
+}
+
+bool GpuNRSolver::compute_pf(const CscC& Ybus, CplxVect& V, const CplxVect& Sbus, const IntVect& slack_ids,
+                             const RealVect& slack_weights, const IntVect& pv, const IntVect& pq, int max_iter,
+                             double tol) {
+    check_sizes(Ybus, V.size(), Sbus.size(), single_ ? "BaseNRSingleSlackAlgo::compute_pf" : "BaseNRAlgo::compute_pf");
+    auto t0 = clk::now();
+    timer_Fx_ = timer_solve_ = timer_check_ = timer_total_ = timer_initialize_ = timer_pre_proc_ = 0;
+    ensure_grid(Ybus, slack_ids, slack_weights, pv, pq);
+    const int n = Ybus.ncol;
+    V_ = V;
+    int it = 0, st = 0;
+    Jx_.assign(info_.nnz_jac, 0.0);
+    pfgpu_stats stats{};
+    check_rc(pfgpu_solve_batch(engine_ctx(), grid_, 1, reinterpret_cast<const double*>(Ybus.x.data()), 0, nullptr,
+                               nullptr, nullptr, reinterpret_cast<const double*>(Sbus.data()), 0,
+                               reinterpret_cast<double*>(V_.data()), n, max_iter, tol,
+                               precision_, 0, &it, &st, Jx_.data(), &stats),
+             "pfgpu_solve_batch");
+    nr_iter_ = it;
+    err_ = (ErrorType)st;
+    timer_solve_ = stats.kernel_s;
+    timer_pre_proc_ = stats.h2d_s;
+    timer_total_ = since(t0);
+    return err_ == ErrorType::NoError;
+}
+
+void GpuNRSolver::solve_batch(const CscC& Ybus, const IntVect& slack_ids, const RealVect& w, const IntVect& pv,
+                              const IntVect& pq, int ninst, const IntVect* dptr, const IntVect* dpos,
+                              const CplxVect* dval, const CplxVect& Sbus, bool sbus_per_instance, const CplxVect& V0,
+                              bool v0_per_instance, int max_iter, double tol, CplxVect& Vout, IntVect& iters,
+                              IntVect& status, pfgpu_stats* st) {
+    ensure_grid(Ybus, slack_ids, w, pv, pq);
+    const int n = Ybus.ncol;
+    Vout.assign((size_t)ninst * n, cplx(0, 0));
+    if (v0_per_instance) std::copy(V0.begin(), V0.end(), Vout.begin());
+    else std::copy(V0.begin(), V0.begin() + n, Vout.begin());
+    iters.assign(ninst, 0);
+    status.assign(ninst, 0);
+    if (ninst == 0) return;
+    check_rc(pfgpu_solve_batch(engine_ctx(), grid_, ninst, reinterpret_cast<const double*>(Ybus.x.data()), 0,
+                               dptr ? dptr->data() : nullptr, dpos ? dpos->data() : nullptr,
+                               dval ? reinterpret_cast<const double*>(dval->data()) : nullptr,
+                               reinterpret_cast<const double*>(Sbus.data()), sbus_per_instance ? n : 0,
+                               reinterpret_cast<double*>(Vout.data()), v0_per_instance ? n : 0, max_iter, tol,
+                               precision_, 0, iters.data(), status.data(), nullptr, st),
+             "pfgpu_solve_batch");
+}
+
+CscR GpuNRSolver::get_J() const {
+    CscR J;
+    if (!grid_) return J;
+    J.nrow = J.ncol = info_.n_jac;
+    J.p.resize(info_.n_jac + 1);
+    J.i.resize(info_.nnz_jac);
+    check_rc(pfgpu_grid_get_J_pattern(grid_, J.p.data(), J.i.data()), "pfgpu_grid_get_J_pattern");
+    J.x = Jx_;
+    if ((int)J.x.size() != info_.nnz_jac) J.x.assign(info_.nnz_jac, 0.0);
+    return J;
+}
+
+// ---------------------------------------------------------------------------
+// ChooseSolver
+// ---------------------------------------------------------------------------
+ChooseSolver::ChooseSolver() : type_(SolverType::AMDGPU) {
+    s_[0] = std::make_unique<GpuNRSolver>(false, PFGPU_FP64);
+    s_[1] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64);
+    s_[2] = std::make_unique<GpuNRSolver>(false, PFGPU_FP32);
+    s_[3] = std::make_unique<GpuNRSolver>(true, PFGPU_FP32);
+}
+
+std::vector<SolverType> ChooseSolver::available_solvers() const {
+    return {SolverType::AMDGPU, SolverType::AMDGPUSingleSlack, SolverType::AMDGPUMixed,
+            SolverType::AMDGPUMixedSingleSlack};
+}
+
+void ChooseSolver::change_solver(SolverType t) {
+    if (t == type_) return;
+    if (!is_gpu_type(t)) {
+        // ChooseSolver.hpp:155-192: asking for a solver that is not compiled in raises
+        std::ostringstream os;
+        os << "Impossible to use the solver " << solver_type_name(t)
+           << ": it is not available in this build (available: AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, "
+              "AMDGPUMixedSingleSlack).";
+        throw std::runtime_error(os.str());
+    }
+    current().reset();
+    type_ = t;
+    current().reset();
+}
+
+GpuNRSolver& ChooseSolver::current() { return *s_[(int)type_ - (int)SolverType::AMDGPU]; }
+const GpuNRSolver& ChooseSolver::current() const { return *s_[(int)type_ - (int)SolverType::AMDGPU]; }
+
+// ---------------------------------------------------------------------------
+// GridModel
+// ---------------------------------------------------------------------------
+void GridModel::init_bus(int n_sub, int n_busbar_per_sub, const RealVect& bus_vn_kv, int nb_line, int nb_trafo) {
+    const int nb = n_sub * std::max(1, n_busbar_per_sub);
+    if ((int)bus_vn_kv.size() != nb && (int)bus_vn_kv.size() != n_sub)
+        throw std::runtime_error("GridModel::init_bus: bus_vn_kv must have n_sub or n_sub * n_busbar_per_sub entries");
+    bus_vn_kv_.assign(nb, 0.0);
+    for (int b = 0; b < nb; ++b) bus_vn_kv_[b] = bus_vn_kv[b % bus_vn_kv.size()];
+    (void)nb_line;
+    if (nb_trafo != 0) throw std::runtime_error("GridModel::init_bus: transformers are not part of this build");
+}
+
+void GridModel::init_powerlines(const RealVect& r, const RealVect& x, const CplxVect& h, const IntVect& from,
+                                const IntVect& to) {
+    const size_t m = r.size();
+    if (x.size() != m || h.size() != m || from.size() != m || to.size() != m)
+        throw std::runtime_error("GridModel::init_powerlines: inconsistent sizes");
+    for (size_t k = 0; k < m; ++k)
+        if (from[k] < 0 || from[k] >= total_bus() || to[k] < 0 || to[k] >= total_bus())
+            throw std::runtime_error("GridModel::init_powerlines: bus id out of range");
+    line_r_ = r; line_x_ = x; line_h_ = h; line_from_ = from; line_to_ = to;
+    line_status_.assign(m, true);
+}
+
+void GridModel::init_generators(const RealVect& p, const RealVect& v, const RealVect& min_q, const RealVect& max_q,
+                                const IntVect& bus) {
+    const size_t m = p.size();
+    if (v.size() != m || bus.size() != m || min_q.size() != m || max_q.size() != m)
+        throw std::runtime_error("GridModel::init_generators: inconsistent sizes");
+    gen_p_ = p; gen_v_ = v; gen_bus_ = bus;
+    gen_slack_.assign(m, false);
+    gen_slack_w_.assign(m, 0.0);
+}
+
+void GridModel::init_loads(const RealVect& p, const RealVect& q, const IntVect& bus) {
+    if (q.size() != p.size() || bus.size() != p.size())
+        throw std::runtime_error("GridModel::init_loads: inconsistent sizes");
+    load_p_ = p; load_q_ = q; load_bus_ = bus;
+}
+
+void GridModel::init_sgens(const RealVect& p, const RealVect& q, const RealVect&, const RealVect&, const RealVect&,
+                           const RealVect&, const IntVect& bus) {
+    if (q.size() != p.size() || bus.size() != p.size())
+        throw std::runtime_error("GridModel::init_sgens: inconsistent sizes");
+    sgen_p_ = p; sgen_q_ = q; sgen_bus_ = bus;
+}
+
+void GridModel::add_gen_slackbus(int gen_id, double weight) {
+    if (gen_id < 0 || gen_id >= (int)gen_bus_.size()) throw std::runtime_error("GridModel::add_gen_slackbus: bad id");
+    if (weight <= 0.0) throw std::runtime_error("GridModel::add_gen_slackbus: weight must be > 0");
+    gen_slack_[gen_id] = true;
+    gen_slack_w_[gen_id] = weight;
+}
+void GridModel::remove_gen_slackbus(int gen_id) {
+    if (gen_id < 0 || gen_id >= (int)gen_bus_.size()) throw std::runtime_error("GridModel::remove_gen_slackbus: bad id");
+    gen_slack_[gen_id] = false;
+    gen_slack_w_[gen_id] = 0.0;
+}
+void GridModel::deactivate_powerline(int id) { line_status_.at(id) = false; }
+void GridModel::reactivate_powerline(int id) { line_status_.at(id) = true; }
+
+void GridModel::branch_admittance(int k, cplx& yff, cplx& yft, cplx& ytf, cplx& ytt) const {
+    // TwoSidesContainer_rxh_A::_update_model_coeffs_one_el with h split 0.5 / 0.5 (LineContainer.cpp:13-34)
+    const cplx ys = 1.0 / cplx(line_r_[k], line_x_[k]);
+    yff = ys + 0.5 * line_h_[k];
+    ytt = ys + 0.5 * line_h_[k];
+    yft = -ys;
+    ytf = -ys;
+}
+
+// GridModel::pre_process_solver (GridModel.cpp:450-571) for this subset.
+SolverInput GridModel::pre_process_solver(const CplxVect& Vinit) const {
+    const int nb = total_bus();
+    if ((int)Vinit.size() != nb) {
+        std::ostringstream os;
+        os << "GridModel::ac_pf: Size of the Vinit should be the same as the total number of buses. Currently:  "
+           << "Vinit: " << Vinit.size() << " and there are " << nb << " buses.";
+        throw std::runtime_error(os.str());
+    }
+    SolverInput in;
+    // active buses: touched by a connected element
+    std::vector<char> active(nb, 0);
+    for (size_t k = 0; k < line_r_.size(); ++k)
+        if (line_status_[k]) active[line_from_[k]] = active[line_to_[k]] = 1;
+    for (int b : gen_bus_) active[b] = 1;
+    for (int b : load_bus_) active[b] = 1;
+    for (int b : sgen_bus_) active[b] = 1;
+    in.id_me_to_solver.assign(nb, -1);
+    for (int b = 0; b < nb; ++b)
+        if (active[b]) { in.id_me_to_solver[b] = (int)in.id_solver_to_me.size(); in.id_solver_to_me.push_back(b); }
+    const int ns = (int)in.id_solver_to_me.size();
+    // Ybus
+    std::vector<std::vector<std::pair<int, cplx>>> cols(ns);
+    auto add = [&](int r, int c, cplx v) { cols[c].push_back({r, v}); };
+    for (size_t k = 0; k < line_r_.size(); ++k) {
+        if (!line_status_[k]) continue;
+        cplx yff, yft, ytf, ytt;
+        branch_admittance((int)k, yff, yft, ytf, ytt);
+        const int f = in.id_me_to_solver[line_from_[k]], t = in.id_me_to_solver[line_to_[k]];
+        add(f, f, yff); add(t, t, ytt); add(f, t, yft); add(t, f, ytf);
+    }
+    in.Ybus.nrow = in.Ybus.ncol = ns;
+    in.Ybus.p.assign(ns + 1, 0);
+    for (int c = 0; c < ns; ++c) {
+        auto& v = cols[c];
+        bool diag = false;
+        for (auto& e : v) diag |= (e.first == c);
+        if (!diag) v.push_back({c, cplx(0, 0)});  // keep every diagonal entry structurally present
+        std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t q = 0; q < v.size(); ++q) {
+            if (!in.Ybus.i.empty() && (int)in.Ybus.i.size() > in.Ybus.p[c] && in.Ybus.i.back() == v[q].first) {
+                in.Ybus.x.back() += v[q].second;  // parallel elements sum up (setFromTriplets)
+            } else {
+                in.Ybus.i.push_back(v[q].first);
+                in.Ybus.x.push_back(v[q].second);
+            }
+        }
+        in.Ybus.p[c + 1] = (int)in.Ybus.i.size();
+    }
+    // Sbus (MW -> pu)
+    in.Sbus.assign(ns, cplx(0, 0));
+    for (size_t g = 0; g < gen_p_.size(); ++g) in.Sbus[in.id_me_to_solver[gen_bus_[g]]] += gen_p_[g];
+    for (size_t g = 0; g < sgen_p_.size(); ++g) in.Sbus[in.id_me_to_solver[sgen_bus_[g]]] += cplx(sgen_p_[g], sgen_q_[g]);
+    for (size_t l = 0; l < load_p_.size(); ++l) in.Sbus[in.id_me_to_solver[load_bus_[l]]] -= cplx(load_p_[l], load_q_[l]);
+    if (std::fabs(sn_mva_ - 1.0) > kTolEqualFloat)
+        for (auto& s : in.Sbus) s /= sn_mva_;
+    // slack buses (slack generators' buses, generator order), weights normalised per solver bus
+    std::vector<char> is_slack(ns, 0);
+    in.slack_weights.assign(ns, 0.0);
+    for (size_t g = 0; g < gen_bus_.size(); ++g) {
+        if (!gen_slack_[g] || std::fabs(gen_slack_w_[g]) < kTolEqualFloat) continue;
+        const int b = in.id_me_to_solver[gen_bus_[g]];
+        if (!is_slack[b]) { is_slack[b] = 1; in.slack_ids.push_back(b); }
+        in.slack_weights[b] += gen_slack_w_[g];
+    }
+    if (in.slack_ids.empty()) throw std::runtime_error("GridModel: no slack generator (use add_gen_slackbus)");
+    const double wsum = std::accumulate(in.slack_weights.begin(), in.slack_weights.end(), 0.0);
+    for (auto& w : in.slack_weights) w /= wsum;
+    // PV buses: voltage-regulating generators not on a slack bus (GeneratorContainer::fillpv)
+    std::vector<char> added(ns, 0);
+    for (size_t g = 0; g < gen_bus_.size(); ++g) {
+        const int b = in.id_me_to_solver[gen_bus_[g]];
+        if (is_slack[b] || added[b]) continue;
+        in.pv.push_back(b);
+        added[b] = 1;
+    }
+    for (int b = 0; b < ns; ++b)
+        if (!is_slack[b] && !added[b]) in.pq.push_back(b);
+    // initial V and generator magnitude setpoints (GeneratorContainer::set_vm)
+    in.V.assign(ns, cplx(init_vm_pu_, 0.0));
+    for (int b = 0; b < ns; ++b) in.V[b] = Vinit[in.id_solver_to_me[b]];
+    for (size_t g = 0; g < gen_bus_.size(); ++g) {
+        const int b = in.id_me_to_solver[gen_bus_[g]];
+        double m = std::abs(in.V[b]);
+        if (std::fabs(m) < kTolEqualFloat) { in.V[b] = 1.0; m = 1.0; }
+        in.V[b] *= gen_v_[g] / m;
+    }
+    return in;
+}
+
+CplxVect GridModel::ac_pf(const CplxVect& Vinit, int max_iter, double tol) {
+    last_ = pre_process_solver(Vinit);
+    CplxVect V = last_.V;
+    const bool conv = solver_->compute_pf(last_.Ybus, V, last_.Sbus, last_.slack_ids, last_.slack_weights, last_.pv,
+                                          last_.pq, max_iter, tol / sn_mva_);
+    CplxVect res;
+    if (!conv) return res;
+    V_solver_ = solver_->current().get_V();
+    res.assign(total_bus(), cplx(0, 0));
+    for (size_t b = 0; b < last_.id_solver_to_me.size(); ++b) res[last_.id_solver_to_me[b]] = V_solver_[b];
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Batch drivers
+// ---------------------------------------------------------------------------
+// amps / active power at the origin side of every line for every row of voltages
+// (BaseBatchSolverSynch.hpp:86-192): I = yff Ef + yft Et, S = Ef conj(I);
+// A = |S| sn_mva / (sqrt(3) |Ef| vn_kv(from)), P = Re(S) sn_mva.
+void BaseBatchSolverSynch::compute_flows_from_Vs(bool amps) {
+    if (voltages_.empty())
+        throw std::runtime_error("BaseMultiplePowerflow::compute_flows_from_Vs: cannot compute the flows as the "
+                                 "voltages are not set. Have you called compute(...) ? ");
+    auto t0 = clk::now();
+    const int nl = grid_model_.nb_powerline(), nb = grid_model_.total_bus();
+    const double sn = grid_model_.get_sn_mva(), s3 = std::sqrt(3.0);
+    RealVect& out = amps ? amps_ : pflows_;
+    out.assign((size_t)nrows_ * nl, 0.0);
+    for (int k = 0; k < nl; ++k) {
+        if (!grid_model_.line_status(k)) continue;
+        cplx yff, yft, ytf, ytt;
+        grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
+        const int f = grid_model_.line_from(k), t = grid_model_.line_to(k);
+        const double vn = grid_model_.bus_vn_kv(f);
+        for (int r = 0; r < nrows_; ++r) {
+            const cplx Ef = voltages_[(size_t)r * nb + f], Et = voltages_[(size_t)r * nb + t];
+            const cplx S = Ef * std::conj(yff * Ef + yft * Et);
+            out[(size_t)r * nl + k] = amps ? std::abs(S) * sn / (s3 * std::abs(Ef) * vn) : S.real() * sn;
+        }
+    }
+    if (amps) timer_amps_ = since(t0);
+}
+
+void ContingencyAnalysis::add_all_n1() {
+    for (int k = 0; k < grid_model_.nb_powerline(); ++k) defaults_.insert({k});
+}
+void ContingencyAnalysis::add_n1(int id) {
+    if (id < 0 || id >= grid_model_.nb_powerline()) {
+        std::ostringstream os;
+        os << "ContingencyAnalysis::add_n1: impossible to add the line " << id << " there are only "
+           << grid_model_.nb_powerline() << " branches on the grid.";
+        throw std::runtime_error(os.str());
+    }
+    defaults_.insert({id});
+}
+void ContingencyAnalysis::add_nk(const IntVect& ids) {
+    std::set<int> s;
+    for (int id : ids) {
+        if (id < 0 || id >= grid_model_.nb_powerline()) throw std::runtime_error("ContingencyAnalysis::add_nk: bad id");
+        s.insert(id);
+    }
+    defaults_.insert(s);
+}
+void ContingencyAnalysis::add_multiple_n1(const IntVect& ids) { for (int id : ids) add_n1(id); }
+void ContingencyAnalysis::remove_n1(int id) { defaults_.erase({id}); }
+void ContingencyAnalysis::remove_nk(const IntVect& ids) { defaults_.erase(std::set<int>(ids.begin(), ids.end())); }
+void ContingencyAnalysis::remove_multiple_n1(const IntVect& ids) { for (int id : ids) remove_n1(id); }
+void ContingencyAnalysis::clear() {
+    defaults_.clear();
+    clear_results_only();
+}
+void ContingencyAnalysis::clear_results_only() {
+    voltages_.clear(); amps_.clear(); pflows_.clear();
+    nrows_ = 0;
+    nb_solved_ = 0;
+    timer_total_ = timer_solver_ = timer_pre_proc_ = timer_amps_ = timer_modif_ybus_ = 0;
+}
+std::vector<IntVect> ContingencyAnalysis::my_defaults() const {
+    std::vector<IntVect> r;
+    for (const auto& s : defaults_) r.emplace_back(s.begin(), s.end());
+    return r;
+}
+
+namespace {
+// per-contingency Ybus coefficients (ContingencyAnalysis::init_li_coeffs), as CSC value positions
+struct ContDeltas {
+    IntVect dptr, dpos;
+    CplxVect dval;
+};
+int csc_pos(const CscC& Y, int r, int c) {
+    auto b = Y.i.begin() + Y.p[c], e = Y.i.begin() + Y.p[c + 1];
+    auto it = std::lower_bound(b, e, r);
+    if (it == e || *it != r) throw std::runtime_error("ContingencyAnalysis: coefficient not in the Ybus pattern");
+    return (int)(it - Y.i.begin());
+}
+// BFS with |y| > 1e-8 (ContingencyAnalysis::check_invertible)
+bool connected(const CscC& Y, const CplxVect& x) {
+    const int n = Y.ncol;
+    std::vector<char> visited(n, 0), added(n, 0);
+    std::queue<int> q;
+    int col = 0;
+    while (true) {
+        visited[col] = 1;
+        for (int k = Y.p[col]; k < Y.p[col + 1]; ++k) {
+            const int r = Y.i[k];
+            if (!visited[r] && !added[r] && std::abs(x[k]) > 1e-8) { q.push(r); added[r] = 1; }
+        }
+        if (q.empty()) break;
+        col = q.front();
+        q.pop();
+    }
+    for (char v : visited) if (!v) return false;
+    return true;
+}
+}  // namespace
+
+IntVect ContingencyAnalysis::is_grid_connected_after_contingency() {
+    SolverInput in = grid_model_.pre_process_solver(CplxVect(grid_model_.total_bus(), cplx(1, 0)));
+    IntVect res;
+    CplxVect x = in.Ybus.x;
+    for (const auto& cont : defaults_) {
+        std::vector<std::pair<int, cplx>> edits;
+        for (int k : cont) {
+            if (!grid_model_.line_status(k)) continue;
+            cplx yff, yft, ytf, ytt;
+            grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
+            const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
+            edits.push_back({csc_pos(in.Ybus, f, f), yff});
+            edits.push_back({csc_pos(in.Ybus, t, t), ytt});
+            edits.push_back({csc_pos(in.Ybus, f, t), yft});
+            edits.push_back({csc_pos(in.Ybus, t, f), ytf});
+        }
+        for (auto& e : edits) x[e.first] -= e.second;
+        res.push_back(connected(in.Ybus, x) ? 1 : 0);
+        for (auto& e : edits) x[e.first] += e.second;
+    }
+    return res;
+}
+
+// ContingencyAnalysis::compute (ContingencyAnalysis.cpp:166-233) as one batched GPU call.
+void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double tol) {
+    auto t_all = clk::now();
+    auto t_pre = clk::now();
+    nb_solved_ = 0;
+    timer_solver_ = timer_modif_ybus_ = 0;
+    const double sn = grid_model_.get_sn_mva();
+    SolverInput in = grid_model_.pre_process_solver(Vinit);
+    const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
+    const int ncont = (int)defaults_.size();
+    nrows_ = ncont;
+    voltages_.assign((size_t)ncont * nb, cplx(0, 0));
+    amps_.clear();
+    pflows_.clear();
+    GpuNRSolver& s = solver_.current();
+    s.reset();
+    // base case ("powerflow in n", BaseBatchSolverSynch.hpp:296-313): tol as given
+    CplxVect Vb = in.V;
+    const bool base_conv = s.compute_pf(in.Ybus, Vb, in.Sbus, in.slack_ids, in.slack_weights, in.pv, in.pq, max_iter, tol);
+    timer_solver_ += s.get_computation_time();
+    CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
+    timer_pre_proc_ = since(t_pre);
+    if (!base_conv) { timer_total_ = since(t_all); return; }
+    // Ybus edits and connectivity
+    auto t_y = clk::now();
+    ContDeltas D;
+    D.dptr.push_back(0);
+    IntVect which;
+    CplxVect x = in.Ybus.x;
+    int c = 0;
+    for (const auto& cont : defaults_) {
+        std::vector<std::pair<int, cplx>> edits;
+        for (int k : cont) {
+            if (!grid_model_.line_status(k)) continue;
+            cplx yff, yft, ytf, ytt;
+            grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
+            const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
+            edits.push_back({csc_pos(in.Ybus, f, f), yff});
+            edits.push_back({csc_pos(in.Ybus, t, t), ytt});
+            edits.push_back({csc_pos(in.Ybus, f, t), yft});
+            edits.push_back({csc_pos(in.Ybus, t, f), ytf});
+        }
+        for (auto& e : edits) x[e.first] -= e.second;
+        const bool inv = connected(in.Ybus, x);
+        for (auto& e : edits) x[e.first] += e.second;
+        if (inv) {
+            which.push_back(c);
+            for (auto& e : edits) { D.dpos.push_back(e.first); D.dval.push_back(e.second); }
+            D.dptr.push_back((int)D.dpos.size());
+        }
+        ++c;
+    }
+    timer_modif_ybus_ = since(t_y);
+    // batched solve, tol / sn_mva (ContingencyAnalysis.cpp:223)
+    CplxVect Vout;
+    IntVect iters, status;
+    pfgpu_stats st{};
+    auto t_s = clk::now();
+    s.solve_batch(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, (int)which.size(), &D.dptr, &D.dpos, &D.dval,
+                  in.Sbus, false, Vstart, false, max_iter, tol / sn, Vout, iters, status, &st);
+    timer_solver_ += since(t_s);
+    nb_solved_ = (int)which.size();
+    for (size_t w = 0; w < which.size(); ++w) {
+        if (status[w] != 0) continue;
+        const int row = which[w];
+        for (int b = 0; b < ns; ++b) voltages_[(size_t)row * nb + in.id_solver_to_me[b]] = Vout[w * ns + b];
+    }
+    timer_total_ = since(t_all);
+}
+
+// ContingencyAnalysis::clean_flows (ContingencyAnalysis.cpp:237-250)
+void ContingencyAnalysis::clean_flows(bool amps) {
+    RealVect& out = amps ? amps_ : pflows_;
+    const int nl = grid_model_.nb_powerline();
+    int c = 0;
+    for (const auto& cont : defaults_) {
+        for (int k : cont) {
+            double& el = out[(size_t)c * nl + k];
+            if (std::isfinite(el)) el = 0.0;
+        }
+        ++c;
+    }
+}
+void ContingencyAnalysis::compute_flows() {
+    compute_flows_from_Vs(true);
+    clean_flows(true);
+}
+void ContingencyAnalysis::compute_power_flows() {
+    compute_flows_from_Vs(false);
+    clean_flows(false);
+}
+
+void TimeSeries::clear() {
+    voltages_.clear(); amps_.clear(); pflows_.clear(); sbuses_.clear();
+    nrows_ = 0;
+    nb_solved_ = 0;
+    status_ = 1;
+    timer_total_ = timer_solver_ = timer_pre_proc_ = timer_amps_ = 0;
+}
+
+// TimeSeries::compute_Vs (TimeSeries.cpp:14-109). The steps are cut in chains of
+// chain_length consecutive steps; chain heads start from the initial V, every
+// other step from the previous step's V of its chain; all chains advance
+// together, one batched GPU call per position in the chain.
+int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p, const RealVect& load_p,
+                           const RealVect& load_q, const CplxVect& Vinit, int max_iter, double tol) {
+    auto t_all = clk::now();
+    auto t_pre = clk::now();
+    status_ = 0;
+    nb_solved_ = 0;
+    timer_solver_ = 0;
+    const int ng = grid_model_.nb_gen(), nsg = grid_model_.nb_sgen(), nl = grid_model_.nb_load();
+    if ((int)gen_p.size() != T * ng || (int)sgen_p.size() != T * nsg || (int)load_p.size() != T * nl ||
+        (int)load_q.size() != T * nl)
+        throw std::runtime_error("TimeSeries::compute_Vs: injection matrices must be [T x nb_element]");
+    const double sn = grid_model_.get_sn_mva();
+    SolverInput in = grid_model_.pre_process_solver(Vinit);
+    const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
+    nbus_solver_ = ns;
+    sbuses_.assign((size_t)T * ns, cplx(0, 0));
+    for (int t = 0; t < T; ++t) {
+        cplx* S = &sbuses_[(size_t)t * ns];
+        for (int g = 0; g < ng; ++g) S[in.id_me_to_solver[grid_model_.gen_bus(g)]] += gen_p[(size_t)t * ng + g];
+        for (int g = 0; g < nsg; ++g) S[in.id_me_to_solver[grid_model_.sgen_bus(g)]] += sgen_p[(size_t)t * nsg + g];
+        for (int l = 0; l < nl; ++l)
+            S[in.id_me_to_solver[grid_model_.load_bus(l)]] -= cplx(load_p[(size_t)t * nl + l], load_q[(size_t)t * nl + l]);
+        if (std::fabs(sn - 1.0) > kTolEqualFloat)
+            for (int b = 0; b < ns; ++b) S[b] /= sn;
+    }
+    nrows_ = T;
+    voltages_.assign((size_t)T * nb, cplx(0, 0));
+    amps_.clear();
+    pflows_.clear();
+    GpuNRSolver& s = solver_.current();
+    s.reset();
+    CplxVect Vb = in.V;
+    const bool base_conv = s.compute_pf(in.Ybus, Vb, in.Sbus, in.slack_ids, in.slack_weights, in.pv, in.pq, max_iter, tol);
+    timer_solver_ += s.get_computation_time();
+    timer_pre_proc_ = since(t_pre);
+    if (!base_conv) { status_ = 0; timer_total_ = since(t_all); return -1; }
+    const CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
+    const int L = std::max(1, std::min(chain_len_ > 0 ? chain_len_ : T, T));
+    const int nch = (T + L - 1) / L;
+    CplxVect Vcur((size_t)nch * ns);
+    for (int c = 0; c < nch; ++c) std::copy(Vstart.begin(), Vstart.end(), Vcur.begin() + (size_t)c * ns);
+    std::vector<int> conv_step(T, 0), iters_step(T, 0);
+    for (int pos = 0; pos < L; ++pos) {
+        // chains that still have a step at this position
+        IntVect steps;
+        for (int c = 0; c < nch; ++c) {
+            const int t = c * L + pos;
+            if (t < T) steps.push_back(t);
+        }
+        if (steps.empty()) break;
+        const int m = (int)steps.size();
+        CplxVect Sb((size_t)m * ns), V0((size_t)m * ns), Vout;
+        for (int k = 0; k < m; ++k) {
+            std::copy(sbuses_.begin() + (size_t)steps[k] * ns, sbuses_.begin() + (size_t)(steps[k] + 1) * ns,
+                      Sb.begin() + (size_t)k * ns);
+            std::copy(Vcur.begin() + (size_t)(steps[k] / L) * ns, Vcur.begin() + (size_t)(steps[k] / L + 1) * ns,
+                      V0.begin() + (size_t)k * ns);
+        }
+        IntVect it, stt;
+        pfgpu_stats st{};
+        auto t_s = clk::now();
+        s.solve_batch(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, m, nullptr, nullptr, nullptr, Sb, true, V0,
+                      true, max_iter, tol / sn, Vout, it, stt, &st);
+        timer_solver_ += since(t_s);
+        for (int k = 0; k < m; ++k) {
+            const int t = steps[k], c = t / L;
+            conv_step[t] = stt[k] == 0;
+            iters_step[t] = it[k];
+            // compute_one_powerflow only takes V from the solver on convergence (BaseBatchSolverSynch.cpp:37-39)
+            if (conv_step[t]) std::copy(Vout.begin() + (size_t)k * ns, Vout.begin() + (size_t)(k + 1) * ns,
+                                        Vcur.begin() + (size_t)c * ns);
+            for (int b = 0; b < ns && conv_step[t]; ++b)
+                voltages_[(size_t)t * nb + in.id_solver_to_me[b]] = Vout[(size_t)k * ns + b];
+        }
+    }
+    // the reference stops at the first divergence (TimeSeries.cpp:94-99)
+    int first_fail = T;
+    for (int t = 0; t < T; ++t)
+        if (!conv_step[t]) { first_fail = t; break; }
+    for (int t = first_fail; t < T; ++t)
+        std::fill(voltages_.begin() + (size_t)t * nb, voltages_.begin() + (size_t)(t + 1) * nb, cplx(0, 0));
+    nb_solved_ = std::min(T, first_fail + 1);
+    status_ = first_fail == T ? 1 : 0;
+    timer_total_ = since(t_all);
+    return status_;
+}
+
+}  // namespace ls2g

@@ -278,3 +278,27 @@ def time_series_sbus(grid: Grid, T: int = 8760, seed: int = 1) -> np.ndarray:
     base_load = grid.load_p.sum()
     gen_scale = P.sum(axis=1) / base_load if base_load > 0 else np.ones(T)
     return grid.loading * (grid.gen_p[None, :] * gen_scale[:, None] - (P + 1j * Q))
+
+
+def to_gridmodel(grid: Grid, sn_mva: float = 1.0):
+    """A GridModel (lightsim2grid_amd.GridModel) holding the same grid: lines with
+    h = j*b, one voltage-regulating generator per slack/PV bus, one load per PQ bus,
+    slack generator(s) as the slack model says. Powers are in MW with sn_mva."""
+    from .lightsim2grid_amd_cpp import GridModel
+
+    gm = GridModel()
+    gm.set_sn_mva(sn_mva)
+    gm.set_init_vm_pu(1.0)
+    gm.init_bus(grid.n, 1, np.ones(grid.n), grid.nbr, 0)
+    gm.init_powerlines(grid.r, grid.x, 1j * grid.b, grid.f, grid.t)
+    gen_bus = np.r_[grid.slack, grid.pv].astype(np.int32)
+    gen_p = grid.loading * grid.gen_p[gen_bus] * sn_mva
+    gm.init_generators(gen_p, grid.vset[gen_bus], -1e9 * np.ones(len(gen_bus)), 1e9 * np.ones(len(gen_bus)), gen_bus)
+    load_bus = grid.pq.astype(np.int32)
+    gm.init_loads(grid.loading * grid.load_p[load_bus] * sn_mva, grid.loading * grid.load_q[load_bus] * sn_mva, load_bus)
+    if grid.distributed_slack:
+        for g in range(len(gen_bus)):
+            gm.add_gen_slackbus(g, 1.0)
+    else:
+        gm.add_gen_slackbus(0, 1.0)
+    return gm

@@ -136,7 +136,22 @@ def test_error_statuses(gpu_ctx):
     assert r["status"][0] == 3  # InifiniteValue
     r = gr.solve(Yx, d["Sbus"], d["V0"], 0, 1e-8)
     assert r["status"][0] == 2 and r["iters"][0] == 0
-    # already converged start: zero iterations
-    ok = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8)
-    r = gr.solve(Yx, d["Sbus"], ok["V"][0], 10, 1e-8)
-    assert r["status"][0] == 0 and r["iters"][0] == 0
+
+def test_restart_from_converged_voltages(gpu_ctx, oracle_mod):
+    """Restarting from a converged V: the single-slack form needs 0 iterations; the
+    distributed form re-initialises slack_absorbed = Re(sum Sbus) on every call
+    (BaseNRAlgo.tpp:60), so it takes what the oracle takes (1 here)."""
+    from lightsim2grid_amd import abi
+    d = load_fixture("case14")
+    for single in (False, True):
+        _say("restart", "single" if single else "dist")
+        gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], single_slack=single)
+        Yx = gr.ybus_values(d["Ybus"])
+        ok = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8)
+        r = gr.solve(Yx, d["Sbus"], ok["V"][0], 10, 1e-8)
+        o = oracle_mod.OracleNR(single)
+        assert o.compute_pf(d["Ybus"], ok["V"][0], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"],
+                            10, 1e-8)
+        assert r["status"][0] == 0 and r["iters"][0] == o.get_nb_iter()
+        if single:
+            assert r["iters"][0] == 0
