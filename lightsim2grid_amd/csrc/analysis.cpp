@@ -430,64 +430,77 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
         return k < npvpq ? J.pvpq[k] : J.pq[k - npvpq];
     };
     P.j_ypos.resize(P.nJ); P.j_rbus.resize(P.nJ); P.j_cbus.resize(P.nJ); P.j_sel.resize(P.nJ); P.j_cval.resize(P.nJ);
-    std::vector<int> slot_jsrc(S.nvals, -1);
+    P.j_slot.resize(P.nJ);
+    std::vector<int> npairs_of(S.nvals, 0);
+    for (int l = 0; l < nlev; ++l)
+        for (int t = sch.lvl_ptr[l]; t < sch.lvl_ptr[l + 1]; ++t)
+            npairs_of[sch.task_out[t]] = sch.task_pend[t] - sch.task_pbeg[t];
+    std::vector<char> is_jslot(S.nvals, 0);
     for (int c = 0; c < N; ++c)
         for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
             P.j_ypos[q] = J.src[q];
             P.j_sel[q] = J.sel[q];
             P.j_cval[q] = J.cval[q];
             if (J.src[q] >= 0) {
-                // Ybus entry (r, c) in CSC: find its row bus
                 P.j_rbus[q] = Y.i[J.src[q]];
-                int cb = 0;
                 // column bus of the Ybus entry: the bus whose CSC column holds src
-                cb = (int)(std::upper_bound(Y.p.begin(), Y.p.end(), J.src[q]) - Y.p.begin()) - 1;
-                P.j_cbus[q] = cb;
+                P.j_cbus[q] = (int)(std::upper_bound(Y.p.begin(), Y.p.end(), J.src[q]) - Y.p.begin()) - 1;
             } else {
                 P.j_rbus[q] = jbus(J.Ji[q]);
                 P.j_cbus[q] = jbus(c);
             }
-            slot_jsrc[S.jslot[q]] = q;
+            const int slot = S.jslot[q];
+            is_jslot[slot] = 1;
+            const bool recip = slot < N && npairs_of[slot] == 0;  // pivot with nothing to eliminate
+            P.j_slot[q] = nw[slot] | (recip ? (1 << 30) : 0);
         }
-    // --- factor tasks ----------------------------------------------------------
-    P.f_lvl = sch.lvl_ptr;
-    P.f_maxp.assign(nlev, 0);
-    const int ntask = sch.lvl_ptr[nlev];
-    P.t_out.resize(ntask); P.t_pbeg.resize(ntask + 1); P.t_jsrc.resize(ntask); P.t_kind.resize(ntask);
-    P.p_l.resize(sch.npairs); P.p_u.resize(sch.npairs); P.p_d.resize(sch.npairs);
-    for (int l = 0; l < nlev; ++l)
+    for (int64_t sl = 0; sl < S.off_y; ++sl)
+        if (!is_jslot[sl]) P.zero_slots.push_back(nw[sl]);
+    // --- factor tasks (tasks without pairs are covered by the J fill / mismatch) ---------
+    std::vector<int> lvl(1, 0);
+    for (int l = 0; l < nlev; ++l) {
+        int mp = 0;
         for (int t = sch.lvl_ptr[l]; t < sch.lvl_ptr[l + 1]; ++t) {
-            P.t_out[t] = nw[sch.task_out[t]];
-            P.t_pbeg[t] = sch.task_pbeg[t];
-            P.t_jsrc[t] = slot_jsrc[sch.task_out[t]];
-            P.t_kind[t] = sch.task_diag[t] ? 1 : (sch.task_out[t] >= S.off_y ? 2 : 0);
-            P.f_maxp[l] = std::max(P.f_maxp[l], sch.task_pend[t] - sch.task_pbeg[t]);
+            const int np = sch.task_pend[t] - sch.task_pbeg[t];
+            if (np == 0) continue;
+            const int o = sch.task_out[t];
+            const int kind = sch.task_diag[t] ? 1 : (o >= S.off_y ? 2 : 0);
+            P.tasks.insert(P.tasks.end(), {nw[o], sch.task_pbeg[t], sch.task_pend[t], kind});
+            mp = std::max(mp, np);
         }
-    P.t_pbeg[ntask] = (int)sch.npairs;
+        const int cnt = (int)P.tasks.size() / 4;
+        if (cnt == lvl.back()) continue;  // empty level after pruning
+        lvl.push_back(cnt);
+        P.f_maxp.push_back(mp);
+    }
+    P.f_lvl = lvl;
+    P.nlev = (int)lvl.size() - 1;
+    P.pairs.resize(4 * sch.npairs);
     for (int64_t p = 0; p < sch.npairs; ++p) {
-        P.p_l[p] = nw[sch.pair_l[p]];
-        P.p_u[p] = nw[sch.pair_u[p]];
-        P.p_d[p] = nw[sch.pair_d[p]];
+        P.pairs[4 * p] = nw[sch.pair_l[p]];
+        P.pairs[4 * p + 1] = nw[sch.pair_u[p]];
+        P.pairs[4 * p + 2] = nw[sch.pair_d[p]];
+        P.pairs[4 * p + 3] = 0;
     }
     // --- backward tasks ---------------------------------------------------------
     P.b_lvl = sch.blvl_ptr;
     P.b_maxp.assign(nblev, 0);
-    P.bt_out.resize(N); P.bt_pbeg.resize(N + 1); P.bt_diag.resize(N);
+    P.btasks.resize(4 * N);
     int bp = 0;
     for (int l = 0; l < nblev; ++l)
         for (int q = sch.blvl_ptr[l]; q < sch.blvl_ptr[l + 1]; ++q) {
             const int j = sch.bcol[q];
-            P.bt_out[q] = ys(j);
-            P.bt_diag[q] = nw[j];
-            P.bt_pbeg[q] = bp;
+            P.btasks[4 * q] = ys(j);
+            P.btasks[4 * q + 1] = bp;
             for (int a = S.cp[j]; a < S.cp[j + 1]; ++a) {
-                P.bp_u.push_back(nw[S.off_U + a]);
-                P.bp_x.push_back(ys(S.ci[a]));
+                P.bpairs.push_back(nw[S.off_U + a]);
+                P.bpairs.push_back(ys(S.ci[a]));
                 ++bp;
             }
+            P.btasks[4 * q + 2] = bp;
+            P.btasks[4 * q + 3] = nw[j];
             P.b_maxp[l] = std::max(P.b_maxp[l], S.cp[j + 1] - S.cp[j]);
         }
-    P.bt_pbeg[N] = bp;
     return P;
 }
 

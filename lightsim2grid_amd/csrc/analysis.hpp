@@ -103,22 +103,27 @@ struct DevicePlan {
     // per J unknown (original J column order): bus, kind (0 slack_absorbed, 1 Va, 2 Vm), value slot of dx
     std::vector<int> ux_bus, ux_slot;
     std::vector<int8_t> ux_kind;
-    // J entries (reference CSC order): Ybus CSC pos, row bus, col bus, selector, constant
-    std::vector<int> j_ypos, j_rbus, j_cbus;
+    // J entries (reference CSC order): Ybus CSC pos, row bus, col bus, selector, constant,
+    // and the value slot the J-fill phase writes (bit 30 set: store the reciprocal, for a
+    // pivot whose column has nothing to eliminate)
+    std::vector<int> j_ypos, j_rbus, j_cbus, j_slot;
     std::vector<uint8_t> j_sel;
     std::vector<double> j_cval;
-    // factor + forward tasks (level-major)
+    // LU slots that hold no J entry (fill-in): zeroed by the J-fill phase
+    std::vector<int> zero_slots;
+    // factor + forward tasks (level-major), 4 ints each: {out slot, pair begin, pair end, kind};
+    // kind 0 = L/U entry, 1 = pivot (stores the reciprocal), 2 = forward-solve entry.
+    // Every task starts from the value already in its slot (J entry, 0, or F) and
+    // subtracts sum X[l]*X[u]*X[d] over its pairs. Tasks without pairs are not listed.
     std::vector<int> f_lvl;                 // nlev + 1
     std::vector<int> f_maxp;                // per level max pair count
-    std::vector<int> t_out, t_pbeg, t_jsrc; // per task; pair range [t_pbeg[i], t_pbeg[i+1])
-    // task kind: 0 = L/U entry (initial value: its J entry or 0 for fill),
-    // 1 = diagonal (initial value: its J entry; stores the reciprocal),
-    // 2 = forward-solve entry y_k (initial value: the F slot already in place)
-    std::vector<uint8_t> t_kind;
-    std::vector<int> p_l, p_u, p_d;         // per pair
-    // backward tasks (level-major): x slot (in/out), pair range, slot of 1/d
-    std::vector<int> b_lvl, b_maxp, bt_out, bt_pbeg, bt_diag;
-    std::vector<int> bp_u, bp_x;
+    std::vector<int> tasks;                 // 4 * ntask
+    std::vector<int> pairs;                 // 4 * npairs: {l, u, d, 0}
+    // backward tasks (level-major), 4 ints each: {x slot, pair begin, pair end, slot of 1/d};
+    // x = (x - sum X[u]*X[x']) * X[1/d]
+    std::vector<int> b_lvl, b_maxp;
+    std::vector<int> btasks;                // 4 * N
+    std::vector<int> bpairs;                // 2 * nbpairs: {u, x'}
     double flops = 0;
     int64_t nnzLU = 0;
 };

@@ -38,8 +38,8 @@
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBlock = 1024;
+constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
 
 thread_local std::string g_last_error;
@@ -79,17 +79,19 @@ int guarded(F&& f) {
 // Kernel arguments
 // ---------------------------------------------------------------------------
 struct KArgs {
-    int n, N, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter;
+    int n, N, nJ, nzero, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter;
     double tol;
     const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
     const int8_t* ux_kind;
-    const int *j_ypos, *j_rbus, *j_cbus;
+    const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
     const double* j_cval;
-    const int *f_lvl, *f_team, *t_out, *t_pbeg, *t_jsrc;
-    const uint8_t* t_kind;
-    const int *p_l, *p_u, *p_d;
-    const int *b_lvl, *b_team, *bt_out, *bt_pbeg, *bt_diag, *bp_u, *bp_x;
+    const int *f_lvl, *f_team;
+    const int4* tasks;   // {out, pair begin, pair end, kind}
+    const int4* pairs;   // {l, u, 1/d, -}
+    const int *b_lvl, *b_team;
+    const int4* btasks;  // {x, pair begin, pair end, 1/d}
+    const int2* bpairs;  // {u, x'}
     const double* w;
     const double* Yx;
     long long y_stride;  // doubles between instances (0 = shared)
@@ -116,8 +118,9 @@ __device__ __forceinline__ c64 ld_c(const double* p, int i) { return {p[2 * i], 
 
 // Reduce over the nslot lanes of one instance: shuffles inside the wave
 // (lanes g, g+G, ... of a wave), then the kWaves partials through LDS.
-template <int G, bool IS_MAX>
+template <int G, int NT, bool IS_MAX>
 __device__ __forceinline__ double inst_reduce(double v, double (*part)[kMaxG]) {
+    constexpr int kWaves = NT / 64;
 #pragma unroll
     for (int o = G; o < 64; o <<= 1) {
         const double u = __shfl_xor(v, o);
@@ -136,8 +139,9 @@ __device__ __forceinline__ double inst_reduce(double v, double (*part)[kMaxG]) {
 
 __device__ __forceinline__ double inf_if_nonfinite(double a) { return (a <= 1.7976931348623157e308) ? a : INFINITY; }
 
-template <typename FT, int G>
-__global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
+template <typename FT, int G, int NT>
+__global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
+    constexpr int kWaves = NT / 64;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     FT* Xl = reinterpret_cast<FT*>(lds_raw);
     __shared__ double s_part[kWaves][kMaxG];
@@ -147,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
 
     const int tid = threadIdx.x;
     const int g = tid % G, s = tid / G;
-    constexpr int nslot = kBlock / G;
+    constexpr int nslot = NT / G;
     const long long inst = (long long)blockIdx.x * G + g;
     const bool valid = inst < a.ninst;
     const long long ii = valid ? inst : 0;
@@ -214,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
                 if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
             }
         }
-        return inst_reduce<G, true>(fm, s_part);
+        return inst_reduce<G, NT, true>(fm, s_part);
     };
 
     double fmx = mismatch();
@@ -257,6 +261,23 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
         bool act = valid && !s_done[g];
         if (s == 0 && act) s_it[g] += 1;
 
+        // ---- J fill: zero the fill-in slots, write every J entry --------------------
+        if (act) {
+            for (int i = s; i < a.nzero; i += nslot) XS(a.zero_slots[i], (FT)0);
+            for (int q = s; q < a.nJ; q += nslot) {
+                const double jv = jvalue(q);
+                if (a.Jx_last != nullptr && inst == 0) a.Jx_last[q] = jv;
+                const int js = a.j_slot[q];
+                FT v = (FT)jv;
+                if (js & (1 << 30)) {  // pivot with nothing to eliminate
+                    if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                    v = (FT)1 / v;
+                }
+                XS(js & ((1 << 30) - 1), v);
+            }
+        }
+        __syncthreads();
+
         // ---- factor + forward solve --------------------------------------------
         for (int l = 0; l < a.nlev; ++l) {
             const int team = a.f_team[l];
@@ -265,26 +286,41 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
             const int hi = a.f_lvl[l + 1];
             if (act) {
                 for (int t = a.f_lvl[l] + s / team; t < hi; t += nteams) {
-                    const int pb = a.t_pbeg[t], pe = a.t_pbeg[t + 1];
+                    const int4 td = a.tasks[t];
                     FT acc = (FT)0;
-                    for (int p = pb + tpos; p < pe; p += team) acc += XL(a.p_l[p]) * XL(a.p_u[p]) * XL(a.p_d[p]);
+                    int p = td.y + tpos;
+                    for (; p + 3 * team < td.z; p += 4 * team) {
+                        const int4 q0 = a.pairs[p], q1 = a.pairs[p + team];
+                        const int4 q2 = a.pairs[p + 2 * team], q3 = a.pairs[p + 3 * team];
+                        const FT x0 = XL(q0.x) * XL(q0.y) * XL(q0.z);
+                        const FT x1 = XL(q1.x) * XL(q1.y) * XL(q1.z);
+                        const FT x2 = XL(q2.x) * XL(q2.y) * XL(q2.z);
+                        const FT x3 = XL(q3.x) * XL(q3.y) * XL(q3.z);
+                        acc += (x0 + x1) + (x2 + x3);
+                    }
+                    for (; p < td.z; p += team) {
+                        const int4 q = a.pairs[p];
+                        acc += XL(q.x) * XL(q.y) * XL(q.z);
+                    }
                     for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
                     if (tpos == 0) {
-                        const int kind = a.t_kind[t], o = a.t_out[t], js = a.t_jsrc[t];
-                        FT av = (FT)0;
-                        if (kind == 2) {
-                            av = XL(o);
-                        } else if (js >= 0) {
-                            const double jv = jvalue(js);
-                            if (a.Jx_last != nullptr && inst == 0) a.Jx_last[js] = jv;
-                            av = (FT)jv;
+                        // one explicit read-modify-write per address space (a merged generic
+                        // pointer for this slot trips the gfx950 backend)
+                        if (td.x < nlds) {
+                            FT v = Xl[td.x * G + g] - acc;
+                            if (td.w == 1) {
+                                if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                v = (FT)1 / v;
+                            }
+                            Xl[td.x * G + g] = v;
+                        } else {
+                            FT v = Xgb[(size_t)(td.x - nlds) * G + g] - acc;
+                            if (td.w == 1) {
+                                if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                v = (FT)1 / v;
+                            }
+                            Xgb[(size_t)(td.x - nlds) * G + g] = v;
                         }
-                        FT v = av - acc;
-                        if (kind == 1) {
-                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                            v = (FT)1 / v;
-                        }
-                        XS(o, v);
                     }
                 }
             }
@@ -305,14 +341,20 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
             const int hi = a.b_lvl[l + 1];
             if (act) {
                 for (int t = a.b_lvl[l] + s / team; t < hi; t += nteams) {
-                    const int pb = a.bt_pbeg[t], pe = a.bt_pbeg[t + 1];
+                    const int4 bt = a.btasks[t];
                     FT acc = (FT)0;
-                    for (int p = pb + tpos; p < pe; p += team) acc += XL(a.bp_u[p]) * XL(a.bp_x[p]);
-                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
-                    if (tpos == 0) {
-                        const int o = a.bt_out[t];
-                        XS(o, (XL(o) - acc) * XL(a.bt_diag[t]));
+                    int p = bt.y + tpos;
+                    for (; p + 3 * team < bt.z; p += 4 * team) {
+                        const int2 q0 = a.bpairs[p], q1 = a.bpairs[p + team];
+                        const int2 q2 = a.bpairs[p + 2 * team], q3 = a.bpairs[p + 3 * team];
+                        acc += (XL(q0.x) * XL(q0.y) + XL(q1.x) * XL(q1.y)) + (XL(q2.x) * XL(q2.y) + XL(q3.x) * XL(q3.y));
                     }
+                    for (; p < bt.z; p += team) {
+                        const int2 q = a.bpairs[p];
+                        acc += XL(q.x) * XL(q.y);
+                    }
+                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
+                    if (tpos == 0) XS(bt.x, (XL(bt.x) - acc) * XL(bt.w));
                 }
             }
             __syncthreads();
@@ -345,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_nr(const KArgs a) {
                 V[2 * b + 1] = m * sn;
                 vmin = fmin(vmin, m);
             }
-        vmin = inst_reduce<G, false>(vmin, s_part);
+        vmin = inst_reduce<G, NT, false>(vmin, s_part);
         if (act && vmin < 0.0)
             for (int b = s; b < n; b += nslot) {
                 const double vr = V[2 * b], vi = V[2 * b + 1];
@@ -445,39 +487,45 @@ struct pfgpu_grid {
     pfgpu::JPattern J;
     pfgpu::DevicePlan P;
     pfgpu_grid_info info{};
-    DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, f_lvl, t_out, t_pbeg,
-        t_jsrc, p_l, p_u, p_d, b_lvl, bt_out, bt_pbeg, bt_diag, bp_u, bp_x;
+    DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots,
+        f_lvl, tasks, pairs, b_lvl, btasks, bpairs;
     DevBuf<int8_t> ux_kind;
-    DevBuf<uint8_t> j_sel, t_kind;
+    DevBuf<uint8_t> j_sel;
     DevBuf<double> j_cval, w;
-    // per-G team tables
+    // per-(G, block size) team tables
     struct Teams {
         DevBuf<int> f, b;
     };
-    std::vector<std::unique_ptr<Teams>> teams;  // index log2(G)
+    std::vector<std::unique_ptr<Teams>> teams;  // index 2*log2(G) + (block == 1024)
 };
 
 namespace {
 
-template <typename FT, int G>
+template <typename FT, int G, int NT>
 void launch_nr(const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
-    auto kfn = k_nr<FT, G>;
+    auto kfn = k_nr<FT, G, NT>;
     if (lds > 65536) HIP_OK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(NT), lds, st, a);
     HIP_OK(hipGetLastError());
 }
 
-template <typename FT>
+template <typename FT, int NT>
 void launch_nr_g(int G, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
     switch (G) {
-        case 1: launch_nr<FT, 1>(a, nblocks, lds, st); break;
-        case 2: launch_nr<FT, 2>(a, nblocks, lds, st); break;
-        case 4: launch_nr<FT, 4>(a, nblocks, lds, st); break;
-        case 8: launch_nr<FT, 8>(a, nblocks, lds, st); break;
-        case 16: launch_nr<FT, 16>(a, nblocks, lds, st); break;
-        case 32: launch_nr<FT, 32>(a, nblocks, lds, st); break;
+        case 1: launch_nr<FT, 1, NT>(a, nblocks, lds, st); break;
+        case 2: launch_nr<FT, 2, NT>(a, nblocks, lds, st); break;
+        case 4: launch_nr<FT, 4, NT>(a, nblocks, lds, st); break;
+        case 8: launch_nr<FT, 8, NT>(a, nblocks, lds, st); break;
+        case 16: launch_nr<FT, 16, NT>(a, nblocks, lds, st); break;
+        case 32: launch_nr<FT, 32, NT>(a, nblocks, lds, st); break;
         default: throw ApiError(PFGPU_E_INVALID, "instances_per_wg must be 1, 2, 4, 8, 16 or 32");
     }
+}
+
+template <typename FT>
+void launch_nr_any(int G, int NT, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
+    if (NT == 1024) launch_nr_g<FT, 1024>(G, a, nblocks, lds, st);
+    else launch_nr_g<FT, 256>(G, a, nblocks, lds, st);
 }
 
 int log2i(int g) {
@@ -487,7 +535,7 @@ int log2i(int g) {
 }
 
 struct LaunchChoice {
-    int G, nlds, nspill, nblocks;
+    int G, NT, nlds, nspill, nblocks;
     size_t lds;
 };
 
@@ -495,7 +543,8 @@ struct LaunchChoice {
 // keeping at least two workgroups per CU; otherwise G = 1 with the hottest
 // slots in LDS and the rest spilled to global memory.
 LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req) {
-    const size_t lds_avail = (size_t)std::max(0, ctx->lds_max - 2048);  // keep room for static LDS
+    // keep room for the kernel's static LDS (per-instance scalars and the reduction buffer: < 5 KB)
+    const size_t lds_avail = (size_t)std::max(0, ctx->lds_max - 8192);
     LaunchChoice c{};
     if (G_req > 0) {
         c.G = G_req;
@@ -519,17 +568,22 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     c.nspill = P.nvals - c.nlds;
     c.nblocks = (ninst + c.G - 1) / c.G;
     c.lds = (size_t)c.nlds * c.G * fsize;
+    // one workgroup per CU when the value arrays take more than half of the LDS: give it 16 waves
+    c.NT = c.lds > lds_avail / 2 ? 1024 : 256;
     return c;
 }
 
-void ensure_teams(pfgpu_grid* gr, int G) {
-    const int lg = log2i(G);
-    if ((int)gr->teams.size() <= lg) gr->teams.resize(lg + 1);
-    if (gr->teams[lg]) return;
+int team_index(int G, int NT) { return 2 * log2i(G) + (NT == 1024 ? 1 : 0); }
+
+void ensure_teams(pfgpu_grid* gr, int G, int NT) {
+    const int ix = team_index(G, NT);
+    if ((int)gr->teams.size() <= ix) gr->teams.resize(ix + 1);
+    if (gr->teams[ix]) return;
     auto t = std::make_unique<pfgpu_grid::Teams>();
-    t->f.upload(pfgpu::level_teams(gr->P.f_lvl, gr->P.f_maxp, G, kBlock / G));
-    t->b.upload(pfgpu::level_teams(gr->P.b_lvl, gr->P.b_maxp, G, kBlock / G));
-    gr->teams[lg] = std::move(t);
+    t->f.upload(pfgpu::level_teams(gr->P.f_lvl, gr->P.f_maxp, G, NT / G));
+    t->b.upload(pfgpu::level_teams(gr->P.b_lvl, gr->P.b_maxp, G, NT / G));
+    if (!t->f.p) t->f.upload(std::vector<int>(1, 1));
+    gr->teams[ix] = std::move(t);
 }
 
 void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
@@ -561,7 +615,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     if (max_iter < 0) throw ApiError(PFGPU_E_INVALID, "max_iter must be >= 0");
     const int fsize = precision == PFGPU_FP32 ? 4 : 8;
     LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req);
-    ensure_teams(gr, lc.G);
+    ensure_teams(gr, lc.G, lc.NT);
     const bool deltas = d_dptr != nullptr;
     size_t off = 0;
     const size_t o_y = off;
@@ -591,25 +645,28 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
         HIP_OK(hipGetLastError());
     }
     KArgs a{};
-    a.n = n; a.N = P.N; a.nlev = P.nlev; a.nblev = P.nblev; a.nlds = lc.nlds; a.nspill = lc.nspill;
+    a.n = n; a.N = P.N; a.nJ = P.nJ; a.nzero = (int)P.zero_slots.size(); a.nlev = P.nlev; a.nblev = P.nblev;
+    a.nlds = lc.nlds; a.nspill = lc.nspill;
     a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
     a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
     a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
-    auto& tm = *gr->teams[log2i(lc.G)];
-    a.f_lvl = gr->f_lvl.p; a.f_team = tm.f.p; a.t_out = gr->t_out.p; a.t_pbeg = gr->t_pbeg.p; a.t_jsrc = gr->t_jsrc.p;
-    a.t_kind = gr->t_kind.p; a.p_l = gr->p_l.p; a.p_u = gr->p_u.p; a.p_d = gr->p_d.p;
-    a.b_lvl = gr->b_lvl.p; a.b_team = tm.b.p; a.bt_out = gr->bt_out.p; a.bt_pbeg = gr->bt_pbeg.p; a.bt_diag = gr->bt_diag.p;
-    a.bp_u = gr->bp_u.p; a.bp_x = gr->bp_x.p; a.w = gr->w.p;
+    a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
+    auto& tm = *gr->teams[team_index(lc.G, lc.NT)];
+    a.f_lvl = gr->f_lvl.p; a.f_team = tm.f.p;
+    a.tasks = reinterpret_cast<const int4*>(gr->tasks.p); a.pairs = reinterpret_cast<const int4*>(gr->pairs.p);
+    a.b_lvl = gr->b_lvl.p; a.b_team = tm.b.p;
+    a.btasks = reinterpret_cast<const int4*>(gr->btasks.p); a.bpairs = reinterpret_cast<const int2*>(gr->bpairs.p);
+    a.w = gr->w.p;
     a.Yx = Ypt; a.y_stride = ystride;
     a.S = d_S; a.s_stride = (long long)sbus_stride * 2;
     a.V = d_V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
     a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
     HIP_OK(hipEventRecord(ctx->ev0, stream));
-    if (precision == PFGPU_FP32) launch_nr_g<float>(lc.G, a, lc.nblocks, lc.lds, stream);
-    else launch_nr_g<double>(lc.G, a, lc.nblocks, lc.lds, stream);
+    if (precision == PFGPU_FP32) launch_nr_any<float>(lc.G, lc.NT, a, lc.nblocks, lc.lds, stream);
+    else launch_nr_any<double>(lc.G, lc.NT, a, lc.nblocks, lc.lds, stream);
     HIP_OK(hipEventRecord(ctx->ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     if (st) {
@@ -727,23 +784,21 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         g->info.n_slots = P.nvals;
         g->info.n_levels = P.nlev;
         g->info.n_back_levels = P.nblev;
-        g->info.n_pairs = (int64_t)P.p_l.size();
+        g->info.n_pairs = (int64_t)P.pairs.size() / 4;
         g->info.flops = P.flops;
         g->info.single_slack = single_slack ? 1 : 0;
         g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
         g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
         g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
         g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
-        g->j_cval.upload(P.j_cval);
-        g->f_lvl.upload(P.f_lvl); g->t_out.upload(P.t_out); g->t_pbeg.upload(P.t_pbeg); g->t_jsrc.upload(P.t_jsrc);
-        g->t_kind.upload(P.t_kind);
-        g->p_l.upload(P.p_l); g->p_u.upload(P.p_u); g->p_d.upload(P.p_d);
-        g->b_lvl.upload(P.b_lvl); g->bt_out.upload(P.bt_out); g->bt_pbeg.upload(P.bt_pbeg); g->bt_diag.upload(P.bt_diag);
-        g->bp_u.upload(P.bp_u); g->bp_x.upload(P.bp_x);
+        g->j_cval.upload(P.j_cval); g->j_slot.upload(P.j_slot);
+        g->zero_slots.upload(P.zero_slots.empty() ? std::vector<int>(1, 0) : P.zero_slots);
+        g->f_lvl.upload(P.f_lvl);
+        g->tasks.upload(P.tasks.empty() ? std::vector<int>(4, 0) : P.tasks);
+        g->pairs.upload(P.pairs.empty() ? std::vector<int>(4, 0) : P.pairs);
+        g->b_lvl.upload(P.b_lvl); g->btasks.upload(P.btasks);
+        g->bpairs.upload(P.bpairs.empty() ? std::vector<int>(2, 0) : P.bpairs);
         g->w.upload(std::vector<double>(slack_weights, slack_weights + n));
-        // buffers that may be empty still need a valid pointer for the kernel
-        if (!g->p_l.p) { std::vector<int> z(1, 0); g->p_l.upload(z); g->p_u.upload(z); g->p_d.upload(z); }
-        if (!g->bp_u.p) { std::vector<int> z(1, 0); g->bp_u.upload(z); g->bp_x.upload(z); }
         *out = g.release();
         return 0;
     });

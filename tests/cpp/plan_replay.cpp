@@ -41,20 +41,21 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     for (int b = 0; b < n; ++b) slack_abs += Sc[b].real();
     auto rnd = [&](double v) { return use_fp32 ? (double)(float)v : v; };
 
-    // schedule conflict check: every slot read by a level must have been written at an earlier level
-    // (or be a y slot / J value read by the task itself), and no slot written twice
+    // schedule conflict check: no slot written twice by the factor tasks, every slot a task
+    // reads is written at an earlier level (or by the J fill / mismatch before the levels)
     {
         std::vector<int> wlev(P.nvals, -1);
         int conflicts = 0;
         for (int l = 0; l < P.nlev; ++l)
             for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
-                if (wlev[P.t_out[t]] >= 0) ++conflicts;
-                wlev[P.t_out[t]] = l;
+                if (wlev[P.tasks[4 * t]] >= 0) ++conflicts;
+                wlev[P.tasks[4 * t]] = l;
             }
         for (int l = 0; l < P.nlev; ++l)
             for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t)
-                for (int p = P.t_pbeg[t]; p < P.t_pbeg[t + 1]; ++p)
-                    if (wlev[P.p_l[p]] >= l || wlev[P.p_u[p]] >= l || wlev[P.p_d[p]] >= l) ++conflicts;
+                for (int p = P.tasks[4 * t + 1]; p < P.tasks[4 * t + 2]; ++p)
+                    for (int k = 0; k < 3; ++k)
+                        if (wlev[P.pairs[4 * p + k]] >= l) ++conflicts;
         *conflicts_out = conflicts;
     }
 
@@ -92,26 +93,30 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     int it = 0, status = 0;
     while (!conv && it < max_iter) {
         ++it;
+        // J fill (zero the fill-in slots, write every J entry; reciprocal for pivots with no pairs)
+        for (int z : P.zero_slots) X[z] = 0.0;
+        for (int q = 0; q < P.nJ; ++q) {
+            const double v = jval(q);
+            if (Jx_out) Jx_out[q] = v;
+            const int sl = P.j_slot[q] & ((1 << 30) - 1);
+            X[sl] = (P.j_slot[q] >> 30) & 1 ? rnd(1.0 / rnd(v)) : rnd(v);
+        }
         for (int l = 0; l < P.nlev; ++l)
             for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
-                const int o = P.t_out[t];
-                const int kind = P.t_kind[t];
-                double a = 0.0;
-                if (kind == 2) a = X[o];  // F already in place
-                else if (P.t_jsrc[t] >= 0) {
-                    a = rnd(jval(P.t_jsrc[t]));
-                    if (Jx_out) Jx_out[P.t_jsrc[t]] = jval(P.t_jsrc[t]);
-                }
+                const int o = P.tasks[4 * t], kind = P.tasks[4 * t + 3];
                 double acc = 0;
-                for (int p = P.t_pbeg[t]; p < P.t_pbeg[t + 1]; ++p) acc += X[P.p_l[p]] * X[P.p_u[p]] * X[P.p_d[p]];
-                double v = rnd(a - acc);
+                for (int p = P.tasks[4 * t + 1]; p < P.tasks[4 * t + 2]; ++p)
+                    acc += X[P.pairs[4 * p]] * X[P.pairs[4 * p + 1]] * X[P.pairs[4 * p + 2]];
+                const double v = rnd(X[o] - acc);
                 X[o] = kind == 1 ? rnd(1.0 / v) : v;
             }
         for (int l = 0; l < P.nblev; ++l)
             for (int t = P.b_lvl[l]; t < P.b_lvl[l + 1]; ++t) {
                 double acc = 0;
-                for (int p = P.bt_pbeg[t]; p < P.bt_pbeg[t + 1]; ++p) acc += X[P.bp_u[p]] * X[P.bp_x[p]];
-                X[P.bt_out[t]] = rnd((X[P.bt_out[t]] - acc) * X[P.bt_diag[t]]);
+                for (int p = P.btasks[4 * t + 1]; p < P.btasks[4 * t + 2]; ++p)
+                    acc += X[P.bpairs[2 * p]] * X[P.bpairs[2 * p + 1]];
+                const int o = P.btasks[4 * t];
+                X[o] = rnd((X[o] - acc) * X[P.btasks[4 * t + 3]]);
             }
         if (single_slack)
             for (int b = 0; b < n; ++b) { Va[b] = std::arg(Vc[b]); Vm[b] = std::abs(Vc[b]); }
