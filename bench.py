@@ -79,7 +79,12 @@ def main():
                                deltas=(ddp, ddq, ddv), instances_per_wg=args.instances_per_wg)
 
     results = {}
+    tuned = {}
     for prec, name in ((abi.FP64, "fp64"), (abi.FP32, "fp32")):
+        if not args.instances_per_wg:
+            # the project's launch autotuner (instances per workgroup x block size), untimed
+            tuned[name] = gr.autotune_device(dY, 0, dS, 0, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec,
+                                             deltas=(ddp, ddq, ddv))
         for _ in range(args.warmup):
             sweep(prec)
         ksum = 0.0
@@ -99,7 +104,8 @@ def main():
             converged=conv, ninst=ninst, wall_s=wall, ms_per_step=1e3 * wall / args.steps,
             solves_per_s=conv * args.steps / wall, kernel_ms=1e3 * kavg, iters_mean=float(np.mean(iters)),
             achieved_gbs=bytes_launch / kavg / 1e9, bytes_per_launch=bytes_launch, lds_slots=st["lds_slots"],
-            instances_per_wg=st["instances_per_wg"], status_hist=np.bincount(status, minlength=8).tolist())
+            instances_per_wg=st["instances_per_wg"], status_hist=np.bincount(status, minlength=8).tolist(),
+            autotune=tuned.get(name))
 
     cpu = None
     if not args.no_cpu:

@@ -144,6 +144,35 @@ int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
                              int max_iter, double tol, int precision, int instances_per_wg,
                              int* d_iters, int* d_status, pfgpu_stats* stats);
 
+/*
+ * Launch autotuner: runs the batch described by the arguments (same meaning as
+ * pfgpu_solve_batch_device; outputs are overwritten) once per candidate launch
+ * shape — instances per workgroup G in {1, 2, 4, 8, 16, 32} x block size in
+ * {256, 1024} — keeps the fastest for this grid and precision, and returns it.
+ * Later solves with instances_per_wg == 0 use the tuned shape.
+ */
+int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
+                          const double* d_Yx, int Yx_stride,
+                          const int* d_dptr, const int* d_dpos, const double* d_dval,
+                          const double* d_Sbus, int sbus_stride,
+                          const double* d_V0, int v0_stride, double* d_V,
+                          int max_iter, double tol, int precision,
+                          int* d_iters, int* d_status, int* best_G, int* best_block, double* best_seconds);
+
+/*
+ * Branch flows at the origin side for every instance, on device-resident voltages
+ * (the post-processing of BaseBatchSolverSynch::compute_flows_from_Vs,
+ * src/batch_algorithm/BaseBatchSolverSynch.hpp:86-192):
+ *   I = yff Ef + yft Et,  S = Ef conj(I),
+ *   amps = |S| sn_mva / (sqrt(3) |Ef| vn_kv_from)  (kA),  p = Re(S) sn_mva  (MW).
+ * d_V is [ninst x n] complex; d_f/d_t/d_yff/d_yft/d_vn_kv_from describe nbr branches
+ * (complex admittances interleaved); d_amps / d_p are [ninst x nbr] (either may be NULL).
+ * A row of zero voltages gives NaN amps, as the reference's division does.
+ */
+int pfgpu_branch_flows_device(pfgpu_ctx* ctx, int ninst, int n, const double* d_V, int nbr,
+                              const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
+                              const double* d_vn_kv_from, double sn_mva, double* d_amps, double* d_p);
+
 /* Device memory helpers bound to the context's device and stream. */
 int pfgpu_dev_alloc(pfgpu_ctx* ctx, size_t bytes, void** dptr);
 int pfgpu_dev_free(pfgpu_ctx* ctx, void* dptr);

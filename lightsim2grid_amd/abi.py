@@ -63,6 +63,11 @@ SIGNATURES = {
     "pfgpu_solve_batch_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                                 ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.POINTER(Stats)]),
+    "pfgpu_autotune_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                             ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, _vp, _vp, _i32p, _i32p, _f64p]),
+    "pfgpu_branch_flows_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp,
+                                                 _vp, _vp, ctypes.c_double, _vp, _vp]),
     "pfgpu_dev_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "pfgpu_dev_free": (ctypes.c_int, [_vp, _vp]),
     "pfgpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
@@ -288,6 +293,17 @@ class Grid:
             out["J"] = sp.csc_matrix((Jx, Ji, Jp), shape=(N, N))
         return out
 
+    def autotune_device(self, d_Yx, yx_stride, d_S, s_stride, d_V0, v0_stride, d_V, ninst, d_iters, d_status,
+                        max_iter=10, tol=1e-8, precision=FP64, deltas=None):
+        """Time the launch shapes on this batch; the fastest becomes the default. Returns (G, block, seconds)."""
+        bG, bNT, bt = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        dp, dq, dv = deltas if deltas is not None else (None, None, None)
+        _check(self.ctx._L.pfgpu_autotune_device(
+            self.ctx.h, self.h, int(ninst), d_Yx.ptr, int(yx_stride), dp.ptr if dp else None, dq.ptr if dq else None,
+            dv.ptr if dv else None, d_S.ptr, int(s_stride), d_V0.ptr, int(v0_stride), d_V.ptr, int(max_iter),
+            float(tol), int(precision), d_iters.ptr, d_status.ptr, ctypes.byref(bG), ctypes.byref(bNT), ctypes.byref(bt)))
+        return bG.value, bNT.value, bt.value
+
     def solve_device(self, d_Yx, yx_stride, d_S, s_stride, d_V0, v0_stride, d_V, ninst, d_iters, d_status,
                      max_iter=10, tol=1e-8, precision=FP64, deltas=None, instances_per_wg=0):
         """Batched solve on device-resident arrays (DeviceArray); returns the stats dict."""
@@ -298,3 +314,12 @@ class Grid:
             dv.ptr if dv else None, d_S.ptr, int(s_stride), d_V0.ptr, int(v0_stride), d_V.ptr, int(max_iter),
             float(tol), int(precision), int(instances_per_wg), d_iters.ptr, d_status.ptr, ctypes.byref(st)))
         return st.as_dict()
+
+
+def branch_flows_device(ctx: Context, d_V: DeviceArray, ninst: int, n: int, d_f, d_t, d_yff, d_yft, d_vn, sn_mva: float,
+                        d_amps=None, d_p=None):
+    """Origin-side amps / active power of every branch for every instance (device arrays)."""
+    nbr = d_f.shape[0]
+    _check(ctx._L.pfgpu_branch_flows_device(ctx.h, int(ninst), int(n), d_V.ptr, int(nbr), d_f.ptr, d_t.ptr, d_yff.ptr,
+                                            d_yft.ptr, d_vn.ptr, float(sn_mva), d_amps.ptr if d_amps else None,
+                                            d_p.ptr if d_p else None))

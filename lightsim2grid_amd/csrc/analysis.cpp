@@ -163,7 +163,7 @@ static inline int find_sorted(const std::vector<int>& v, int b, int e, int x) {
 // S_k = struct(A(k+1:,k)) U  union over etree children c of (S_c \ {k}),
 // parent(k) = min S_k (George & Liu's symbolic Cholesky on A + A^T).
 // ---------------------------------------------------------------------------
-SymbolicLU symbolic_lu(const JPattern& J) {
+SymbolicLU symbolic_lu(const JPattern& J, int max_tail) {
     SymbolicLU S;
     const int N = J.N;
     S.N = N;
@@ -216,6 +216,14 @@ SymbolicLU symbolic_lu(const JPattern& J) {
         fl += 2.0 * c * c + c + 1;  // rank-1 update of the arm block, scaling, reciprocal
     }
     S.flops = fl;
+    // dense tail = the top chain of the elimination tree (parent(k) == k+1 up to the root)
+    int k0 = N - 1;
+    while (k0 > 0 && S.parent[k0 - 1] == k0) --k0;
+    int m = N - k0;
+    if (m > max_tail) m = max_tail;
+    if (m < 8) m = 0;
+    S.m = m;
+    S.k0 = N - m;
     return S;
 }
 
@@ -233,9 +241,10 @@ Schedule build_schedule(const SymbolicLU& S) {
         if (a > b) return (int)S.off_L + find_sorted(S.ci, S.cp[b], S.cp[b + 1], a);
         return (int)S.off_U + find_sorted(S.ci, S.cp[a], S.cp[a + 1], b);
     };
-    // count pairs per target
+    const int k0 = S.k0;
+    // count pairs per target (pivots of the dense tail are applied by the dense phase)
     std::vector<int> cnt(nv + 1, 0);
-    for (int k = 0; k < N; ++k) {
+    for (int k = 0; k < k0; ++k) {
         const int b0 = S.cp[k], b1 = S.cp[k + 1];
         for (int qa = b0; qa < b1; ++qa) {
             const int a = S.ci[qa];
@@ -248,7 +257,7 @@ Schedule build_schedule(const SymbolicLU& S) {
     const int64_t npairs = start[nv];
     std::vector<int> pl(npairs), pu(npairs), pd(npairs);
     std::vector<int> fillp(start.begin(), start.end() - 1);
-    for (int k = 0; k < N; ++k) {
+    for (int k = 0; k < k0; ++k) {
         const int b0 = S.cp[k], b1 = S.cp[k + 1];
         for (int qa = b0; qa < b1; ++qa) {
             const int a = S.ci[qa];
@@ -302,7 +311,7 @@ Schedule build_schedule(const SymbolicLU& S) {
     for (int64_t i = 0; i < nv; ++i) {
         const int t = order[i];
         sch.task_out[i] = t;
-        sch.task_diag[i] = (t < N) ? 1 : 0;
+        sch.task_diag[i] = (t < k0) ? 1 : 0;  // tail pivots are inverted by the dense phase
         sch.task_pbeg[i] = (int)pp;
         for (int p = start[t]; p < start[t + 1]; ++p, ++pp) {
             sch.pair_l[pp] = pl[p];
@@ -313,23 +322,23 @@ Schedule build_schedule(const SymbolicLU& S) {
         sch.max_pairs_per_task = std::max(sch.max_pairs_per_task, start[t + 1] - start[t]);
     }
     sch.npairs = npairs;
-    // backward levels
-    std::vector<int> bl(N, 0);
+    // backward levels (columns of the dense tail are solved by the dense phase first)
+    std::vector<int> bl(N, -1);  // tail columns: solved before the first level
     int nb = 0;
-    for (int j = N - 1; j >= 0; --j) {
+    for (int j = k0 - 1; j >= 0; --j) {
         int lv = 0;
         for (int q = S.cp[j]; q < S.cp[j + 1]; ++q) lv = std::max(lv, bl[S.ci[q]] + 1);
         bl[j] = lv;
         nb = std::max(nb, lv + 1);
     }
     std::vector<int> bc(nb + 1, 0);
-    for (int j = 0; j < N; ++j) bc[bl[j] + 1]++;
+    for (int j = 0; j < k0; ++j) bc[bl[j] + 1]++;
     for (int l = 0; l < nb; ++l) bc[l + 1] += bc[l];
     sch.blvl_ptr = bc;
-    sch.bcol.resize(N);
+    sch.bcol.resize(k0);
     {
         std::vector<int> f(bc.begin(), bc.end() - 1);
-        for (int j = 0; j < N; ++j) sch.bcol[f[bl[j]]++] = j;
+        for (int j = 0; j < k0; ++j) sch.bcol[f[bl[j]]++] = j;
     }
     return sch;
 }
@@ -388,11 +397,30 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
             hot[j] = std::max(hot[j], lv);
             for (int a = S.cp[j]; a < S.cp[j + 1]; ++a) hot[S.off_U + a] = std::max(hot[S.off_U + a], lv);
         }
-    std::vector<int> ord(S.nvals);
-    std::iota(ord.begin(), ord.end(), 0);
+    // dense tail first: tile [0, m*m) row-major, then its vector [m*m, m*m+m); every other
+    // slot follows by decreasing hotness
+    const int m = S.m, k0 = S.k0;
+    P.m = m;
+    std::vector<int> colof(S.nnzL);
+    for (int k = 0; k < N; ++k)
+        for (int q = S.cp[k]; q < S.cp[k + 1]; ++q) colof[q] = k;
+    std::vector<int> nw(S.nvals, -1);
+    for (int j = k0; j < N; ++j) {
+        nw[j] = (j - k0) * (m + 1);
+        nw[S.off_y + j] = m * m + (j - k0);
+        for (int q = S.cp[j]; q < S.cp[j + 1]; ++q) {
+            const int r = S.ci[q];  // > j >= k0
+            nw[S.off_L + q] = (r - k0) * m + (j - k0);
+            nw[S.off_U + q] = (j - k0) * m + (r - k0);
+        }
+    }
+    std::vector<int> ord;
+    for (int i = 0; i < (int)S.nvals; ++i)
+        if (nw[i] < 0) ord.push_back(i);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hot[a] > hot[b]; });
-    std::vector<int> nw(S.nvals);
-    for (int i = 0; i < (int)S.nvals; ++i) nw[ord[i]] = i;
+    const int tail_slots = m * m + m;
+    for (size_t i = 0; i < ord.size(); ++i) nw[ord[i]] = tail_slots + (int)i;
+    P.nvals = tail_slots + (int)ord.size();
     auto ys = [&](int k) { return nw[S.off_y + k]; };  // slot of y/x for permuted index k
 
     // --- Ybus row view ----------------------------------------------------
@@ -451,11 +479,18 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
             }
             const int slot = S.jslot[q];
             is_jslot[slot] = 1;
-            const bool recip = slot < N && npairs_of[slot] == 0;  // pivot with nothing to eliminate
+            const bool recip = slot < k0 && npairs_of[slot] == 0;  // pivot with nothing to eliminate
             P.j_slot[q] = nw[slot] | (recip ? (1 << 30) : 0);
         }
-    for (int64_t sl = 0; sl < S.off_y; ++sl)
-        if (!is_jslot[sl]) P.zero_slots.push_back(nw[sl]);
+    {
+        std::vector<char> jfinal(P.nvals, 0);
+        for (int64_t sl = 0; sl < S.off_y; ++sl)
+            if (is_jslot[sl]) jfinal[nw[sl]] = 1;
+        std::vector<char> isy(P.nvals, 0);
+        for (int k = 0; k < N; ++k) isy[nw[S.off_y + k]] = 1;
+        for (int sl = 0; sl < P.nvals; ++sl)
+            if (!jfinal[sl] && !isy[sl]) P.zero_slots.push_back(sl);
+    }
     // --- factor tasks (tasks without pairs are covered by the J fill / mismatch) ---------
     std::vector<int> lvl(1, 0);
     for (int l = 0; l < nlev; ++l) {
@@ -485,7 +520,7 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
     // --- backward tasks ---------------------------------------------------------
     P.b_lvl = sch.blvl_ptr;
     P.b_maxp.assign(nblev, 0);
-    P.btasks.resize(4 * N);
+    P.btasks.resize(4 * (size_t)k0);
     int bp = 0;
     for (int l = 0; l < nblev; ++l)
         for (int q = sch.blvl_ptr[l]; q < sch.blvl_ptr[l + 1]; ++q) {

@@ -65,11 +65,15 @@ struct SymbolicLU {
     int64_t off_L = 0, off_U = 0, off_y = 0;
     // J entry t -> value slot
     std::vector<int> jslot;
+    // dense tail: the trailing m = N - k0 columns (the top chain of the elimination tree) are
+    // factored as one dense m x m block; the Crout tasks only apply pivots k < k0 to them
+    int k0 = 0, m = 0;
     // flops of one refactorisation (multiply-adds counted as 2, divisions 1)
     double flops = 0;
 };
 
-SymbolicLU symbolic_lu(const JPattern& J);
+// max_tail: largest dense tail (0 disables it)
+SymbolicLU symbolic_lu(const JPattern& J, int max_tail = 96);
 
 struct Schedule {
     // Factor + forward-solve tasks, grouped by level.
@@ -95,6 +99,8 @@ Schedule build_schedule(const SymbolicLU& S);
 // in LDS and spill the rest to a per-instance global buffer.
 struct DevicePlan {
     int n = 0, N = 0, off = 0, nnzY = 0, nJ = 0, nvals = 0, slack0 = 0;
+    // dense tail: slots [0, m*m) hold the row-major m x m block, [m*m, m*m+m) its forward/backward vector
+    int m = 0;
     int nlev = 0, nblev = 0;
     // Ybus row view for Ibus = Y V: row r covers [yr_ptr[r], yr_ptr[r+1]) with column yr_col and CSC value pos yr_pos
     std::vector<int> yr_ptr, yr_col, yr_pos;

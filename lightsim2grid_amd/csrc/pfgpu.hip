@@ -79,7 +79,7 @@ int guarded(F&& f) {
 // Kernel arguments
 // ---------------------------------------------------------------------------
 struct KArgs {
-    int n, N, nJ, nzero, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter;
+    int n, N, nJ, nzero, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter, m;
     double tol;
     const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
     const int8_t* ux_kind;
@@ -137,13 +137,43 @@ __device__ __forceinline__ double inst_reduce(double v, double (*part)[kMaxG]) {
     return r;
 }
 
+// J value at the current V / Ibus (dS_dVa / dS_dVm, BaseNRAlgo.tpp:202-260) of a J entry
+// with selector sel (0/1: Re/Im dS_dVa, 2/3: Re/Im dS_dVm, 4: constant cv) built on the
+// Ybus entry y = Y(r, c); diag says r == c.
+__device__ __forceinline__ double jvalue_of(int sel, bool diag, c64 y, c64 vr, c64 vc, c64 ib, double cv) {
+    if (sel == 4) return cv;
+    if (sel >= 2) {
+        const double ic = 1.0 / sqrt(vc.re * vc.re + vc.im * vc.im);
+        c64 d = cmul(cconj(cmul(y, {vc.re * ic, vc.im * ic})), vr);
+        if (diag) {
+            const double ir = 1.0 / sqrt(vr.re * vr.re + vr.im * vr.im);
+            d = cadd(d, cmul(cconj(ib), {vr.re * ir, vr.im * ir}));
+        }
+        return sel == 2 ? d.re : d.im;
+    }
+    c64 t = cmul(y, vc);
+    if (diag) t = csub(t, ib);
+    const c64 d = cmul(cconj({-t.re, -t.im}), {-vr.im, vr.re});  // * (j V_r)
+    return sel == 0 ? d.re : d.im;
+}
+
+// Orders LDS accesses between the lanes of one wave (they run in lockstep; the
+// fence makes the compiler wait for outstanding LDS operations and not reorder).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double inf_if_nonfinite(double a) { return (a <= 1.7976931348623157e308) ? a : INFINITY; }
 
 template <typename FT, int G, int NT>
 __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     constexpr int kWaves = NT / 64;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    FT* Xl = reinterpret_cast<FT*>(lds_raw);
+    // explicit LDS address space: every access is a ds_read/ds_write, never a flat
+    // pointer into LDS (flat pointers into LDS mixed with global ones trip the gfx950 backend)
+    typedef __attribute__((address_space(3))) FT lds_ft;
+    lds_ft* Xl = (lds_ft*)(lds_raw);
     __shared__ double s_part[kWaves][kMaxG];
     __shared__ double s_slack[kMaxG];
     __shared__ int s_done[kMaxG], s_it[kMaxG], s_status[kMaxG], s_piv[kMaxG];
@@ -228,28 +258,6 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     }
     __syncthreads();
 
-    // J value of J entry q at the current V / Ibus (dS_dVa / dS_dVm, BaseNRAlgo.tpp:202-260)
-    auto jvalue = [&](int q) -> double {
-        const int sel = a.j_sel[q];
-        if (sel == 4) return a.j_cval[q];
-        const int r = a.j_rbus[q], c = a.j_cbus[q];
-        const c64 y = ld_c(Y, a.j_ypos[q]);
-        const c64 vr = ld_c(V, r), vc = ld_c(V, c);
-        if (sel >= 2) {
-            const double ic = 1.0 / sqrt(vc.re * vc.re + vc.im * vc.im);
-            c64 d = cmul(cconj(cmul(y, {vc.re * ic, vc.im * ic})), vr);
-            if (r == c) {
-                const double ir = 1.0 / sqrt(vr.re * vr.re + vr.im * vr.im);
-                d = cadd(d, cmul(cconj(ld_c(Ib, r)), {vr.re * ir, vr.im * ir}));
-            }
-            return sel == 2 ? d.re : d.im;
-        }
-        c64 t = cmul(y, vc);
-        if (r == c) t = csub(t, ld_c(Ib, r));
-        const c64 d = cmul(cconj({-t.re, -t.im}), {-vr.im, vr.re});  // * (j V_r)
-        return sel == 0 ? d.re : d.im;
-    };
-
     for (;;) {
         if (tid == 0) {
             int nd = 0;
@@ -262,18 +270,44 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (s == 0 && act) s_it[g] += 1;
 
         // ---- J fill: zero the fill-in slots, write every J entry --------------------
+        // (4 entries per lane per step: their loads are issued together)
         if (act) {
             for (int i = s; i < a.nzero; i += nslot) XS(a.zero_slots[i], (FT)0);
-            for (int q = s; q < a.nJ; q += nslot) {
-                const double jv = jvalue(q);
-                if (a.Jx_last != nullptr && inst == 0) a.Jx_last[q] = jv;
-                const int js = a.j_slot[q];
-                FT v = (FT)jv;
-                if (js & (1 << 30)) {  // pivot with nothing to eliminate
-                    if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                    v = (FT)1 / v;
+            for (int q0 = s; q0 < a.nJ; q0 += 4 * nslot) {
+                int qq[4], sel[4], rb[4], cb[4], yp[4], js[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = q0 + u * nslot;
+                    qq[u] = q < a.nJ ? q : -1;
+                    const int qc = q < a.nJ ? q : 0;
+                    sel[u] = a.j_sel[qc];
+                    rb[u] = a.j_rbus[qc];
+                    cb[u] = a.j_cbus[qc];
+                    yp[u] = a.j_ypos[qc];
+                    js[u] = a.j_slot[qc];
                 }
-                XS(js & ((1 << 30) - 1), v);
+                c64 y[4], vr[4], vc[4], ib[4];
+                double cv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    y[u] = ld_c(Y, yp[u] >= 0 ? yp[u] : 0);
+                    vr[u] = ld_c(V, rb[u]);
+                    vc[u] = ld_c(V, cb[u]);
+                    ib[u] = ld_c(Ib, rb[u]);
+                    cv[u] = a.j_cval[qq[u] >= 0 ? qq[u] : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (qq[u] < 0) continue;
+                    const double jv = jvalue_of(sel[u], rb[u] == cb[u], y[u], vr[u], vc[u], ib[u], cv[u]);
+                    if (a.Jx_last != nullptr && inst == 0) a.Jx_last[qq[u]] = jv;
+                    FT v = (FT)jv;
+                    if (js[u] & (1 << 30)) {  // pivot with nothing to eliminate
+                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                        v = (FT)1 / v;
+                    }
+                    XS(js[u] & ((1 << 30) - 1), v);
+                }
             }
         }
         __syncthreads();
@@ -326,6 +360,42 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
             __syncthreads();
         }
+        // ---- dense tail: wave 0 factors the m x m tile of every instance --------------
+        // (tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m); both in LDS)
+        if (a.m > 0) {
+            if (tid < 64 && act) {
+                const int m = a.m;
+                constexpr int nl = 64 / G;
+                const int r = tid / G;
+                // plain index arithmetic on the LDS array (no generic pointers into LDS)
+                const int yb = m * m;
+                for (int k = 0; k < m; ++k) {
+                    const FT d = Xl[(k * m + k) * G + g];
+                    if (r == 0 && (!(fabs((double)d) > 0.0) || !isfinite((double)d))) s_piv[g] = 1;
+                    const FT rd = (FT)1 / d;
+                    for (int i = k + 1 + r; i < m; i += nl) Xl[(i * m + k) * G + g] *= rd;
+                    wave_sync();
+                    const int w = m - k - 1;
+                    for (int idx = r; idx < w * w; idx += nl) {
+                        const int i = k + 1 + idx / w, j = k + 1 + idx % w;
+                        Xl[(i * m + j) * G + g] -= Xl[(i * m + k) * G + g] * Xl[(k * m + j) * G + g];
+                    }
+                    const FT yk = Xl[(yb + k) * G + g];
+                    for (int i = k + 1 + r; i < m; i += nl) Xl[(yb + i) * G + g] -= Xl[(i * m + k) * G + g] * yk;
+                    wave_sync();
+                    if (r == 0) Xl[(k * m + k) * G + g] = rd;
+                }
+                wave_sync();
+                for (int k = m - 1; k >= 0; --k) {
+                    const FT x = Xl[(yb + k) * G + g] * Xl[(k * m + k) * G + g];
+                    wave_sync();
+                    if (r == 0) Xl[(yb + k) * G + g] = x;
+                    for (int i = r; i < k; i += nl) Xl[(yb + i) * G + g] -= Xl[(i * m + k) * G + g] * x;
+                    wave_sync();
+                }
+            }
+            __syncthreads();
+        }
         if (s == 0 && act && s_piv[g]) {  // zero / non-finite pivot: klu_refactor failure
             s_status[g] = 6;
             s_done[g] = 1;
@@ -354,7 +424,15 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                         acc += XL(q.x) * XL(q.y);
                     }
                     for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
-                    if (tpos == 0) XS(bt.x, (XL(bt.x) - acc) * XL(bt.w));
+                    if (tpos == 0) {
+                        const FT rd = XL(bt.w);
+                        if (bt.x < nlds) {
+                            Xl[bt.x * G + g] = (Xl[bt.x * G + g] - acc) * rd;
+                        } else {
+                            FT* xp = &Xgb[(size_t)(bt.x - nlds) * G + g];
+                            *xp = (*xp - acc) * rd;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -444,6 +522,30 @@ __global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n
     }
 }
 
+// Branch flows at the origin side, one (instance, branch) per thread.
+__global__ void k_branch_flows(int ninst, int n, const double* __restrict__ V, int nbr, const int* __restrict__ f,
+                               const int* __restrict__ t, const double* __restrict__ yff,
+                               const double* __restrict__ yft, const double* __restrict__ vn, double sn_mva,
+                               double* __restrict__ amps, double* __restrict__ p) {
+    const long long total = (long long)ninst * nbr;
+    const double inv_sqrt3 = 0.57735026918962576451;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long i = k / nbr;
+        const int b = (int)(k - i * nbr);
+        const double* Vi = V + 2 * i * (long long)n;
+        const c64 ef = ld_c(Vi, f[b]), et = ld_c(Vi, t[b]);
+        const c64 I = cadd(cmul(ld_c(yff, b), ef), cmul(ld_c(yft, b), et));
+        const c64 S = cmul(ef, cconj(I));
+        if (p) p[k] = S.re * sn_mva;
+        if (amps) {
+            const double sabs = sqrt(S.re * S.re + S.im * S.im);
+            const double eabs = sqrt(ef.re * ef.re + ef.im * ef.im);
+            amps[k] = sabs * sn_mva * inv_sqrt3 / (eabs * vn[b]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -497,6 +599,8 @@ struct pfgpu_grid {
         DevBuf<int> f, b;
     };
     std::vector<std::unique_ptr<Teams>> teams;  // index 2*log2(G) + (block == 1024)
+    // autotuned launch shape per precision (G, block), 0 = not tuned
+    int tuned_G[2] = {0, 0}, tuned_NT[2] = {0, 0};
 };
 
 namespace {
@@ -542,7 +646,8 @@ struct LaunchChoice {
 // Launch heuristic: the largest G whose value arrays fit entirely in LDS while
 // keeping at least two workgroups per CU; otherwise G = 1 with the hottest
 // slots in LDS and the rest spilled to global memory.
-LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req) {
+LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req,
+                           int NT_req = 0) {
     // keep room for the kernel's static LDS (per-instance scalars and the reduction buffer: < 5 KB)
     const size_t lds_avail = (size_t)std::max(0, ctx->lds_max - 8192);
     LaunchChoice c{};
@@ -563,6 +668,8 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
             }
         }
     }
+    // the dense tail must be LDS-resident
+    while (c.G > 1 && (size_t)(P.m * P.m + P.m) * c.G * fsize > lds_avail) c.G >>= 1;
     const size_t cap = lds_avail / ((size_t)c.G * fsize);
     c.nlds = (int)std::min<size_t>((size_t)P.nvals, cap);
     c.nspill = P.nvals - c.nlds;
@@ -570,6 +677,7 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     c.lds = (size_t)c.nlds * c.G * fsize;
     // one workgroup per CU when the value arrays take more than half of the LDS: give it 16 waves
     c.NT = c.lds > lds_avail / 2 ? 1024 : 256;
+    if (NT_req == 256 || NT_req == 1024) c.NT = NT_req;
     return c;
 }
 
@@ -604,7 +712,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
                   const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
                   int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
-                  double* d_Jx_last, pfgpu_stats* st) {
+                  double* d_Jx_last, pfgpu_stats* st, int NT_req = 0) {
     const pfgpu::DevicePlan& P = gr->P;
     const int n = P.n;
     if (ninst <= 0) return;
@@ -614,7 +722,11 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     if (v0_stride != 0 && v0_stride != n) throw ApiError(PFGPU_E_INVALID, "v0_stride must be 0 or n");
     if (max_iter < 0) throw ApiError(PFGPU_E_INVALID, "max_iter must be >= 0");
     const int fsize = precision == PFGPU_FP32 ? 4 : 8;
-    LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req);
+    if (G_req == 0 && NT_req == 0 && gr->tuned_G[precision]) {
+        G_req = gr->tuned_G[precision];
+        NT_req = gr->tuned_NT[precision];
+    }
+    LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req, NT_req);
     ensure_teams(gr, lc.G, lc.NT);
     const bool deltas = d_dptr != nullptr;
     size_t off = 0;
@@ -646,6 +758,8 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     }
     KArgs a{};
     a.n = n; a.N = P.N; a.nJ = P.nJ; a.nzero = (int)P.zero_slots.size(); a.nlev = P.nlev; a.nblev = P.nblev;
+    a.m = P.m;
+    if (lc.nlds < P.m * P.m + P.m) throw ApiError(PFGPU_E_INTERNAL, "dense tail does not fit in LDS");
     a.nlds = lc.nlds; a.nspill = lc.nspill;
     a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
     a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
@@ -826,6 +940,62 @@ int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji) {
         if (!grid || !Jp || !Ji) throw ApiError(PFGPU_E_INVALID, "NULL argument");
         std::copy(grid->J.Jp.begin(), grid->J.Jp.end(), Jp);
         std::copy(grid->J.Ji.begin(), grid->J.Ji.end(), Ji);
+        return 0;
+    });
+}
+
+int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
+                          const int* d_dptr, const int* d_dpos, const double* d_dval, const double* d_Sbus,
+                          int sbus_stride, const double* d_V0, int v0_stride, double* d_V, int max_iter, double tol,
+                          int precision, int* d_iters, int* d_status, int* best_G, int* best_block,
+                          double* best_seconds) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !d_Yx || !d_Sbus || !d_V0 || !d_V || !d_iters || !d_status || ninst <= 0)
+            throw ApiError(PFGPU_E_INVALID, "invalid autotune arguments");
+        if (precision != PFGPU_FP64 && precision != PFGPU_FP32) throw ApiError(PFGPU_E_INVALID, "bad precision");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        double best = 1e300;
+        int bG = 1, bNT = 256;
+        for (int G = 1; G <= kMaxG; G <<= 1) {
+            for (int NT : {256, 1024}) {
+                if (64 / G < 1) continue;
+                LaunchChoice lc = choose_launch(ctx, grid->P, ninst, precision == PFGPU_FP32 ? 4 : 8, G, NT);
+                if (lc.G != G) continue;  // the dense tail did not fit at this G
+                double tmin = 1e300;
+                for (int rep = 0; rep < 2; ++rep) {
+                    pfgpu_stats st{};
+                    solve_device(ctx, grid, ninst, d_Yx, Yx_stride, d_dptr, d_dpos, d_dval, d_Sbus, sbus_stride, d_V0,
+                                 v0_stride, d_V, max_iter, tol, precision, G, d_iters, d_status, nullptr, &st, NT);
+                    tmin = std::min(tmin, st.kernel_s);
+                }
+                if (tmin < best) { best = tmin; bG = G; bNT = NT; }
+            }
+        }
+        grid->tuned_G[precision] = bG;
+        grid->tuned_NT[precision] = bNT;
+        if (best_G) *best_G = bG;
+        if (best_block) *best_block = bNT;
+        if (best_seconds) *best_seconds = best;
+        return 0;
+    });
+}
+
+int pfgpu_branch_flows_device(pfgpu_ctx* ctx, int ninst, int n, const double* d_V, int nbr, const int* d_f,
+                              const int* d_t, const double* d_yff, const double* d_yft, const double* d_vn_kv_from,
+                              double sn_mva, double* d_amps, double* d_p) {
+    return guarded([&]() -> int {
+        if (!ctx || ninst < 0 || n <= 0 || nbr < 0 || (nbr > 0 && (!d_V || !d_f || !d_t || !d_yff || !d_yft || !d_vn_kv_from)))
+            throw ApiError(PFGPU_E_INVALID, "invalid branch-flow arguments");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        const long long total = (long long)ninst * nbr;
+        if (total == 0 || (!d_amps && !d_p)) return 0;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_branch_flows, dim3(blocks), dim3(256), 0, ctx->stream, ninst, n, d_V, nbr, d_f, d_t, d_yff,
+                           d_yft, d_vn_kv_from, sn_mva, d_amps, d_p);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
 }

@@ -110,6 +110,26 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                 const double v = rnd(X[o] - acc);
                 X[o] = kind == 1 ? rnd(1.0 / v) : v;
             }
+        // dense tail: right-looking LU of the m x m tile (slots [0, m*m)), forward on its vector
+        // (slots [m*m, m*m+m)), then backward; the tile diagonal ends up holding 1/d
+        if (P.m > 0) {
+            const int m = P.m;
+            auto T = [&](int i, int j) -> double& { return X[i * m + j]; };
+            auto yt = [&](int i) -> double& { return X[m * m + i]; };
+            for (int k = 0; k < m; ++k) {
+                const double rd = rnd(1.0 / T(k, k));
+                for (int i = k + 1; i < m; ++i) T(i, k) = rnd(T(i, k) * rd);
+                for (int i = k + 1; i < m; ++i)
+                    for (int j = k + 1; j < m; ++j) T(i, j) = rnd(T(i, j) - T(i, k) * T(k, j));
+                for (int i = k + 1; i < m; ++i) yt(i) = rnd(yt(i) - T(i, k) * yt(k));
+                T(k, k) = rd;
+            }
+            for (int k = m - 1; k >= 0; --k) {
+                const double x = rnd(yt(k) * T(k, k));
+                yt(k) = x;
+                for (int i = 0; i < k; ++i) yt(i) = rnd(yt(i) - T(i, k) * x);
+            }
+        }
         for (int l = 0; l < P.nblev; ++l)
             for (int t = P.b_lvl[l]; t < P.b_lvl[l + 1]; ++t) {
                 double acc = 0;

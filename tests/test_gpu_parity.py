@@ -163,3 +163,55 @@ def test_restart_from_converged_voltages(gpu_ctx, oracle_mod):
         assert r["status"][0] == 0 and r["iters"][0] == o.get_nb_iter()
         if single:
             assert r["iters"][0] == 0
+
+
+def test_branch_flows_device(gpu_ctx, oracle_mod):
+    """Device branch flows on the batch's resident voltages vs the branch equations in numpy."""
+    from lightsim2grid_amd import abi
+    g, conts, deltas = _n1("case118_synth")
+    d = g.solver_inputs()
+    _say("branch flows")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    r = gr.solve(gr.ybus_values(d["Ybus"]), d["Sbus"], d["V0"], 10, 1e-8, deltas=deltas)
+    V = r["V"].copy()
+    V[3] = 0.0  # a row of zero voltages (non-converged contingency) -> NaN amps
+    yff, ytt, yft, ytf = g.branch_admittances()
+    vn = np.full(g.nbr, 138.0)
+    dV = gpu_ctx.to_device(V)
+    dA = gpu_ctx.empty((V.shape[0], g.nbr), np.float64)
+    dP = gpu_ctx.empty((V.shape[0], g.nbr), np.float64)
+    abi.branch_flows_device(gpu_ctx, dV, V.shape[0], g.n, gpu_ctx.to_device(g.f.astype(np.int32)),
+                            gpu_ctx.to_device(g.t.astype(np.int32)), gpu_ctx.to_device(yff), gpu_ctx.to_device(yft),
+                            gpu_ctx.to_device(vn), 100.0, dA, dP)
+    A, P = dA.to_host(), dP.to_host()
+    Ef, Et = V[:, g.f], V[:, g.t]
+    S = Ef * np.conj(yff * Ef + yft * Et)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        Aref = np.abs(S) * 100.0 / (np.sqrt(3) * np.abs(Ef) * vn)
+    ok = np.ones(V.shape[0], bool)
+    ok[3] = False
+    assert np.allclose(A[ok], Aref[ok], rtol=1e-12, atol=1e-12)
+    assert np.all(np.isnan(A[3]))
+    assert np.allclose(P, S.real * 100.0, rtol=1e-12, atol=1e-10)
+
+
+def test_autotuner_choice_keeps_results(gpu_ctx):
+    from lightsim2grid_amd import abi
+    g, conts, deltas = _n1("case118_synth")
+    d = g.solver_inputs()
+    _say("autotune")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    ref = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8, deltas=deltas)
+    n, ninst = g.n, len(conts)
+    dY, dS, dV0 = gpu_ctx.to_device(Yx), gpu_ctx.to_device(np.ascontiguousarray(d["Sbus"])), gpu_ctx.to_device(d["V0"])
+    dV = gpu_ctx.empty((ninst, n), np.complex128)
+    dIt, dSt = gpu_ctx.empty(ninst, np.int32), gpu_ctx.empty(ninst, np.int32)
+    dd = tuple(gpu_ctx.to_device(x) for x in deltas)
+    G, NT, t = gr.autotune_device(dY, 0, dS, 0, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=0, deltas=dd)
+    assert G in (1, 2, 4, 8, 16, 32) and NT in (256, 1024) and t > 0
+    st = gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=0, deltas=dd)
+    assert st["instances_per_wg"] == G
+    assert np.array_equal(dSt.to_host(), ref["status"])
+    assert np.array_equal(dIt.to_host(), ref["iters"])
+    assert np.max(np.abs(dV.to_host() - ref["V"])) <= 1e-12
