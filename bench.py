@@ -32,6 +32,171 @@ def algorithmic_bytes(info, iters, w):
     return float(np.sum(iters)) * per_it
 
 
+def serving_run(seconds: float, rate: float = 120.0, seed: int = 7) -> dict:
+    """Serving mix: Poisson arrivals over resident 14/118/2000/10000-bus grids (weights
+    0.4/0.3/0.2/0.1); single power flow 50 %, contingency batch of 1-256 outages 35 %,
+    time-series chunk of 1-96 steps 15 %; both precision forms."""
+    import threading
+
+    from lightsim2grid_amd import gridgen
+    from lightsim2grid_amd.lightsim2grid_amd_cpp import ServeLoop
+
+    names = ["case14_synth", "case118_synth", "case2000_synth", "case10000_synth"]
+    weights = np.array([0.4, 0.3, 0.2, 0.1])
+    grids = [gridgen.make(nm) for nm in names]
+    inputs = [g.solver_inputs() for g in grids]
+    nonbridge = [np.nonzero(~gridgen.bridges(g))[0] for g in grids]
+    out = {"workload": "Poisson arrivals %.0f req/s for %.1f s; grids 14/118/2000/10000 buses (0.4/0.3/0.2/0.1); "
+                       "single PF 50%%, N-1 batch 1-256 35%%, time-series chunk 1-96 15%%" % (rate, seconds)}
+    # warm-start voltages: each grid's base case, solved once (FP64) and kept resident
+    base_v = []
+    tmp = ServeLoop(0, 16, 0)
+    for d in inputs:
+        t0 = tmp.add_grid(d["Ybus"], d["Sbus"], d["V0"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+        base_v.append(tmp.result(tmp.submit_pf(t0, None), 120.0)["V"][0])
+    tmp.close()
+    for prec, pname in ((1, "fp32"), (0, "fp64")):
+        sl = ServeLoop(0, 4096, prec)
+        gids = [sl.add_grid(d["Ybus"], d["Sbus"], vb, d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+                for d, vb in zip(inputs, base_v)]
+        rng = np.random.default_rng(seed)
+        reqs = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        while True:
+            t_next += rng.exponential(1.0 / rate)
+            if t_next - t_start > seconds:
+                break
+            k = int(rng.choice(4, p=weights))
+            g = grids[k]
+            u = rng.random()
+            if u < 0.5:
+                kind, cnt = "pf", 1
+                payload = None
+            elif u < 0.85:
+                cnt = int(rng.integers(1, 257))
+                sel = rng.choice(nonbridge[k], min(cnt, len(nonbridge[k])), replace=False)
+                payload = g.ybus_deltas([[int(b)] for b in sel])
+                kind, cnt = "ca", len(sel)
+            else:
+                cnt = int(rng.integers(1, 97))
+                payload = gridgen.time_series_sbus(g, cnt, seed=int(rng.integers(1 << 30)))
+                kind = "ts"
+            reqs.append((t_next, k, kind, cnt, payload))
+        ids = []
+
+        def submitter():
+            for (ta, k, kind, cnt, payload) in reqs:
+                dt = ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                if kind == "pf":
+                    ids.append(sl.submit_pf(gids[k], None))
+                elif kind == "ca":
+                    ids.append(sl.submit_contingencies(gids[k], *payload))
+                else:
+                    ids.append(sl.submit_time_series(gids[k], payload))
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        th.join()
+        lat, solves, conv = [], 0, 0
+        for i in ids:
+            r = sl.result(i, 300.0)
+            lat.append(r["latency_s"])
+            solves += len(r["status"])
+            conv += int(np.sum(r["status"] == 0))
+        wall = time.perf_counter() - t0
+        launches = sl.launches()
+        sl.close()
+        lat = np.array(lat)
+        out[pname] = {"requests": len(ids), "solves": solves, "converged": conv, "wall_s": wall,
+                      "solves_per_s": conv / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                      "p99_ms": 1e3 * float(np.percentile(lat, 99)), "launches": launches}
+    return out
+
+
+def time_series_run(with_cpu: bool, T: int = 8760) -> dict:
+    """118-bus, 8760 hourly steps, warm-started chains (TimeSeriesCPP), both forms."""
+    import lightsim2grid_amd as L
+    from lightsim2grid_amd import gridgen
+    g = gridgen.make("case118_synth")
+    Sb = gridgen.time_series_sbus(g, T, seed=1)
+    gen_bus = np.r_[g.slack, g.pv]
+    gen_p = Sb[:, gen_bus].real * (gen_bus != g.slack)
+    load_p, load_q = -Sb[:, g.pq].real, -Sb[:, g.pq].imag
+    out = {"workload": "case118_synth, %d hourly steps, chains of 8 warm-started steps" % T}
+    for st, nm in ((L.SolverType.AMDGPUMixed, "fp32"), (L.SolverType.AMDGPU, "fp64")):
+        ts = L.TimeSeriesCPP(gridgen.to_gridmodel(g))
+        ts.change_solver(st)
+        ts.set_chain_length(8)
+        ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)  # warm-up
+        t0 = time.perf_counter()
+        status = ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)
+        wall = time.perf_counter() - t0
+        out[nm] = {"status": int(status), "steps": T, "wall_s": wall, "solves_per_s": T / wall,
+                   "solver_time_s": ts.solver_time()}
+    if with_cpu:
+        from oracle import oracle
+        d = g.solver_inputs()
+        nth = min(16, os.cpu_count() or 1)
+        _, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
+                                                         d["pq"], g.v0(), 10, 1e-8, False, nth)
+        out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nth, "kind": "port", "wall_s": wall}
+    return out
+
+
+def large_grid_run(with_cpu: bool, n_out: int = 5000, n_snap: int = 4) -> dict:
+    """10000-bus grid: n_out seeded non-bridge outages x n_snap load snapshots (load x U(0.8, 1.1))."""
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make("case10000_synth")
+    d = g.solver_inputs()
+    rng = np.random.default_rng(10000)
+    nb = np.nonzero(~gridgen.bridges(g))[0]
+    outs = np.sort(rng.choice(nb, min(n_out, len(nb)), replace=False))
+    scales = rng.uniform(0.8, 1.1, n_snap)
+    conts = [[int(b)] for b in outs for _ in range(n_snap)]
+    dptr, dpos, dval = g.ybus_deltas(conts)
+    S = np.stack([g.sbus(load_scale=sc, gen_scale=sc) for sc in scales])
+    Sall = np.ascontiguousarray(np.tile(S, (len(outs), 1)))
+    ninst = len(conts)
+    ctx = abi.Context(0)
+    t0 = time.perf_counter()
+    gr = abi.Grid(ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    t_load = time.perf_counter() - t0
+    Yx = gr.ybus_values(d["Ybus"])
+    base = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8)
+    Vb = base["V"][0]
+    dY, dS, dV0 = ctx.to_device(Yx), ctx.to_device(Sall), ctx.to_device(Vb)
+    dV = ctx.empty((ninst, g.n), np.complex128)
+    dIt, dSt = ctx.empty(ninst, np.int32), ctx.empty(ninst, np.int32)
+    dd = (ctx.to_device(dptr), ctx.to_device(dpos), ctx.to_device(dval))
+    out = {"workload": "case10000_synth: %d non-bridge outages x %d load snapshots = %d solves" % (len(outs), n_snap, ninst),
+           "grid_load_s": t_load, "n_jac": gr.info["n_jac"], "nnz_lu": gr.info["nnz_lu"]}
+    for prec, nm in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+        gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
+        t0 = time.perf_counter()
+        st = gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
+        wall = time.perf_counter() - t0
+        status = dSt.to_host()
+        conv = int(np.sum(status == 0))
+        out[nm] = {"wall_s": wall, "kernel_ms": 1e3 * st["kernel_s"], "converged": conv, "solves_per_s": conv / wall,
+                   "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"]}
+    if with_cpu:
+        from oracle import oracle
+        nth = min(16, os.cpu_count() or 1)
+        k = 16 * nth
+        sub = (dptr[:k + 1], dpos[:dptr[k]], dval[:dptr[k]])
+        _, _, _, conv, _, wall = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"],
+                                                    d["pq"], Vb, sub, 10, 1e-8, False, False, nth)
+        out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nth, "kind": "port",
+                               "sample": "%d outage solves (base injections), %.2f s" % (k, wall)}
+    gr.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +206,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0: min(16, nproc))")
     ap.add_argument("--instances-per-wg", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the serving / time-series / 10000-bus lines")
+    ap.add_argument("--serving-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the N-1 sweep would shard by "
@@ -126,6 +293,16 @@ def main():
                           f"{ninst}-outage sweep on {nthreads} threads, {wall_mt:.2f} s",
                    one_thread=dict(value=float(np.sum(cconv1)) / wall_1, sample=f"{k1} outages, {wall_1:.2f} s"))
 
+    extras = {}
+    if not args.no_extra:
+        for name, fn in (("serving", lambda: serving_run(args.serving_seconds)),
+                         ("time_series_118", lambda: time_series_run(not args.no_cpu)),
+                         ("contingency_10000", lambda: large_grid_run(not args.no_cpu))):
+            try:
+                extras[name] = fn()
+            except Exception as exc:  # an extra configuration never hides the headline line
+                extras[name] = {"error": repr(exc)}
+
     r32, r64 = results["fp32"], results["fp64"]
     out = {
         "metric": "power flows/sec at 1e-8 pu mismatch (N-1, 2000 bus)",
@@ -153,6 +330,7 @@ def main():
                       "converged": r64["converged"]},
         "fp32_over_fp64": r32["solves_per_s"] / r64["solves_per_s"],
         "details": {"fp32": r32, "fp64": r64},
+        "other_configs": extras,
     }
     print(json.dumps(out), flush=True)
 

@@ -178,6 +178,7 @@ int pfgpu_dev_alloc(pfgpu_ctx* ctx, size_t bytes, void** dptr);
 int pfgpu_dev_free(pfgpu_ctx* ctx, void* dptr);
 int pfgpu_memcpy_h2d(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
 int pfgpu_memcpy_d2h(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+int pfgpu_memcpy_d2d(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,7 @@ SIGNATURES = {
     "pfgpu_dev_free": (ctypes.c_int, [_vp, _vp]),
     "pfgpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "pfgpu_memcpy_d2h": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "pfgpu_memcpy_d2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
 }
 
 _lib = None

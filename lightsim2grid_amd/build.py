@@ -49,10 +49,10 @@ def build_module() -> str:
         return ""
     out = module_path()
     inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{ROOT}/include", f"-I{CSRC}"]
-    srcs = [src] + [os.path.join(CSRC, f) for f in ("gridmodel.cpp",) if os.path.exists(os.path.join(CSRC, f))]
+    srcs = [src] + [os.path.join(CSRC, f) for f in ("gridmodel.cpp", "serve.cpp") if os.path.exists(os.path.join(CSRC, f))]
     tmp = out + ".tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", *inc, *srcs, "-o", tmp,
-          f"-L{HERE}", "-lpfgpu", "-Wl,-rpath,$ORIGIN"])
+          f"-L{HERE}", "-lpfgpu", "-Wl,-rpath,$ORIGIN", "-pthread"])
     os.replace(tmp, out)
     return out
 

@@ -9,6 +9,7 @@
 #include <pybind11/stl.h>
 
 #include "host.hpp"
+#include "serve.hpp"
 
 namespace py = pybind11;
 using namespace ls2g;
@@ -239,6 +240,61 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
         })
         .def("get_voltages", [](const TimeSeries& t) { return to_np2(t.get_voltages(), t.nb_rows(), t.nb_bus_total()); })
         .def("get_sbuses", [](const TimeSeries& t) { return to_np2(t.get_sbuses(), t.nb_rows(), t.nb_bus_solver()); });
+
+    py::class_<ServeLoop>(m, "ServeLoop",
+                          "Continuous-batching request loop over grids resident on the GPU (one worker and stream per grid).")
+        .def(py::init<size_t, int, int>(), py::arg("hbm_budget") = 0, py::arg("max_instances_per_launch") = 4096,
+             py::arg("precision") = 1)
+        .def("add_grid", [](ServeLoop& sl, py::object Ybus, const carr& Sbus, const carr& V0, const iarr& slack_ids,
+                            const darr& w, const iarr& pv, const iarr& pq, int max_iter, double tol) {
+            CscC Y = csc_from_py(Ybus);
+            CplxVect S = cvec(Sbus), V = cvec(V0);
+            IntVect sl_ = ivec(slack_ids), pv_ = ivec(pv), pq_ = ivec(pq);
+            RealVect w_ = rvec(w);
+            py::gil_scoped_release nogil;
+            return sl.add_grid(Y, S, V, sl_, w_, pv_, pq_, max_iter, tol);
+        }, py::arg("Ybus"), py::arg("Sbus"), py::arg("V0"), py::arg("slack_ids"), py::arg("slack_weights"),
+           py::arg("pv"), py::arg("pq"), py::arg("max_iter") = 10, py::arg("tol") = 1e-8)
+        .def("submit_pf", [](ServeLoop& sl, int grid, py::object Sbus) {
+            CplxVect S;
+            if (!Sbus.is_none()) S = cvec(Sbus.cast<carr>());
+            return sl.submit(grid, 0, 1, S, {}, {}, {});
+        }, py::arg("grid"), py::arg("Sbus") = py::none())
+        .def("submit_contingencies", [](ServeLoop& sl, int grid, const iarr& dptr, const iarr& dpos, const carr& dval) {
+            IntVect p = ivec(dptr);
+            return sl.submit(grid, 1, (int)p.size() - 1, {}, p, ivec(dpos), cvec(dval));
+        })
+        .def("submit_time_series", [](ServeLoop& sl, int grid, const carr& Sbus_rows) {
+            const int T = Sbus_rows.ndim() == 2 ? (int)Sbus_rows.shape(0) : 1;
+            return sl.submit(grid, 2, T, cvec(Sbus_rows), {}, {}, {});
+        })
+        .def("ready", &ServeLoop::ready)
+        .def("result", [](ServeLoop& sl, uint64_t id, double timeout) -> py::object {
+            ServeResult r;
+            bool ok;
+            {
+                py::gil_scoped_release nogil;
+                ok = sl.wait(id, timeout, r);
+            }
+            if (!ok) return py::none();
+            if (!r.ok) throw std::runtime_error("ServeLoop request failed: " + r.error);
+            py::dict d;
+            const int n = r.ninst ? (int)(r.V.size() / r.ninst) : 0;
+            d["V"] = to_np2(r.V, r.ninst, n);
+            d["iters"] = to_np(r.iters);
+            d["status"] = to_np(r.status);
+            d["latency_s"] = r.latency_s;
+            d["queue_s"] = r.queue_s;
+            d["launch_instances"] = r.launch_instances;
+            return d;
+        }, py::arg("id"), py::arg("timeout") = -1.0)
+        .def("launches", &ServeLoop::launches)
+        .def("budget", &ServeLoop::budget)
+        .def("in_use", &ServeLoop::in_use)
+        .def("close", [](ServeLoop& sl) {
+            py::gil_scoped_release nogil;
+            sl.close();
+        });
 
     py::class_<ContingencyAnalysis>(m, "ContingencyAnalysisCPP", "Batched N-1 / N-k security analysis (ContingencyAnalysisCPP).")
         .def(py::init<const GridModel&>())

@@ -1036,6 +1036,16 @@ int pfgpu_memcpy_d2h(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
     });
 }
 
+int pfgpu_memcpy_d2d(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ApiError(PFGPU_E_INVALID, "NULL ctx");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
 int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
                              const int* d_dptr, const int* d_dpos, const double* d_dval, const double* d_Sbus,
                              int sbus_stride, const double* d_V0, int v0_stride, double* d_V, int max_iter, double tol,

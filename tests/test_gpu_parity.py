@@ -215,3 +215,36 @@ def test_autotuner_choice_keeps_results(gpu_ctx):
     assert np.array_equal(dSt.to_host(), ref["status"])
     assert np.array_equal(dIt.to_host(), ref["iters"])
     assert np.max(np.abs(dV.to_host() - ref["V"])) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_large_grid_contingencies_with_snapshots(gpu_ctx, oracle_mod, prec):
+    """10000-bus grid: outages x load snapshots (per-instance Sbus + per-instance Ybus deltas)."""
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make("case10000_synth")
+    d = g.solver_inputs()
+    _say("10000-bus outages x snapshots", "fp32" if prec else "fp64")
+    rng = np.random.default_rng(1)
+    nb = np.nonzero(~gridgen.bridges(g))[0]
+    outs = np.sort(rng.choice(nb, 12, replace=False))
+    scales = np.array([0.85, 1.05])
+    conts = [[int(b)] for b in outs for _ in scales]
+    deltas = g.ybus_deltas(conts)
+    S = np.stack([g.sbus(load_scale=sc, gen_scale=sc) for sc in scales])
+    Sall = np.tile(S, (len(outs), 1))
+    o = oracle_mod.OracleNR(False)
+    assert o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    Vb = o.get_V()
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    r = gr.solve(gr.ybus_values(d["Ybus"]), Sall, Vb, 10, 1e-8, precision=prec, deltas=deltas)
+    Y = d["Ybus"].tocsc()
+    Y.sort_indices()
+    for i, (cont, Si) in enumerate(zip(conts, Sall)):
+        Yc = Y.copy()
+        Yc.data[deltas[1][deltas[0][i]:deltas[0][i + 1]]] -= deltas[2][deltas[0][i]:deltas[0][i + 1]]
+        oc = oracle_mod.OracleNR(False)
+        conv = oc.compute_pf(Yc, Vb, Si, d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+        assert (r["status"][i] == 0) == conv
+        if conv:
+            assert abs(int(r["iters"][i]) - oc.get_nb_iter()) <= (0 if prec == 0 else 1)
+            assert np.max(np.abs(r["V"][i] - oc.get_V())) <= vtol(prec)

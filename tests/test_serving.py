@@ -1,0 +1,95 @@
+"""Serving loop (continuous batching over resident grids): a request's result is
+the same alone and in a mix of concurrent requests (same converged flag and
+iterations, |dV| <= 1e-10 pu in FP64), it matches the oracle, and concurrent
+requests share launches."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _grid(name):
+    from lightsim2grid_amd import gridgen
+    g = gridgen.make(name)
+    return g, g.solver_inputs()
+
+
+def test_serving_alone_vs_mix(oracle_mod):
+    import lightsim2grid_amd as L
+    from lightsim2grid_amd import gridgen
+    _say("serving alone vs mix")
+    sl = L.lightsim2grid_amd_cpp.ServeLoop(0, 4096, 0)  # FP64 form
+    grids = {}
+    for name in ("case14_synth", "case118_synth"):
+        g, d = _grid(name)
+        o = oracle_mod.OracleNR(False)
+        assert o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+        gid = sl.add_grid(d["Ybus"], d["Sbus"], o.get_V(), d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+        grids[name] = (gid, g, d, o.get_V())
+    gid, g, d, Vb = grids["case118_synth"]
+    nb = np.nonzero(~gridgen.bridges(g))[0][:40]
+    dptr, dpos, dval = g.ybus_deltas([[int(k)] for k in nb])
+    Sts = gridgen.time_series_sbus(g, 12, seed=5)
+    # alone
+    r_ca = sl.result(sl.submit_contingencies(gid, dptr, dpos, dval), 60.0)
+    r_ts = sl.result(sl.submit_time_series(gid, Sts), 60.0)
+    launches_alone = sl.launches()
+    # in a mix: many concurrent requests on both grids from several threads
+    ids = {}
+
+    def client(k):
+        out = []
+        for j in range(6):
+            gname = "case14_synth" if (k + j) % 2 else "case118_synth"
+            gi, gg, dd, _ = grids[gname]
+            out.append(sl.submit_pf(gi, None))
+            out.append(sl.submit_time_series(gi, gridgen.time_series_sbus(gg, 3, seed=k * 10 + j)))
+        ids[k] = out
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    id_ca = sl.submit_contingencies(gid, dptr, dpos, dval)
+    id_ts = sl.submit_time_series(gid, Sts)
+    for t in th:
+        t.join()
+    m_ca = sl.result(id_ca, 60.0)
+    m_ts = sl.result(id_ts, 60.0)
+    others = [sl.result(i, 60.0) for k in ids for i in ids[k]]
+    n_req = 2 + sum(len(v) for v in ids.values())
+    assert sl.launches() - launches_alone < n_req  # requests were co-batched
+    assert max(r["launch_instances"] for r in others + [m_ca, m_ts]) > 40
+    for a, b in ((r_ca, m_ca), (r_ts, m_ts)):
+        assert np.array_equal(a["status"], b["status"])
+        assert np.array_equal(a["iters"], b["iters"])
+        assert np.max(np.abs(a["V"] - b["V"])) <= 1e-10
+    assert all(np.all(r["status"] == 0) for r in others)
+    # against the oracle
+    Vr, itr, _, convr, _, _ = oracle_mod.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"],
+                                                      d["pq"], Vb, (dptr, dpos, dval), 10, 1e-8, False, False, 2)
+    assert np.array_equal(m_ca["status"] == 0, convr)
+    assert np.max(np.abs(m_ca["V"][convr] - Vr[convr])) <= 1e-8
+    assert np.array_equal(m_ca["iters"][convr], itr[convr])
+    sl.close()
+
+
+def test_serving_rejects_oversized_request():
+    import lightsim2grid_amd as L
+    from lightsim2grid_amd import gridgen
+    _say("serving budget")
+    g, d = _grid("case118_synth")
+    sl = L.lightsim2grid_amd_cpp.ServeLoop(4 << 20, 4096, 1)  # 4 MiB budget
+    gid = sl.add_grid(d["Ybus"], d["Sbus"], d["V0"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    with pytest.raises(RuntimeError):
+        sl.submit_time_series(gid, gridgen.time_series_sbus(g, 2000, seed=1))
+    r = sl.result(sl.submit_pf(gid, None), 60.0)
+    assert r["status"][0] == 0
+    sl.close()
