@@ -166,6 +166,34 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ double inf_if_nonfinite(double a) { return (a <= 1.7976931348623157e308) ? a : INFINITY; }
 
+// Polar <-> rectangular conversions of one lane's buses (b0, b0 + step, ...).
+// Kept out of line: inlined FP64 atan2 / sincos raise the NR kernel's register
+// count past 128 VGPRs, which at 1024 lanes per workgroup means scratch spills
+// in the factor loop.
+// Va = arg(V), Vm = |V| (BaseNRAlgo.tpp:77-79, BaseNRSingleSlackAlgo.tpp:131-132)
+__device__ __noinline__ void polar_of(const double* __restrict__ V, double* __restrict__ Va,
+                                      double* __restrict__ Vm, int b0, int n, int step) {
+    for (int b = b0; b < n; b += step) {
+        const double vr = V[2 * b], vi = V[2 * b + 1];
+        Va[b] = atan2(vi, vr);
+        Vm[b] = sqrt(vr * vr + vi * vi);
+    }
+}
+// V = Vm * exp(j Va) (BaseNRAlgo.tpp:150-151); returns min Vm over the lane's buses
+__device__ __noinline__ double rect_of(double* __restrict__ V, const double* __restrict__ Va,
+                                       const double* __restrict__ Vm, int b0, int n, int step) {
+    double vmin = INFINITY;
+    for (int b = b0; b < n; b += step) {
+        double sn, cs;
+        sincos(Va[b], &sn, &cs);
+        const double m = Vm[b];
+        V[2 * b] = m * cs;
+        V[2 * b + 1] = m * sn;
+        vmin = fmin(vmin, m);
+    }
+    return vmin;
+}
+
 template <typename FT, int G, int NT>
 __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     constexpr int kWaves = NT / 64;
@@ -204,13 +232,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 
     // ---- init: Va/Vm from V, slack_absorbed = Re(sum Sbus) (BaseNRAlgo.tpp:60,77-79)
     double ssum = 0.0;
-    if (valid)
-        for (int b = s; b < n; b += nslot) {
-            const double vr = V[2 * b], vi = V[2 * b + 1];
-            Va[b] = atan2(vi, vr);
-            Vm[b] = sqrt(vr * vr + vi * vi);
-            ssum += S[2 * b];
-        }
+    if (valid) {
+        polar_of(V, Va, Vm, s, n, nslot);
+        for (int b = s; b < n; b += nslot) ssum += S[2 * b];
+    }
     // sum-reduce (as max of a single value is not a sum, do a dedicated pass)
 #pragma unroll
     for (int o = G; o < 64; o <<= 1) ssum += __shfl_xor(ssum, o);
@@ -270,13 +295,15 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (s == 0 && act) s_it[g] += 1;
 
         // ---- J fill: zero the fill-in slots, write every J entry --------------------
-        // (4 entries per lane per step: their loads are issued together)
+        // (kJU entries per lane per step: their loads are issued together; a deeper
+        // unroll costs more registers than it saves in latency)
+        constexpr int kJU = 2;
         if (act) {
             for (int i = s; i < a.nzero; i += nslot) XS(a.zero_slots[i], (FT)0);
-            for (int q0 = s; q0 < a.nJ; q0 += 4 * nslot) {
-                int qq[4], sel[4], rb[4], cb[4], yp[4], js[4];
+            for (int q0 = s; q0 < a.nJ; q0 += kJU * nslot) {
+                int qq[kJU], sel[kJU], rb[kJU], cb[kJU], yp[kJU], js[kJU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kJU; ++u) {
                     const int q = q0 + u * nslot;
                     qq[u] = q < a.nJ ? q : -1;
                     const int qc = q < a.nJ ? q : 0;
@@ -286,10 +313,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     yp[u] = a.j_ypos[qc];
                     js[u] = a.j_slot[qc];
                 }
-                c64 y[4], vr[4], vc[4], ib[4];
-                double cv[4];
+                c64 y[kJU], vr[kJU], vc[kJU], ib[kJU];
+                double cv[kJU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kJU; ++u) {
                     y[u] = ld_c(Y, yp[u] >= 0 ? yp[u] : 0);
                     vr[u] = ld_c(V, rb[u]);
                     vc[u] = ld_c(V, cb[u]);
@@ -297,7 +324,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     cv[u] = a.j_cval[qq[u] >= 0 ? qq[u] : 0];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kJU; ++u) {
                     if (qq[u] < 0) continue;
                     const double jv = jvalue_of(sel[u], rb[u] == cb[u], y[u], vr[u], vc[u], ib[u], cv[u]);
                     if (a.Jx_last != nullptr && inst == 0) a.Jx_last[qq[u]] = jv;
@@ -439,12 +466,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         }
 
         // ---- voltage update ------------------------------------------------------------
-        if (act && a.single_slack)  // BaseNRSingleSlackAlgo.tpp:131-132
-            for (int b = s; b < n; b += nslot) {
-                const double vr = V[2 * b], vi = V[2 * b + 1];
-                Va[b] = atan2(vi, vr);
-                Vm[b] = sqrt(vr * vr + vi * vi);
-            }
+        if (act && a.single_slack) polar_of(V, Va, Vm, s, n, nslot);  // BaseNRSingleSlackAlgo.tpp:131-132
         __syncthreads();
         if (act)
             for (int j = s; j < a.N; j += nslot) {
@@ -456,22 +478,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         __syncthreads();
         double vmin = INFINITY;
-        if (act)
-            for (int b = s; b < n; b += nslot) {
-                double sn, cs;
-                sincos(Va[b], &sn, &cs);
-                const double m = Vm[b];
-                V[2 * b] = m * cs;
-                V[2 * b + 1] = m * sn;
-                vmin = fmin(vmin, m);
-            }
+        if (act) vmin = rect_of(V, Va, Vm, s, n, nslot);
         vmin = inst_reduce<G, NT, false>(vmin, s_part);
-        if (act && vmin < 0.0)
-            for (int b = s; b < n; b += nslot) {
-                const double vr = V[2 * b], vi = V[2 * b + 1];
-                Va[b] = atan2(vi, vr);
-                Vm[b] = sqrt(vr * vr + vi * vi);
-            }
+        if (act && vmin < 0.0) polar_of(V, Va, Vm, s, n, nslot);
         __syncthreads();
 
         // ---- mismatch + convergence ----------------------------------------------------
