@@ -557,4 +557,181 @@ std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<
     return team;
 }
 
+namespace {
+
+// Placement of one level's tasks on the lanes of one instance.
+struct LevelPlacement {
+    int K = 0;                  // items per lane
+    std::vector<int> lane0;     // per task: first lane of its team
+    std::vector<int> lt;        // per task: log2 of its team size
+    std::vector<int> k0;        // per task: first item index in its team's lanes
+};
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+int log2_ceil(int x) {
+    int l = 0;
+    while ((1 << l) < x) ++l;
+    return l;
+}
+
+// For a target of K items per lane, each task takes the smallest power-of-two
+// team T with ceil(np / T) <= K; the tasks of one team size are packed onto
+// teams of that size (largest first, onto the least loaded team that still has
+// room), and the team-size classes are laid out from lane 0 in decreasing size,
+// so every team is aligned to its size. Returns false if the lanes do not suffice.
+bool place_level_k(const std::vector<int>& np, int K, int tmax_log, int nslot, LevelPlacement& pl,
+                   double& cost) {
+    const int nt = (int)np.size();
+    std::vector<std::vector<int>> cls(tmax_log + 1);
+    for (int i = 0; i < nt; ++i) {
+        const int need = std::max(1, ceil_div(std::max(np[i], 1), K));
+        const int l = log2_ceil(need);
+        if (l > tmax_log) return false;
+        cls[l].push_back(i);
+    }
+    pl.K = K;
+    pl.lane0.assign(nt, 0);
+    pl.lt.assign(nt, 0);
+    pl.k0.assign(nt, 0);
+    int lane = 0;
+    double worst = 0;
+    for (int l = tmax_log; l >= 0; --l) {
+        if (cls[l].empty()) continue;
+        const int T = 1 << l;
+        auto items = [&](int i) { return std::max(1, ceil_div(np[i], T)); };
+        std::vector<int>& v = cls[l];
+        std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return items(x) > items(y); });
+        // teams of this class: (load, tasks), least loaded first among those with room
+        std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<>> heap;
+        std::vector<int> load, ntask;
+        for (int i : v) {
+            const int c = items(i);
+            int t;
+            if (!heap.empty() && heap.top().first + c <= K) {
+                t = heap.top().second;
+                heap.pop();
+            } else {
+                t = (int)load.size();
+                load.push_back(0);
+                ntask.push_back(0);
+            }
+            pl.lane0[i] = lane + t * T;
+            pl.lt[i] = l;
+            pl.k0[i] = load[t];
+            load[t] += c;
+            ntask[t] += 1;
+            heap.push({load[t], t});
+        }
+        for (int t = 0; t < (int)load.size(); ++t) worst = std::max(worst, ntask[t] * (2.0 * l + 4.0));
+        lane += (int)load.size() * T;
+        if (lane > nslot) return false;
+    }
+    // per lane: items (one pipelined load and the LDS reads of one pair each) plus the task
+    // ends of its team (a log2(T)-step shuffle reduction and the leader's read-modify-write)
+    cost = K + worst;
+    return true;
+}
+
+LevelPlacement choose_placement(const std::vector<int>& np, int G, int nslot) {
+    const int tmax = std::max(1, std::min(64 / G, nslot));
+    const int tmax_log = log2_ceil(tmax + 1) - 1;  // floor(log2(tmax))
+    long total = 0;
+    int maxnp = 1;
+    for (int x : np) {
+        total += std::max(x, 1);
+        maxnp = std::max(maxnp, x);
+    }
+    const int k_lo = std::max({1, (int)((total + nslot - 1) / nslot), ceil_div(maxnp, 1 << tmax_log)});
+    LevelPlacement best, pl;
+    double best_cost = 1e300;
+    // candidate K: k_lo, then growing by ~12% up to the all-singleton layout
+    const int k_hi = std::max(k_lo, maxnp) * 2 + (int)np.size();
+    for (int K = k_lo; K <= k_hi; K = std::max(K + 1, (int)(K * 1.125))) {
+        double cost;
+        if (!place_level_k(np, K, tmax_log, nslot, pl, cost)) continue;
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = pl;
+        }
+        if (K > 4 * best_cost) break;
+    }
+    if (best.K == 0) throw std::runtime_error("lane streams: no placement fits the lanes");
+    return best;
+}
+
+// Lays out the levels [lvl_ptr[l], lvl_ptr[l+1]) of a task list. emit(task, pair index within
+// the task or -1 for padding, end (is the task's last item), log2 team size, item) fills one
+// 4-int item (pre-set to -1).
+template <class NPairs, class Emit>
+void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs npairs_of, Emit emit,
+                    std::vector<int>& lv, std::vector<int>& stream) {
+    const int nlev = (int)lvl_ptr.size() - 1;
+    lv.assign(4 * (size_t)std::max(nlev, 0), 0);
+    stream.clear();
+    size_t base = 0;
+    for (int l = 0; l < nlev; ++l) {
+        const int t0 = lvl_ptr[l], nt = lvl_ptr[l + 1] - t0;
+        std::vector<int> np(nt);
+        for (int i = 0; i < nt; ++i) np[i] = npairs_of(t0 + i);
+        const LevelPlacement pl = choose_placement(np, G, nslot);
+        const int K = pl.K;
+        int maxlt = 0;
+        for (int x : pl.lt) maxlt = std::max(maxlt, x);
+        lv[4 * l] = (int)base;
+        lv[4 * l + 1] = K;
+        lv[4 * l + 2] = 1 << maxlt;
+        if (base + (size_t)K * nslot >= (size_t)1 << 31) throw std::runtime_error("lane stream too large");
+        stream.resize(4 * (base + (size_t)K * nslot), -1);
+        for (int i = 0; i < nt; ++i) {
+            const int T = 1 << pl.lt[i];
+            const int c = std::max(1, ceil_div(np[i], T));
+            for (int tp = 0; tp < T; ++tp)
+                for (int j = 0; j < c; ++j) {
+                    const int pi = j * T + tp;
+                    const size_t k = (size_t)pl.k0[i] + j;
+                    int* it = &stream[4 * (base + k * nslot + pl.lane0[i] + tp)];
+                    emit(t0 + i, pi < np[i] ? pi : -1, j == c - 1, pl.lt[i], it);
+                }
+        }
+        base += (size_t)K * nslot;
+    }
+}
+
+}  // namespace
+
+LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot) {
+    if (G < 1 || nslot < 1) throw std::invalid_argument("build_lane_streams: bad launch shape");
+    if (P.nvals >= kStreamSlotMask) throw std::invalid_argument("build_lane_streams: too many value slots");
+    LaneStreams L;
+    L.G = G;
+    L.nslot = nslot;
+    layout_streams(
+        P.f_lvl, G, nslot, [&](int t) { return P.tasks[4 * t + 2] - P.tasks[4 * t + 1]; },
+        [&](int t, int pi, bool last, int lt, int* it) {
+            if (pi >= 0) {
+                const int p = P.tasks[4 * t + 1] + pi;
+                it[0] = P.pairs[4 * p];
+                it[1] = P.pairs[4 * p + 1];
+                it[2] = P.pairs[4 * p + 2];
+            }
+            if (last) it[3] = P.tasks[4 * t] | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0);
+        },
+        L.f_lv, L.fs);
+    layout_streams(
+        P.b_lvl, G, nslot, [&](int t) { return P.btasks[4 * t + 2] - P.btasks[4 * t + 1]; },
+        [&](int t, int pi, bool last, int lt, int* it) {
+            if (pi >= 0) {
+                const int p = P.btasks[4 * t + 1] + pi;
+                it[0] = P.bpairs[2 * p];
+                it[1] = P.bpairs[2 * p + 1];
+            }
+            if (last) {
+                it[2] = P.btasks[4 * t] | (lt << kStreamTeamShift);
+                it[3] = P.btasks[4 * t + 3];
+            }
+        },
+        L.b_lv, L.bs);
+    return L;
+}
+
 }  // namespace pfgpu

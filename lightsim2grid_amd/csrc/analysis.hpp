@@ -140,6 +140,34 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
 // launch with G interleaved instances and `nslot` lanes per instance.
 std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<int>& maxp, int G, int nslot);
 
+// Lane streams: the factor/forward and backward task lists of a plan laid out
+// for one launch shape (G interleaved instances, nslot lanes per instance).
+// Item k of lane s at level l sits at base[l] + k * nslot + s, so the address of
+// every work item a lane reads depends on no earlier load: a level's items can
+// be fetched before the barrier that precedes it.
+// Each task is shared by a team of T lanes (power of two, T * G <= 64, aligned
+// to T); every lane of the team holds ceil(pairs / T) consecutive items for it,
+// the last of which marks the task's end and carries log2(T). Per level, the
+// smallest item count K per lane for which the tasks fit is searched (each task
+// takes the smallest team that holds it in K items). Lanes are padded to K.
+constexpr int kStreamTeamShift = 26;                 // end marker: slot | log2(T) << 26 (| 1 << 30)
+constexpr int kStreamSlotMask = (1 << 26) - 1;
+constexpr int kStreamRecip = 1 << 30;
+
+struct LaneStreams {
+    int G = 0, nslot = 0;
+    // per level: {base item, K items per lane, largest team size, 0}
+    std::vector<int> f_lv, b_lv;
+    // forward items {l, u, d, end}: l < 0 = no pair; end < 0 = not the task's last item,
+    // else output slot | log2(T) << 26 | (1 << 30 for a pivot: store the reciprocal)
+    std::vector<int> fs;
+    // backward items {u, x', end, d}: u < 0 = no pair; end < 0 = not the task's last item,
+    // else x slot | log2(T) << 26, with d the slot of 1/d
+    std::vector<int> bs;
+};
+
+LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot);
+
 std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch);
 
 }  // namespace pfgpu

@@ -86,12 +86,12 @@ struct KArgs {
     const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
     const double* j_cval;
-    const int *f_lvl, *f_team;
-    const int4* tasks;   // {out, pair begin, pair end, kind}
-    const int4* pairs;   // {l, u, 1/d, -}
-    const int *b_lvl, *b_team;
-    const int4* btasks;  // {x, pair begin, pair end, 1/d}
-    const int2* bpairs;  // {u, x'}
+    // lane streams (per launch shape): per level {base item, items per lane, -, -};
+    // item k of lane s at base + k * nslot + s
+    const int4* f_lv;
+    const int4* fs;      // {l, u, 1/d, end}: l < 0 no pair; end >= 0: slot | log2(team) << 26 | recip << 30
+    const int4* b_lv;
+    const int4* bs;      // {u, x', end, 1/d}: u < 0 no pair; end >= 0: slot | log2(team) << 26
     const double* w;
     const double* Yx;
     long long y_stride;  // doubles between instances (0 = shared)
@@ -192,6 +192,16 @@ __device__ __noinline__ double rect_of(double* __restrict__ V, const double* __r
         vmin = fmin(vmin, m);
     }
     return vmin;
+}
+
+// Lane streams: loads items k .. k + kSC - 1 of one lane (s) at a level
+// (lv = {base, items per lane, ...}); items past the level's end read as "nothing".
+constexpr int kSC = 2;
+__device__ __forceinline__ void stream_chunk(const int4* __restrict__ st, int4 lv, int k, int nslot, int s,
+                                             int4 (&q)[kSC]) {
+#pragma unroll
+    for (int u = 0; u < kSC; ++u)
+        q[u] = (k + u < lv.y) ? st[(size_t)lv.x + (size_t)(k + u) * nslot + s] : int4{-1, -1, -1, -1};
 }
 
 template <typename FT, int G, int NT>
@@ -340,52 +350,59 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         __syncthreads();
 
         // ---- factor + forward solve --------------------------------------------
-        for (int l = 0; l < a.nlev; ++l) {
-            const int team = a.f_team[l];
-            const int tpos = s & (team - 1);
-            const int nteams = nslot / team;
-            const int hi = a.f_lvl[l + 1];
-            if (act) {
-                for (int t = a.f_lvl[l] + s / team; t < hi; t += nteams) {
-                    const int4 td = a.tasks[t];
+        // Each lane walks its items of the level; the next chunk (of this level, or the
+        // first of the next level) is loaded before the current one is used, so item
+        // loads overlap the LDS work and the barrier between levels.
+        {
+            int4 cur[kSC];
+            if (act) stream_chunk(a.fs, a.nlev > 0 ? a.f_lv[0] : int4{0, 0, 0, 0}, 0, nslot, s, cur);
+            for (int l = 0; l < a.nlev; ++l) {
+                const int4 lv = a.f_lv[l];
+                if (act && lv.y == 0 && l + 1 < a.nlev) stream_chunk(a.fs, a.f_lv[l + 1], 0, nslot, s, cur);
+                if (act) {
                     FT acc = (FT)0;
-                    int p = td.y + tpos;
-                    for (; p + 3 * team < td.z; p += 4 * team) {
-                        const int4 q0 = a.pairs[p], q1 = a.pairs[p + team];
-                        const int4 q2 = a.pairs[p + 2 * team], q3 = a.pairs[p + 3 * team];
-                        const FT x0 = XL(q0.x) * XL(q0.y) * XL(q0.z);
-                        const FT x1 = XL(q1.x) * XL(q1.y) * XL(q1.z);
-                        const FT x2 = XL(q2.x) * XL(q2.y) * XL(q2.z);
-                        const FT x3 = XL(q3.x) * XL(q3.y) * XL(q3.z);
-                        acc += (x0 + x1) + (x2 + x3);
-                    }
-                    for (; p < td.z; p += team) {
-                        const int4 q = a.pairs[p];
-                        acc += XL(q.x) * XL(q.y) * XL(q.z);
-                    }
-                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
-                    if (tpos == 0) {
-                        // one explicit read-modify-write per address space (a merged generic
-                        // pointer for this slot trips the gfx950 backend)
-                        if (td.x < nlds) {
-                            FT v = Xl[td.x * G + g] - acc;
-                            if (td.w == 1) {
-                                if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                v = (FT)1 / v;
+                    for (int k = 0; k < lv.y; k += kSC) {
+                        int4 nxt[kSC];
+                        if (k + kSC < lv.y) stream_chunk(a.fs, lv, k + kSC, nslot, s, nxt);
+                        else if (l + 1 < a.nlev) stream_chunk(a.fs, a.f_lv[l + 1], 0, nslot, s, nxt);
+                        else stream_chunk(a.fs, int4{0, 0, 0, 0}, 0, nslot, s, nxt);
+#pragma unroll
+                        for (int u = 0; u < kSC; ++u) {
+                            const int4 q = cur[u];
+                            if (q.x >= 0) acc += XL(q.x) * XL(q.y) * XL(q.z);
+                            if (q.w >= 0) {  // end of a task: the team's lanes reduce, the leader writes
+                                const int lt = (q.w >> 26) & 15;
+                                for (int o = 1; o < (1 << lt); o <<= 1) acc += __shfl_xor(acc, o * G);
+                                if ((s & ((1 << lt) - 1)) == 0) {
+                                    const int slot = q.w & ((1 << 26) - 1);
+                                    const bool recip = (q.w >> 30) & 1;
+                                    // one explicit read-modify-write per address space (a merged generic
+                                    // pointer for this slot trips the gfx950 backend)
+                                    if (slot < nlds) {
+                                        FT v = Xl[slot * G + g] - acc;
+                                        if (recip) {
+                                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                            v = (FT)1 / v;
+                                        }
+                                        Xl[slot * G + g] = v;
+                                    } else {
+                                        FT v = Xgb[(size_t)(slot - nlds) * G + g] - acc;
+                                        if (recip) {
+                                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                            v = (FT)1 / v;
+                                        }
+                                        Xgb[(size_t)(slot - nlds) * G + g] = v;
+                                    }
+                                }
+                                acc = (FT)0;
                             }
-                            Xl[td.x * G + g] = v;
-                        } else {
-                            FT v = Xgb[(size_t)(td.x - nlds) * G + g] - acc;
-                            if (td.w == 1) {
-                                if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                v = (FT)1 / v;
-                            }
-                            Xgb[(size_t)(td.x - nlds) * G + g] = v;
                         }
+#pragma unroll
+                        for (int u = 0; u < kSC; ++u) cur[u] = nxt[u];
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
         // ---- dense tail: wave 0 factors the m x m tile of every instance --------------
         // (tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m); both in LDS)
@@ -431,38 +448,45 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         act = valid && !s_done[g];
 
         // ---- backward solve ---------------------------------------------------------
-        for (int l = 0; l < a.nblev; ++l) {
-            const int team = a.b_team[l];
-            const int tpos = s & (team - 1);
-            const int nteams = nslot / team;
-            const int hi = a.b_lvl[l + 1];
-            if (act) {
-                for (int t = a.b_lvl[l] + s / team; t < hi; t += nteams) {
-                    const int4 bt = a.btasks[t];
+        {
+            int4 cur[kSC];
+            if (act) stream_chunk(a.bs, a.nblev > 0 ? a.b_lv[0] : int4{0, 0, 0, 0}, 0, nslot, s, cur);
+            for (int l = 0; l < a.nblev; ++l) {
+                const int4 lv = a.b_lv[l];
+                if (act && lv.y == 0 && l + 1 < a.nblev) stream_chunk(a.bs, a.b_lv[l + 1], 0, nslot, s, cur);
+                if (act) {
                     FT acc = (FT)0;
-                    int p = bt.y + tpos;
-                    for (; p + 3 * team < bt.z; p += 4 * team) {
-                        const int2 q0 = a.bpairs[p], q1 = a.bpairs[p + team];
-                        const int2 q2 = a.bpairs[p + 2 * team], q3 = a.bpairs[p + 3 * team];
-                        acc += (XL(q0.x) * XL(q0.y) + XL(q1.x) * XL(q1.y)) + (XL(q2.x) * XL(q2.y) + XL(q3.x) * XL(q3.y));
-                    }
-                    for (; p < bt.z; p += team) {
-                        const int2 q = a.bpairs[p];
-                        acc += XL(q.x) * XL(q.y);
-                    }
-                    for (int o = 1; o < team; o <<= 1) acc += __shfl_xor(acc, o * G);
-                    if (tpos == 0) {
-                        const FT rd = XL(bt.w);
-                        if (bt.x < nlds) {
-                            Xl[bt.x * G + g] = (Xl[bt.x * G + g] - acc) * rd;
-                        } else {
-                            FT* xp = &Xgb[(size_t)(bt.x - nlds) * G + g];
-                            *xp = (*xp - acc) * rd;
+                    for (int k = 0; k < lv.y; k += kSC) {
+                        int4 nxt[kSC];
+                        if (k + kSC < lv.y) stream_chunk(a.bs, lv, k + kSC, nslot, s, nxt);
+                        else if (l + 1 < a.nblev) stream_chunk(a.bs, a.b_lv[l + 1], 0, nslot, s, nxt);
+                        else stream_chunk(a.bs, int4{0, 0, 0, 0}, 0, nslot, s, nxt);
+#pragma unroll
+                        for (int u = 0; u < kSC; ++u) {
+                            const int4 q = cur[u];
+                            if (q.x >= 0) acc += XL(q.x) * XL(q.y);
+                            if (q.z >= 0) {
+                                const int lt = (q.z >> 26) & 15;
+                                for (int o = 1; o < (1 << lt); o <<= 1) acc += __shfl_xor(acc, o * G);
+                                if ((s & ((1 << lt) - 1)) == 0) {
+                                    const int slot = q.z & ((1 << 26) - 1);
+                                    const FT rd = XL(q.w);
+                                    if (slot < nlds) {
+                                        Xl[slot * G + g] = (Xl[slot * G + g] - acc) * rd;
+                                    } else {
+                                        FT* xp = &Xgb[(size_t)(slot - nlds) * G + g];
+                                        *xp = (*xp - acc) * rd;
+                                    }
+                                }
+                                acc = (FT)0;
+                            }
                         }
+#pragma unroll
+                        for (int u = 0; u < kSC; ++u) cur[u] = nxt[u];
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
 
         // ---- voltage update ------------------------------------------------------------
@@ -598,16 +622,15 @@ struct pfgpu_grid {
     pfgpu::JPattern J;
     pfgpu::DevicePlan P;
     pfgpu_grid_info info{};
-    DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots,
-        f_lvl, tasks, pairs, b_lvl, btasks, bpairs;
+    DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots;
     DevBuf<int8_t> ux_kind;
     DevBuf<uint8_t> j_sel;
     DevBuf<double> j_cval, w;
-    // per-(G, block size) team tables
-    struct Teams {
-        DevBuf<int> f, b;
+    // per-(G, block size) lane streams of the factor/forward and backward schedules
+    struct Streams {
+        DevBuf<int> f_lv, fs, b_lv, bs;
     };
-    std::vector<std::unique_ptr<Teams>> teams;  // index 2*log2(G) + (block == 1024)
+    std::vector<std::unique_ptr<Streams>> streams;  // index 2*log2(G) + (block == 1024)
     // autotuned launch shape per precision (G, block), 0 = not tuned
     int tuned_G[2] = {0, 0}, tuned_NT[2] = {0, 0};
 };
@@ -690,17 +713,20 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     return c;
 }
 
-int team_index(int G, int NT) { return 2 * log2i(G) + (NT == 1024 ? 1 : 0); }
+int shape_index(int G, int NT) { return 2 * log2i(G) + (NT == 1024 ? 1 : 0); }
 
-void ensure_teams(pfgpu_grid* gr, int G, int NT) {
-    const int ix = team_index(G, NT);
-    if ((int)gr->teams.size() <= ix) gr->teams.resize(ix + 1);
-    if (gr->teams[ix]) return;
-    auto t = std::make_unique<pfgpu_grid::Teams>();
-    t->f.upload(pfgpu::level_teams(gr->P.f_lvl, gr->P.f_maxp, G, NT / G));
-    t->b.upload(pfgpu::level_teams(gr->P.b_lvl, gr->P.b_maxp, G, NT / G));
-    if (!t->f.p) t->f.upload(std::vector<int>(1, 1));
-    gr->teams[ix] = std::move(t);
+void ensure_streams(pfgpu_grid* gr, int G, int NT) {
+    const int ix = shape_index(G, NT);
+    if ((int)gr->streams.size() <= ix) gr->streams.resize(ix + 1);
+    if (gr->streams[ix]) return;
+    const pfgpu::LaneStreams L = pfgpu::build_lane_streams(gr->P, G, NT / G);
+    auto t = std::make_unique<pfgpu_grid::Streams>();
+    auto up = [](DevBuf<int>& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>(4, -1) : v); };
+    up(t->f_lv, L.f_lv);
+    up(t->fs, L.fs);
+    up(t->b_lv, L.b_lv);
+    up(t->bs, L.bs);
+    gr->streams[ix] = std::move(t);
 }
 
 void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
@@ -736,7 +762,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
         NT_req = gr->tuned_NT[precision];
     }
     LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req, NT_req);
-    ensure_teams(gr, lc.G, lc.NT);
+    ensure_streams(gr, lc.G, lc.NT);
     const bool deltas = d_dptr != nullptr;
     size_t off = 0;
     const size_t o_y = off;
@@ -776,11 +802,9 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
     a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
-    auto& tm = *gr->teams[team_index(lc.G, lc.NT)];
-    a.f_lvl = gr->f_lvl.p; a.f_team = tm.f.p;
-    a.tasks = reinterpret_cast<const int4*>(gr->tasks.p); a.pairs = reinterpret_cast<const int4*>(gr->pairs.p);
-    a.b_lvl = gr->b_lvl.p; a.b_team = tm.b.p;
-    a.btasks = reinterpret_cast<const int4*>(gr->btasks.p); a.bpairs = reinterpret_cast<const int2*>(gr->bpairs.p);
+    const auto& ls = *gr->streams[shape_index(lc.G, lc.NT)];
+    a.f_lv = reinterpret_cast<const int4*>(ls.f_lv.p); a.fs = reinterpret_cast<const int4*>(ls.fs.p);
+    a.b_lv = reinterpret_cast<const int4*>(ls.b_lv.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
     a.w = gr->w.p;
     a.Yx = Ypt; a.y_stride = ystride;
     a.S = d_S; a.s_stride = (long long)sbus_stride * 2;
@@ -916,11 +940,6 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
         g->j_cval.upload(P.j_cval); g->j_slot.upload(P.j_slot);
         g->zero_slots.upload(P.zero_slots.empty() ? std::vector<int>(1, 0) : P.zero_slots);
-        g->f_lvl.upload(P.f_lvl);
-        g->tasks.upload(P.tasks.empty() ? std::vector<int>(4, 0) : P.tasks);
-        g->pairs.upload(P.pairs.empty() ? std::vector<int>(4, 0) : P.pairs);
-        g->b_lvl.upload(P.b_lvl); g->btasks.upload(P.btasks);
-        g->bpairs.upload(P.bpairs.empty() ? std::vector<int>(2, 0) : P.bpairs);
         g->w.upload(std::vector<double>(slack_weights, slack_weights + n));
         *out = g.release();
         return 0;

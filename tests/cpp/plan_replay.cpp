@@ -1,7 +1,8 @@
 // Test-only: serial CPU replay of the device plan built by the product's host
 // analysis (lightsim2grid_amd/csrc/analysis.cpp). It executes the same phases,
-// slots, tasks and pairs as the HIP kernel, one instance at a time, so that
-// the schedule can be checked against the oracle on machines without a GPU.
+// slots and lane streams (items per level, lane and team) as the HIP kernel,
+// one instance at a time, so that the schedule can be checked against the
+// oracle on machines without a GPU.
 // It is compiled by tests/conftest.py into tests/cpp/_build/ and is never
 // part of the product library.
 #include "../../lightsim2grid_amd/csrc/analysis.hpp"
@@ -9,6 +10,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <algorithm>
 #include <set>
 #include <vector>
 
@@ -20,7 +22,7 @@ extern "C" {
 // Also reports write-write / read-before-write conflicts found in the level schedule (conflicts_out).
 int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const double* Sbus,
                  const int* slack_ids, int nslack, const double* w, const int* pv, int npv, const int* pq, int npq,
-                 int single_slack, double* V, int max_iter, double tol, int use_fp32,
+                 int single_slack, double* V, int max_iter, double tol, int use_fp32, int G, int nslot,
                  int* status_out, int* conflicts_out, double* Jx_out) {
     YbusPattern Y;
     Y.n = n;
@@ -30,6 +32,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     SymbolicLU S = symbolic_lu(J);
     Schedule sch = build_schedule(S);
     DevicePlan P = build_device_plan(Y, J, S, sch);
+    const LaneStreams LS = build_lane_streams(P, G, nslot);
     const cplx* Yc = reinterpret_cast<const cplx*>(Yx);
     const cplx* Sc = reinterpret_cast<const cplx*>(Sbus);
     std::vector<cplx> Vc(reinterpret_cast<cplx*>(V), reinterpret_cast<cplx*>(V) + n);
@@ -56,6 +59,55 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                 for (int p = P.tasks[4 * t + 1]; p < P.tasks[4 * t + 2]; ++p)
                     for (int k = 0; k < 3; ++k)
                         if (wlev[P.pairs[4 * p + k]] >= l) ++conflicts;
+        // lane streams: all lanes of a team (aligned group of T lanes, T from the end
+        // marker) end each task at the same item with the same marker, every task is
+        // ended by T lanes, and the items cover each task's pairs exactly once
+        auto check_streams = [&](const std::vector<int>& lv, const std::vector<int>& st, int nl, bool fwd,
+                                 const std::vector<int>& tasks, const std::vector<int>& lvl) {
+            std::vector<long> npair_seen(P.nvals, 0), ends(P.nvals, 0), pair_sum(P.nvals, 0), team_of(P.nvals, 0),
+                recip_of(P.nvals, 0);
+            const int ecol = fwd ? 3 : 2;
+            for (int l = 0; l < nl; ++l) {
+                const int base = lv[4 * l], K = lv[4 * l + 1];
+                auto item = [&](int k, int s) { return &st[4 * ((size_t)base + (size_t)k * nslot + s)]; };
+                for (int s = 0; s < nslot; ++s) {
+                    long c = 0, key = 0;
+                    for (int k = 0; k < K; ++k) {
+                        const int* it = item(k, s);
+                        if (it[0] >= 0) { ++c; key += (long)it[0] * 7 + (long)it[1] * 13 + (fwd ? (long)it[2] * 17 : 0); }
+                        const int e = it[ecol];
+                        if (e >= 0) {
+                            const int T = 1 << ((e >> kStreamTeamShift) & 15);
+                            if (T * G > 64 || T > nslot) ++conflicts;
+                            const int s0 = s & ~(T - 1);
+                            for (int tp = 0; tp < T; ++tp)
+                                if (item(k, s0 + tp)[ecol] != e) ++conflicts;
+                            const int o = e & kStreamSlotMask;
+                            npair_seen[o] += c;
+                            pair_sum[o] += key;
+                            ends[o] += 1;
+                            team_of[o] = T;
+                            recip_of[o] = fwd && (e & kStreamRecip) ? 1 : 0;
+                            c = 0;
+                            key = 0;
+                        }
+                    }
+                    if (c != 0) ++conflicts;  // pairs after the last task end
+                }
+                for (int t = lvl[l]; t < lvl[l + 1]; ++t) {
+                    const int o = tasks[4 * t];
+                    long key = 0;
+                    for (int p = tasks[4 * t + 1]; p < tasks[4 * t + 2]; ++p)
+                        key += fwd ? (long)P.pairs[4 * p] * 7 + (long)P.pairs[4 * p + 1] * 13 + (long)P.pairs[4 * p + 2] * 17
+                                   : (long)P.bpairs[2 * p] * 7 + (long)P.bpairs[2 * p + 1] * 13;
+                    if (npair_seen[o] != tasks[4 * t + 2] - tasks[4 * t + 1] || ends[o] != team_of[o] || pair_sum[o] != key ||
+                        recip_of[o] != (fwd && tasks[4 * t + 3] == 1 ? 1 : 0))
+                        ++conflicts;
+                }
+            }
+        };
+        check_streams(LS.f_lv, LS.fs, P.nlev, true, P.tasks, P.f_lvl);
+        check_streams(LS.b_lv, LS.bs, P.nblev, false, P.btasks, P.b_lvl);
         *conflicts_out = conflicts;
     }
 
@@ -101,15 +153,38 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
             const int sl = P.j_slot[q] & ((1 << 30) - 1);
             X[sl] = (P.j_slot[q] >> 30) & 1 ? rnd(1.0 / rnd(v)) : rnd(v);
         }
-        for (int l = 0; l < P.nlev; ++l)
-            for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
-                const int o = P.tasks[4 * t], kind = P.tasks[4 * t + 3];
-                double acc = 0;
-                for (int p = P.tasks[4 * t + 1]; p < P.tasks[4 * t + 2]; ++p)
-                    acc += X[P.pairs[4 * p]] * X[P.pairs[4 * p + 1]] * X[P.pairs[4 * p + 2]];
-                const double v = rnd(X[o] - acc);
-                X[o] = kind == 1 ? rnd(1.0 / v) : v;
+        std::vector<double> acc(nslot);
+        // one level of a lane stream: every lane accumulates its items; at a task's end
+        // item the team's partial sums are added and the leader (first lane) writes
+        auto run_level = [&](const std::vector<int>& st, int base, int K, bool fwd) {
+            std::fill(acc.begin(), acc.end(), 0.0);
+            const int ecol = fwd ? 3 : 2;
+            for (int k = 0; k < K; ++k) {
+                for (int s = 0; s < nslot; ++s) {
+                    const int* it = &st[4 * ((size_t)base + (size_t)k * nslot + s)];
+                    if (it[0] < 0) continue;
+                    acc[s] = fwd ? rnd(acc[s] + rnd(rnd(X[it[0]] * X[it[1]]) * X[it[2]]))
+                                 : rnd(acc[s] + rnd(X[it[0]] * X[it[1]]));
+                }
+                for (int s = 0; s < nslot; ++s) {
+                    const int* it = &st[4 * ((size_t)base + (size_t)k * nslot + s)];
+                    const int e = it[ecol];
+                    if (e < 0) continue;
+                    const int T = 1 << ((e >> kStreamTeamShift) & 15);
+                    if ((s & (T - 1)) != 0) continue;
+                    double sum = 0;
+                    for (int tp = 0; tp < T; ++tp) { sum = rnd(sum + acc[s + tp]); acc[s + tp] = 0; }
+                    const int o = e & kStreamSlotMask;
+                    if (fwd) {
+                        const double v = rnd(X[o] - sum);
+                        X[o] = (e & kStreamRecip) ? rnd(1.0 / v) : v;
+                    } else {
+                        X[o] = rnd(rnd(X[o] - sum) * X[it[3]]);
+                    }
+                }
             }
+        };
+        for (int l = 0; l < P.nlev; ++l) run_level(LS.fs, LS.f_lv[4 * l], LS.f_lv[4 * l + 1], true);
         // dense tail: right-looking LU of the m x m tile (slots [0, m*m)), forward on its vector
         // (slots [m*m, m*m+m)), then backward; the tile diagonal ends up holding 1/d
         if (P.m > 0) {
@@ -130,14 +205,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                 for (int i = 0; i < k; ++i) yt(i) = rnd(yt(i) - T(i, k) * x);
             }
         }
-        for (int l = 0; l < P.nblev; ++l)
-            for (int t = P.b_lvl[l]; t < P.b_lvl[l + 1]; ++t) {
-                double acc = 0;
-                for (int p = P.btasks[4 * t + 1]; p < P.btasks[4 * t + 2]; ++p)
-                    acc += X[P.bpairs[2 * p]] * X[P.bpairs[2 * p + 1]];
-                const int o = P.btasks[4 * t];
-                X[o] = rnd((X[o] - acc) * X[P.btasks[4 * t + 3]]);
-            }
+        for (int l = 0; l < P.nblev; ++l) run_level(LS.bs, LS.b_lv[4 * l], LS.b_lv[4 * l + 1], false);
         if (single_slack)
             for (int b = 0; b < n; ++b) { Va[b] = std::arg(Vc[b]); Vm[b] = std::abs(Vc[b]); }
         for (int j = 0; j < P.N; ++j) {
