@@ -659,11 +659,13 @@ LevelPlacement choose_placement(const std::vector<int>& np, int G, int nslot) {
     return best;
 }
 
-// Lays out the levels [lvl_ptr[l], lvl_ptr[l+1]) of a task list. emit(task, pair index within
-// the task or -1 for padding, end (is the task's last item), log2 team size, item) fills one
-// 4-int item (pre-set to -1).
+// Lays out the levels [lvl_ptr[l], lvl_ptr[l+1]) of a task list. Level l uses lanes
+// [0, U) and K items per lane, stored lane-minor at base + k * U + s. emit(task, pair
+// index within the task or -1 for none, end (is the task's last item), log2 team size,
+// item) fills one 4-int item (pre-set to {-1, -1, -1, 0}); the lane's last item of the
+// level then gets kStreamLast in its end word.
 template <class NPairs, class Emit>
-void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs npairs_of, Emit emit,
+void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs npairs_of, Emit emit, int end_word,
                     std::vector<int>& lv, std::vector<int>& stream) {
     const int nlev = (int)lvl_ptr.size() - 1;
     lv.assign(4 * (size_t)std::max(nlev, 0), 0);
@@ -675,25 +677,44 @@ void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs np
         for (int i = 0; i < nt; ++i) np[i] = npairs_of(t0 + i);
         const LevelPlacement pl = choose_placement(np, G, nslot);
         const int K = pl.K;
-        int maxlt = 0;
-        for (int x : pl.lt) maxlt = std::max(maxlt, x);
+        int maxlt = 0, U = 0;
+        for (int i = 0; i < nt; ++i) {
+            maxlt = std::max(maxlt, pl.lt[i]);
+            U = std::max(U, pl.lane0[i] + (1 << pl.lt[i]));
+        }
         lv[4 * l] = (int)base;
         lv[4 * l + 1] = K;
         lv[4 * l + 2] = 1 << maxlt;
-        if (base + (size_t)K * nslot >= (size_t)1 << 31) throw std::runtime_error("lane stream too large");
-        stream.resize(4 * (base + (size_t)K * nslot), -1);
+        lv[4 * l + 3] = U;
+        if (base + (size_t)K * U >= (size_t)1 << 31) throw std::runtime_error("lane stream too large");
+        const size_t at = stream.size();
+        stream.resize(4 * (base + (size_t)K * U));
+        for (size_t q = at; q < stream.size(); q += 4) {
+            stream[q] = stream[q + 1] = stream[q + 2] = -1;
+            stream[q + 3] = 0;
+        }
+        if (end_word == 2)  // backward items keep the 1/d slot in the last word
+            for (size_t q = at; q < stream.size(); q += 4) { stream[q + 2] = 0; stream[q + 3] = -1; }
+        std::vector<int> count(U, 0);
         for (int i = 0; i < nt; ++i) {
             const int T = 1 << pl.lt[i];
             const int c = std::max(1, ceil_div(np[i], T));
-            for (int tp = 0; tp < T; ++tp)
+            for (int tp = 0; tp < T; ++tp) {
+                const int lane = pl.lane0[i] + tp;
                 for (int j = 0; j < c; ++j) {
                     const int pi = j * T + tp;
                     const size_t k = (size_t)pl.k0[i] + j;
-                    int* it = &stream[4 * (base + k * nslot + pl.lane0[i] + tp)];
+                    int* it = &stream[4 * (base + k * U + lane)];
                     emit(t0 + i, pi < np[i] ? pi : -1, j == c - 1, pl.lt[i], it);
                 }
+                count[lane] = std::max(count[lane], pl.k0[i] + c);
+            }
         }
-        base += (size_t)K * nslot;
+        for (int lane = 0; lane < U; ++lane) {
+            if (count[lane] == 0) throw std::runtime_error("lane streams: idle lane inside the used range");
+            stream[4 * (base + (size_t)(count[lane] - 1) * U + lane) + end_word] |= kStreamLast;
+        }
+        base += (size_t)K * U;
     }
 }
 
@@ -714,9 +735,11 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot) {
                 it[1] = P.pairs[4 * p + 1];
                 it[2] = P.pairs[4 * p + 2];
             }
-            if (last) it[3] = P.tasks[4 * t] | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0);
+            if (last)
+                it[3] = P.tasks[4 * t] | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0) |
+                        kStreamEnd;
         },
-        L.f_lv, L.fs);
+        3, L.f_lv, L.fs);
     layout_streams(
         P.b_lvl, G, nslot, [&](int t) { return P.btasks[4 * t + 2] - P.btasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {
@@ -726,11 +749,11 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot) {
                 it[1] = P.bpairs[2 * p + 1];
             }
             if (last) {
-                it[2] = P.btasks[4 * t] | (lt << kStreamTeamShift);
+                it[2] = P.btasks[4 * t] | (lt << kStreamTeamShift) | kStreamEnd;
                 it[3] = P.btasks[4 * t + 3];
             }
         },
-        L.b_lv, L.bs);
+        2, L.b_lv, L.bs);
     return L;
 }
 

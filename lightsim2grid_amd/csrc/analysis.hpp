@@ -142,27 +142,31 @@ std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<
 
 // Lane streams: the factor/forward and backward task lists of a plan laid out
 // for one launch shape (G interleaved instances, nslot lanes per instance).
-// Item k of lane s at level l sits at base[l] + k * nslot + s, so the address of
-// every work item a lane reads depends on no earlier load: a level's items can
-// be fetched before the barrier that precedes it.
+// Level l uses lanes [0, U_l); item k of lane s sits at base_l + k * U_l + s, so
+// the address of every work item a lane reads depends on no earlier load: a
+// level's items can be fetched before the barrier that precedes it.
 // Each task is shared by a team of T lanes (power of two, T * G <= 64, aligned
 // to T); every lane of the team holds ceil(pairs / T) consecutive items for it,
 // the last of which marks the task's end and carries log2(T). Per level, the
 // smallest item count K per lane for which the tasks fit is searched (each task
-// takes the smallest team that holds it in K items). Lanes are padded to K.
-constexpr int kStreamTeamShift = 26;                 // end marker: slot | log2(T) << 26 (| 1 << 30)
+// takes the smallest team that holds it in K items). A lane's last item of a
+// level is flagged, so that it loads nothing after it.
+// End word (forward: item[3], backward: item[2]):
+//   bits 0-25 slot, 26-28 log2(T), 29 reciprocal (forward pivots), 30 end of
+//   task, 31 the lane's last item of the level.
+constexpr int kStreamTeamShift = 26;
 constexpr int kStreamSlotMask = (1 << 26) - 1;
-constexpr int kStreamRecip = 1 << 30;
+constexpr int kStreamRecip = 1 << 29;
+constexpr int kStreamEnd = 1 << 30;
+constexpr int kStreamLast = (int)(1u << 31);
 
 struct LaneStreams {
     int G = 0, nslot = 0;
-    // per level: {base item, K items per lane, largest team size, 0}
+    // per level: {base item, K items per lane, largest team size, U used lanes}
     std::vector<int> f_lv, b_lv;
-    // forward items {l, u, d, end}: l < 0 = no pair; end < 0 = not the task's last item,
-    // else output slot | log2(T) << 26 | (1 << 30 for a pivot: store the reciprocal)
+    // forward items {l, u, d, end word}: l < 0 = no pair
     std::vector<int> fs;
-    // backward items {u, x', end, d}: u < 0 = no pair; end < 0 = not the task's last item,
-    // else x slot | log2(T) << 26, with d the slot of 1/d
+    // backward items {u, x', end word, 1/d slot}: u < 0 = no pair
     std::vector<int> bs;
 };
 

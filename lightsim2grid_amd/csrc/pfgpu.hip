@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -86,12 +87,12 @@ struct KArgs {
     const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
     const double* j_cval;
-    // lane streams (per launch shape): per level {base item, items per lane, -, -};
-    // item k of lane s at base + k * nslot + s
-    const int4* f_lv;
+    // lane streams (per launch shape): item k of lane s < U_l at level l sits at
+    // base_l + k * U_l + s, base_l = sum over earlier levels of K * U
+    const int* lv_K;     // {items per lane, used lanes}: nlev factor levels, then nblev backward levels
     const int4* fs;      // {l, u, 1/d, end}: l < 0 no pair; end >= 0: slot | log2(team) << 26 | recip << 30
-    const int4* b_lv;
     const int4* bs;      // {u, x', end, 1/d}: u < 0 no pair; end >= 0: slot | log2(team) << 26
+    int lvk_off;         // byte offset of the level table in dynamic LDS
     const double* w;
     const double* Yx;
     long long y_stride;  // doubles between instances (0 = shared)
@@ -119,7 +120,7 @@ __device__ __forceinline__ c64 ld_c(const double* p, int i) { return {p[2 * i], 
 // Reduce over the nslot lanes of one instance: shuffles inside the wave
 // (lanes g, g+G, ... of a wave), then the kWaves partials through LDS.
 template <int G, int NT, bool IS_MAX>
-__device__ __forceinline__ double inst_reduce(double v, double (*part)[kMaxG]) {
+__device__ __forceinline__ double inst_reduce(double v, double (*part)[G]) {
     constexpr int kWaves = NT / 64;
 #pragma unroll
     for (int o = G; o < 64; o <<= 1) {
@@ -194,15 +195,53 @@ __device__ __noinline__ double rect_of(double* __restrict__ V, const double* __r
     return vmin;
 }
 
-// Lane streams: loads items k .. k + kSC - 1 of one lane (s) at a level
-// (lv = {base, items per lane, ...}); items past the level's end read as "nothing".
-constexpr int kSC = 2;
-__device__ __forceinline__ void stream_chunk(const int4* __restrict__ st, int4 lv, int k, int nslot, int s,
-                                             int4 (&q)[kSC]) {
-#pragma unroll
-    for (int u = 0; u < kSC; ++u)
-        q[u] = (k + u < lv.y) ? st[(size_t)lv.x + (size_t)(k + u) * nslot + s] : int4{-1, -1, -1, -1};
+// Lane streams. Level l uses lanes [0, U) with K items per lane, item k of lane s
+// at base + k * U + s. End word (forward: item.w, backward: item.z): bits 0-25
+// slot, 26-28 log2(team), 29 reciprocal, 30 end of task, 31 the lane's last item
+// of the level (the lane loads nothing after it).
+constexpr int kSC = 2;  // items per chunk
+constexpr int kEnd = 1 << 30;
+__device__ __forceinline__ int4 no_item() { return int4{-1, -1, 0, 0}; }
+
+// Moves v from lane + D to each lane (D < 16: within a row of 16 lanes by DPP row_shl,
+// no LDS traffic; lanes whose source is outside the row get 0).
+template <int D>
+__device__ __forceinline__ float lane_down(float v) {
+    if constexpr (D < 16) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x100 + D, 0xF, 0xF, true));
+    } else {
+        return __shfl_down(v, D);
+    }
 }
+template <int D>
+__device__ __forceinline__ double lane_down(double v) {
+    if constexpr (D < 16) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x100 + D, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + D, 0xF, 0xF, true);
+        return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+    } else {
+        return __shfl_down(v, D);
+    }
+}
+
+// Sum over a team of 2^lt lanes (stride G, aligned to 2^lt * G) into its lowest lane.
+// All lanes of a team share lt, so each step's source lanes are active.
+template <int G, typename FT>
+__device__ __forceinline__ FT team_sum(FT acc, int lt) {
+    if (lt > 0) acc += lane_down<1 * G>(acc);
+    if constexpr (2 * G <= 32) if (lt > 1) acc += lane_down<2 * G>(acc);
+    if constexpr (4 * G <= 32) if (lt > 2) acc += lane_down<4 * G>(acc);
+    if constexpr (8 * G <= 32) if (lt > 3) acc += lane_down<8 * G>(acc);
+    if constexpr (16 * G <= 32) if (lt > 4) acc += lane_down<16 * G>(acc);
+    if constexpr (32 * G <= 32) if (lt > 5) acc += lane_down<32 * G>(acc);
+    return acc;
+}
+
+// Reciprocal of a pivot: the FP32 factor form uses the hardware reciprocal (1 ulp),
+// the FP64 form an IEEE division.
+__device__ __forceinline__ float pivot_recip(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double pivot_recip(double v) { return 1.0 / v; }
 
 template <typename FT, int G, int NT>
 __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
@@ -212,10 +251,15 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // pointer into LDS (flat pointers into LDS mixed with global ones trip the gfx950 backend)
     typedef __attribute__((address_space(3))) FT lds_ft;
     lds_ft* Xl = (lds_ft*)(lds_raw);
-    __shared__ double s_part[kWaves][kMaxG];
-    __shared__ double s_slack[kMaxG];
-    __shared__ int s_done[kMaxG], s_it[kMaxG], s_status[kMaxG], s_piv[kMaxG];
+    __shared__ double s_part[kWaves][G];
+    __shared__ double s_slack[G];
+    __shared__ int s_done[G], s_it[G], s_status[G], s_piv[G];
     __shared__ int s_ndone;
+    // {items per lane, used lanes} of every factor level, then of every backward level
+    // (in LDS after the value slots)
+    typedef __attribute__((address_space(3))) int lds_int;
+    lds_int* lvK = (lds_int*)(lds_raw + a.lvk_off);
+    for (int i = threadIdx.x; i < 2 * (a.nlev + a.nblev); i += NT) lvK[i] = a.lv_K[i];
 
     const int tid = threadIdx.x;
     const int g = tid % G, s = tid / G;
@@ -228,6 +272,16 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     const int nlds = a.nlds;
     auto XL = [&](int slot) -> FT {
         return slot < nlds ? Xl[slot * G + g] : Xgb[(size_t)(slot - nlds) * G + g];
+    };
+    // Operand read for the level loops: always an LDS read (clamped slot) and, only for a
+    // spilled slot, a global load into a separate register. A shared destination register
+    // would make every LDS read wait for all outstanding global loads (the prefetched items).
+    auto XLo = [&](int slot) -> FT {
+        const bool in_lds = slot < nlds;
+        const FT vl = Xl[min(slot, nlds - 1) * G + g];
+        FT vg = (FT)0;
+        if (!in_lds) vg = Xgb[(size_t)(slot - nlds) * G + g];
+        return (in_lds ? vl : (FT)0) + vg;  // a sum, not a select: keeps vl and vg apart
     };
     auto XS = [&](int slot, FT v) {
         if (slot < nlds) Xl[slot * G + g] = v;
@@ -350,95 +404,156 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         __syncthreads();
 
         // ---- factor + forward solve --------------------------------------------
-        // Each lane walks its items of the level; the next chunk (of this level, or the
-        // first of the next level) is loaded before the current one is used, so item
-        // loads overlap the LDS work and the barrier between levels.
+        // The chunks of all levels form one sequence, walked with three register
+        // buffers in turn: while one chunk is worked on, the next two (possibly of the
+        // next level, across the barrier) are loading. The operands of a chunk are read
+        // before its prefetch is issued, so waiting for them leaves the prefetches in flight.
         {
-            int4 cur[kSC];
-            if (act) stream_chunk(a.fs, a.nlev > 0 ? a.f_lv[0] : int4{0, 0, 0, 0}, 0, nslot, s, cur);
-            for (int l = 0; l < a.nlev; ++l) {
-                const int4 lv = a.f_lv[l];
-                if (act && lv.y == 0 && l + 1 < a.nlev) stream_chunk(a.fs, a.f_lv[l + 1], 0, nslot, s, cur);
-                if (act) {
-                    FT acc = (FT)0;
-                    for (int k = 0; k < lv.y; k += kSC) {
-                        int4 nxt[kSC];
-                        if (k + kSC < lv.y) stream_chunk(a.fs, lv, k + kSC, nslot, s, nxt);
-                        else if (l + 1 < a.nlev) stream_chunk(a.fs, a.f_lv[l + 1], 0, nslot, s, nxt);
-                        else stream_chunk(a.fs, int4{0, 0, 0, 0}, 0, nslot, s, nxt);
+            // level sizes of the load cursor (lK, lU) and of the process cursor (pK), uniform
+            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
+            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
+            int lK = lvk(0, 0), lU = lvk(0, 1);
+            bool more = true;                 // the lane has items left in the load cursor's level
+            auto prefetch = [&](int4 (&q)[kSC]) {
+                const bool on = act && more && s < lU;
 #pragma unroll
-                        for (int u = 0; u < kSC; ++u) {
-                            const int4 q = cur[u];
-                            if (q.x >= 0) acc += XL(q.x) * XL(q.y) * XL(q.z);
-                            if (q.w >= 0) {  // end of a task: the team's lanes reduce, the leader writes
-                                const int lt = (q.w >> 26) & 15;
-                                for (int o = 1; o < (1 << lt); o <<= 1) acc += __shfl_xor(acc, o * G);
-                                if ((s & ((1 << lt) - 1)) == 0) {
-                                    const int slot = q.w & ((1 << 26) - 1);
-                                    const bool recip = (q.w >> 30) & 1;
-                                    // one explicit read-modify-write per address space (a merged generic
-                                    // pointer for this slot trips the gfx950 backend)
-                                    if (slot < nlds) {
-                                        FT v = Xl[slot * G + g] - acc;
-                                        if (recip) {
-                                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                            v = (FT)1 / v;
-                                        }
-                                        Xl[slot * G + g] = v;
-                                    } else {
-                                        FT v = Xgb[(size_t)(slot - nlds) * G + g] - acc;
-                                        if (recip) {
-                                            if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                            v = (FT)1 / v;
-                                        }
-                                        Xgb[(size_t)(slot - nlds) * G + g] = v;
+                for (int u = 0; u < kSC; ++u)
+                    q[u] = (on && lk + u < lK) ? a.fs[(size_t)lbase + (size_t)(lk + u) * lU + s] : no_item();
+                lk += kSC;
+                if (lk >= lK) {
+                    lbase += lK * lU;
+                    ++ll;
+                    lk = 0;
+                    lK = lvk(ll, 0);
+                    lU = lvk(ll, 1);
+                    more = true;
+                }
+            };
+            int pl = 0, pk = 0, pK = lK, pU = lU;  // process cursor
+            const int wave_s0 = (threadIdx.x & ~63) / G;  // first instance slot of this wave
+            FT acc = (FT)0;
+            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+                FT prod[kSC];
+                bool last = false;
+                // a wave none of whose lanes is used at this level only keeps the cursors and the barrier
+                const bool wave_on = wave_s0 < pU;
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) {
+                    if (!wave_on) break;
+                    const int4 q = cur[u];
+                    prod[u] = (FT)0;
+                    if (max(max((unsigned)q.x, (unsigned)q.y), (unsigned)q.z) < (unsigned)nlds) prod[u] = Xl[q.x * G + g] * Xl[q.y * G + g] * Xl[q.z * G + g];  // all operands in LDS
+                    else if (q.x >= 0) prod[u] = XL(q.x) * XL(q.y) * XL(q.z);
+                    last |= q.w < 0;
+                }
+                // the lane's last item: later chunks of the load cursor's level are skipped
+                if (last && pl == ll) more = false;
+                prefetch(nxt);
+                if (act && wave_on) {
+#pragma unroll
+                    for (int u = 0; u < kSC; ++u) {
+                        const int e = cur[u].w;
+                        acc += prod[u];
+                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
+                            const int lt = (e >> 26) & 7;
+                            acc = team_sum<G>(acc, lt);
+                            if ((s & ((1 << lt) - 1)) == 0) {
+                                const int slot = e & ((1 << 26) - 1);
+                                const bool recip = (e >> 29) & 1;
+                                // one explicit read-modify-write per address space (a merged generic
+                                // pointer for this slot trips the gfx950 backend)
+                                if (slot < nlds) {
+                                    FT v = Xl[slot * G + g] - acc;
+                                    if (recip) {
+                                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                        v = pivot_recip(v);
                                     }
+                                    Xl[slot * G + g] = v;
+                                } else {
+                                    FT v = Xgb[(size_t)(slot - nlds) * G + g] - acc;
+                                    if (recip) {
+                                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
+                                        v = pivot_recip(v);
+                                    }
+                                    Xgb[(size_t)(slot - nlds) * G + g] = v;
                                 }
-                                acc = (FT)0;
+                            }
+                            acc = (FT)0;
+                        }
+                    }
+                }
+                pk += kSC;
+                if (pk >= pK) {  // the level's last chunk: its writes are visible to the next level
+                    __syncthreads();
+                    ++pl;
+                    pk = 0;
+                    pK = lvk(pl, 0);
+                    pU = lvk(pl, 1);
+                }
+            };
+            int4 A[kSC], B[kSC], C[kSC];
+            prefetch(A);
+            prefetch(B);
+            while (pl < a.nlev) {
+                process(A, C);
+                if (pl >= a.nlev) break;
+                process(B, A);
+                if (pl >= a.nlev) break;
+                process(C, B);
+            }
+        }
+        // ---- dense tail: the m x m tile of every instance, by all lanes ------------------
+        // Tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m), both in LDS.
+        // Right-looking LU with diagonal pivots, one pass (and one barrier) per column k:
+        // pass k updates the trailing block and the vector with L(i,k) = T(i,k) / T(k,k);
+        // column k itself is scaled, and its pivot replaced by the reciprocal, in pass
+        // k + 1 (so that no pass writes what it reads). Then the backward sweep,
+        // one pass per column from the last: x_k = y_k / d_k, y_i -= U(i,k) x_k (i < k),
+        // x_k stored one pass later.
+        if (a.m > 0) {
+            const int m = a.m, yb = m * m;
+            auto T = [&](int i, int j) -> int { return (i * m + j) * G + g; };  // LDS index
+            FT rd_prev = (FT)0;
+            for (int k = 0; k <= m; ++k) {
+                if (act) {
+                    FT rd = (FT)0;
+                    if (k < m) {
+                        const FT d = Xl[T(k, k)];
+                        if (s == 0 && (!(fabs((double)d) > 0.0) || !isfinite((double)d))) s_piv[g] = 1;
+                        rd = pivot_recip(d);
+                        const int w = m - k - 1;  // trailing block w x w plus the vector column
+                        for (int idx = s; idx < w * (w + 1); idx += nslot) {
+                            const int i = k + 1 + idx / (w + 1), jj = idx % (w + 1);
+                            const FT lik = Xl[T(i, k)] * rd;
+                            if (jj < w) {
+                                const int j = k + 1 + jj;
+                                Xl[T(i, j)] -= lik * Xl[T(k, j)];
+                            } else {
+                                Xl[(yb + i) * G + g] -= lik * Xl[(yb + k) * G + g];
                             }
                         }
-#pragma unroll
-                        for (int u = 0; u < kSC; ++u) cur[u] = nxt[u];
                     }
+                    if (k > 0) {  // finish column k - 1
+                        for (int i = k + s; i < m; i += nslot) Xl[T(i, k - 1)] *= rd_prev;
+                        if (s == 0) Xl[T(k - 1, k - 1)] = rd_prev;
+                    }
+                    rd_prev = rd;
                 }
                 __syncthreads();
             }
-        }
-        // ---- dense tail: wave 0 factors the m x m tile of every instance --------------
-        // (tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m); both in LDS)
-        if (a.m > 0) {
-            if (tid < 64 && act) {
-                const int m = a.m;
-                constexpr int nl = 64 / G;
-                const int r = tid / G;
-                // plain index arithmetic on the LDS array (no generic pointers into LDS)
-                const int yb = m * m;
-                for (int k = 0; k < m; ++k) {
-                    const FT d = Xl[(k * m + k) * G + g];
-                    if (r == 0 && (!(fabs((double)d) > 0.0) || !isfinite((double)d))) s_piv[g] = 1;
-                    const FT rd = (FT)1 / d;
-                    for (int i = k + 1 + r; i < m; i += nl) Xl[(i * m + k) * G + g] *= rd;
-                    wave_sync();
-                    const int w = m - k - 1;
-                    for (int idx = r; idx < w * w; idx += nl) {
-                        const int i = k + 1 + idx / w, j = k + 1 + idx % w;
-                        Xl[(i * m + j) * G + g] -= Xl[(i * m + k) * G + g] * Xl[(k * m + j) * G + g];
+            FT x_prev = (FT)0;
+            for (int k = m - 1; k >= -1; --k) {
+                if (act) {
+                    FT xk = (FT)0;
+                    if (k >= 0) {
+                        xk = Xl[(yb + k) * G + g] * Xl[T(k, k)];
+                        for (int i = s; i < k; i += nslot) Xl[(yb + i) * G + g] -= Xl[T(i, k)] * xk;
                     }
-                    const FT yk = Xl[(yb + k) * G + g];
-                    for (int i = k + 1 + r; i < m; i += nl) Xl[(yb + i) * G + g] -= Xl[(i * m + k) * G + g] * yk;
-                    wave_sync();
-                    if (r == 0) Xl[(k * m + k) * G + g] = rd;
+                    if (k + 1 < m && s == 0) Xl[(yb + k + 1) * G + g] = x_prev;
+                    x_prev = xk;
                 }
-                wave_sync();
-                for (int k = m - 1; k >= 0; --k) {
-                    const FT x = Xl[(yb + k) * G + g] * Xl[(k * m + k) * G + g];
-                    wave_sync();
-                    if (r == 0) Xl[(yb + k) * G + g] = x;
-                    for (int i = r; i < k; i += nl) Xl[(yb + i) * G + g] -= Xl[(i * m + k) * G + g] * x;
-                    wave_sync();
-                }
+                __syncthreads();
             }
-            __syncthreads();
         }
         if (s == 0 && act && s_piv[g]) {  // zero / non-finite pivot: klu_refactor failure
             s_status[g] = 6;
@@ -447,45 +562,89 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         __syncthreads();
         act = valid && !s_done[g];
 
-        // ---- backward solve ---------------------------------------------------------
+        // ---- backward solve (same pipelined walk as the factor phase) -----------------
         {
-            int4 cur[kSC];
-            if (act) stream_chunk(a.bs, a.nblev > 0 ? a.b_lv[0] : int4{0, 0, 0, 0}, 0, nslot, s, cur);
-            for (int l = 0; l < a.nblev; ++l) {
-                const int4 lv = a.b_lv[l];
-                if (act && lv.y == 0 && l + 1 < a.nblev) stream_chunk(a.bs, a.b_lv[l + 1], 0, nslot, s, cur);
-                if (act) {
-                    FT acc = (FT)0;
-                    for (int k = 0; k < lv.y; k += kSC) {
-                        int4 nxt[kSC];
-                        if (k + kSC < lv.y) stream_chunk(a.bs, lv, k + kSC, nslot, s, nxt);
-                        else if (l + 1 < a.nblev) stream_chunk(a.bs, a.b_lv[l + 1], 0, nslot, s, nxt);
-                        else stream_chunk(a.bs, int4{0, 0, 0, 0}, 0, nslot, s, nxt);
+            const lds_int* bK = lvK + 2 * a.nlev;
+            // level sizes of the load cursor (lK, lU) and of the process cursor (pK), uniform
+            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
+            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
+            int lK = lvk(0, 0), lU = lvk(0, 1);
+            bool more = true;                 // the lane has items left in the load cursor's level
+            auto prefetch = [&](int4 (&q)[kSC]) {
+                const bool on = act && more && s < lU;
 #pragma unroll
-                        for (int u = 0; u < kSC; ++u) {
-                            const int4 q = cur[u];
-                            if (q.x >= 0) acc += XL(q.x) * XL(q.y);
-                            if (q.z >= 0) {
-                                const int lt = (q.z >> 26) & 15;
-                                for (int o = 1; o < (1 << lt); o <<= 1) acc += __shfl_xor(acc, o * G);
-                                if ((s & ((1 << lt) - 1)) == 0) {
-                                    const int slot = q.z & ((1 << 26) - 1);
-                                    const FT rd = XL(q.w);
-                                    if (slot < nlds) {
-                                        Xl[slot * G + g] = (Xl[slot * G + g] - acc) * rd;
-                                    } else {
-                                        FT* xp = &Xgb[(size_t)(slot - nlds) * G + g];
-                                        *xp = (*xp - acc) * rd;
-                                    }
+                for (int u = 0; u < kSC; ++u)
+                    q[u] = (on && lk + u < lK) ? a.bs[(size_t)lbase + (size_t)(lk + u) * lU + s] : no_item();
+                lk += kSC;
+                if (lk >= lK) {
+                    lbase += lK * lU;
+                    ++ll;
+                    lk = 0;
+                    lK = lvk(ll, 0);
+                    lU = lvk(ll, 1);
+                    more = true;
+                }
+            };
+            int pl = 0, pk = 0, pK = lK, pU = lU;  // process cursor
+            const int wave_s0 = (threadIdx.x & ~63) / G;  // first instance slot of this wave
+            FT acc = (FT)0;
+            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+                FT prod[kSC];
+                bool last = false;
+                // a wave none of whose lanes is used at this level only keeps the cursors and the barrier
+                const bool wave_on = wave_s0 < pU;
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) {
+                    if (!wave_on) break;
+                    const int4 q = cur[u];
+                    prod[u] = (FT)0;
+                    if (max((unsigned)q.x, (unsigned)q.y) < (unsigned)nlds) prod[u] = Xl[q.x * G + g] * Xl[q.y * G + g];  // all operands in LDS
+                    else if (q.x >= 0) prod[u] = XL(q.x) * XL(q.y);
+                    last |= q.z < 0;
+                }
+                // the lane's last item: later chunks of the load cursor's level are skipped
+                if (last && pl == ll) more = false;
+                prefetch(nxt);
+                if (act && wave_on) {
+#pragma unroll
+                    for (int u = 0; u < kSC; ++u) {
+                        const int e = cur[u].z;
+                        acc += prod[u];
+                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
+                            const int lt = (e >> 26) & 7;
+                            acc = team_sum<G>(acc, lt);
+                            if ((s & ((1 << lt) - 1)) == 0) {
+                                const int slot = e & ((1 << 26) - 1);
+                                const FT rd = XL(cur[u].w);
+                                if (slot < nlds) {
+                                    Xl[slot * G + g] = (Xl[slot * G + g] - acc) * rd;
+                                } else {
+                                    FT* xp = &Xgb[(size_t)(slot - nlds) * G + g];
+                                    *xp = (*xp - acc) * rd;
                                 }
-                                acc = (FT)0;
                             }
+                            acc = (FT)0;
                         }
-#pragma unroll
-                        for (int u = 0; u < kSC; ++u) cur[u] = nxt[u];
                     }
                 }
-                __syncthreads();
+                pk += kSC;
+                if (pk >= pK) {  // the level's last chunk: its writes are visible to the next level
+                    __syncthreads();
+                    ++pl;
+                    pk = 0;
+                    pK = lvk(pl, 0);
+                    pU = lvk(pl, 1);
+                }
+            };
+            int4 A[kSC], B[kSC], C[kSC];
+            prefetch(A);
+            prefetch(B);
+            while (pl < a.nblev) {
+                process(A, C);
+                if (pl >= a.nblev) break;
+                process(B, A);
+                if (pl >= a.nblev) break;
+                process(C, B);
             }
         }
 
@@ -628,7 +787,7 @@ struct pfgpu_grid {
     DevBuf<double> j_cval, w;
     // per-(G, block size) lane streams of the factor/forward and backward schedules
     struct Streams {
-        DevBuf<int> f_lv, fs, b_lv, bs;
+        DevBuf<int> lv_K, fs, bs;
     };
     std::vector<std::unique_ptr<Streams>> streams;  // index 2*log2(G) + (block == 1024)
     // autotuned launch shape per precision (G, block), 0 = not tuned
@@ -672,16 +831,52 @@ int log2i(int g) {
 
 struct LaunchChoice {
     int G, NT, nlds, nspill, nblocks;
-    size_t lds;
+    size_t lds;      // dynamic LDS bytes: value slots, then the level table
+    size_t lvk_off;  // byte offset of the level table
 };
+
+template <typename FT, int NT>
+const void* nr_kernel_g(int G) {
+    switch (G) {
+        case 1: return (const void*)k_nr<FT, 1, NT>;
+        case 2: return (const void*)k_nr<FT, 2, NT>;
+        case 4: return (const void*)k_nr<FT, 4, NT>;
+        case 8: return (const void*)k_nr<FT, 8, NT>;
+        case 16: return (const void*)k_nr<FT, 16, NT>;
+        case 32: return (const void*)k_nr<FT, 32, NT>;
+        default: throw ApiError(PFGPU_E_INVALID, "instances_per_wg must be 1, 2, 4, 8, 16 or 32");
+    }
+}
+
+// Static LDS of one k_nr variant (per-instance scalars and the reduction buffer).
+size_t static_lds(int fsize, int G, int NT) {
+    static std::mutex mu;
+    static std::map<int, size_t> cache;
+    const int key = fsize * 100000 + G * 10000 + NT;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const void* fn = fsize == 4 ? (NT == 1024 ? nr_kernel_g<float, 1024>(G) : nr_kernel_g<float, 256>(G))
+                                : (NT == 1024 ? nr_kernel_g<double, 1024>(G) : nr_kernel_g<double, 256>(G));
+    hipFuncAttributes attr{};
+    HIP_OK(hipFuncGetAttributes(&attr, fn));
+    cache[key] = attr.sharedSizeBytes;
+    return attr.sharedSizeBytes;
+}
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // Launch heuristic: the largest G whose value arrays fit entirely in LDS while
 // keeping at least two workgroups per CU; otherwise G = 1 with the hottest
 // slots in LDS and the rest spilled to global memory.
 LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req,
                            int NT_req = 0) {
-    // keep room for the kernel's static LDS (per-instance scalars and the reduction buffer: < 5 KB)
-    const size_t lds_avail = (size_t)std::max(0, ctx->lds_max - 8192);
+    // per workgroup: static LDS of the variant + value slots + the level table
+    const size_t table = align16((size_t)(P.nlev + P.nblev) * 8);
+    auto avail = [&](int G, int NT) -> size_t {
+        const size_t fixed = static_lds(fsize, G, NT) + table + 16;
+        return (size_t)ctx->lds_max > fixed ? (size_t)ctx->lds_max - fixed : 0;
+    };
     LaunchChoice c{};
     if (G_req > 0) {
         c.G = G_req;
@@ -690,26 +885,30 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
         for (int G = kMaxG; G >= 2; G >>= 1) {
             const size_t need = (size_t)P.nvals * G * fsize;
             const long long blocks = (ninst + G - 1) / G;
-            if (need <= lds_avail && blocks >= 2LL * std::max(1, ctx->n_cu)) { c.G = G; break; }
+            if (need <= avail(G, 1024) && blocks >= 2LL * std::max(1, ctx->n_cu)) { c.G = G; break; }
         }
         if (c.G == 1) {
             // small batches: still allow G>1 if it fits and there is more than one instance
             for (int G = std::min(kMaxG, 8); G >= 2; G >>= 1) {
                 const size_t need = (size_t)P.nvals * G * fsize;
-                if (need <= lds_avail && ninst >= 4 * G * std::max(1, ctx->n_cu)) { c.G = G; break; }
+                if (need <= avail(G, 1024) && ninst >= 4 * G * std::max(1, ctx->n_cu)) { c.G = G; break; }
             }
         }
     }
     // the dense tail must be LDS-resident
-    while (c.G > 1 && (size_t)(P.m * P.m + P.m) * c.G * fsize > lds_avail) c.G >>= 1;
-    const size_t cap = lds_avail / ((size_t)c.G * fsize);
+    while (c.G > 1 && (size_t)(P.m * P.m + P.m) * c.G * fsize > avail(c.G, 1024)) c.G >>= 1;
+    // one workgroup per CU when the value arrays take more than half of the LDS: give it 16 waves
+    const size_t full = (size_t)P.nvals * c.G * fsize;
+    c.NT = full > avail(c.G, 1024) / 2 ? 1024 : 256;
+    if (NT_req == 256 || NT_req == 1024) c.NT = NT_req;
+    const size_t cap = avail(c.G, c.NT) / ((size_t)c.G * fsize);
     c.nlds = (int)std::min<size_t>((size_t)P.nvals, cap);
+    if (c.nlds < std::max(1, P.m * P.m + P.m))
+        throw ApiError(PFGPU_E_INVALID, "the grid's dense tail does not fit in LDS");
     c.nspill = P.nvals - c.nlds;
     c.nblocks = (ninst + c.G - 1) / c.G;
-    c.lds = (size_t)c.nlds * c.G * fsize;
-    // one workgroup per CU when the value arrays take more than half of the LDS: give it 16 waves
-    c.NT = c.lds > lds_avail / 2 ? 1024 : 256;
-    if (NT_req == 256 || NT_req == 1024) c.NT = NT_req;
+    c.lvk_off = align16((size_t)c.nlds * c.G * fsize);
+    c.lds = c.lvk_off + table;
     return c;
 }
 
@@ -722,9 +921,11 @@ void ensure_streams(pfgpu_grid* gr, int G, int NT) {
     const pfgpu::LaneStreams L = pfgpu::build_lane_streams(gr->P, G, NT / G);
     auto t = std::make_unique<pfgpu_grid::Streams>();
     auto up = [](DevBuf<int>& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>(4, -1) : v); };
-    up(t->f_lv, L.f_lv);
+    std::vector<int> K;
+    for (size_t l = 0; l < L.f_lv.size() / 4; ++l) { K.push_back(L.f_lv[4 * l + 1]); K.push_back(L.f_lv[4 * l + 3]); }
+    for (size_t l = 0; l < L.b_lv.size() / 4; ++l) { K.push_back(L.b_lv[4 * l + 1]); K.push_back(L.b_lv[4 * l + 3]); }
+    up(t->lv_K, K);
     up(t->fs, L.fs);
-    up(t->b_lv, L.b_lv);
     up(t->bs, L.bs);
     gr->streams[ix] = std::move(t);
 }
@@ -803,8 +1004,8 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
     a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
     const auto& ls = *gr->streams[shape_index(lc.G, lc.NT)];
-    a.f_lv = reinterpret_cast<const int4*>(ls.f_lv.p); a.fs = reinterpret_cast<const int4*>(ls.fs.p);
-    a.b_lv = reinterpret_cast<const int4*>(ls.b_lv.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
+    a.lv_K = ls.lv_K.p; a.lvk_off = (int)lc.lvk_off;
+    a.fs = reinterpret_cast<const int4*>(ls.fs.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
     a.w = gr->w.p;
     a.Yx = Ypt; a.y_stride = ystride;
     a.S = d_S; a.s_stride = (long long)sbus_stride * 2;

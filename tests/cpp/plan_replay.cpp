@@ -60,28 +60,35 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                     for (int k = 0; k < 3; ++k)
                         if (wlev[P.pairs[4 * p + k]] >= l) ++conflicts;
         // lane streams: all lanes of a team (aligned group of T lanes, T from the end
-        // marker) end each task at the same item with the same marker, every task is
-        // ended by T lanes, and the items cover each task's pairs exactly once
+        // word) end each task at the same item with the same end word, every task is
+        // ended by T lanes, the items cover each task's pairs exactly once, and every
+        // used lane has one "last item" flag after which it holds nothing
         auto check_streams = [&](const std::vector<int>& lv, const std::vector<int>& st, int nl, bool fwd,
                                  const std::vector<int>& tasks, const std::vector<int>& lvl) {
             std::vector<long> npair_seen(P.nvals, 0), ends(P.nvals, 0), pair_sum(P.nvals, 0), team_of(P.nvals, 0),
                 recip_of(P.nvals, 0);
             const int ecol = fwd ? 3 : 2;
             for (int l = 0; l < nl; ++l) {
-                const int base = lv[4 * l], K = lv[4 * l + 1];
-                auto item = [&](int k, int s) { return &st[4 * ((size_t)base + (size_t)k * nslot + s)]; };
-                for (int s = 0; s < nslot; ++s) {
+                const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
+                if (U < 1 || U > nslot || K < 1) ++conflicts;
+                auto item = [&](int k, int s) { return &st[4 * ((size_t)base + (size_t)k * U + s)]; };
+                for (int s = 0; s < U; ++s) {
                     long c = 0, key = 0;
+                    int nlast = 0;
                     for (int k = 0; k < K; ++k) {
                         const int* it = item(k, s);
-                        if (it[0] >= 0) { ++c; key += (long)it[0] * 7 + (long)it[1] * 13 + (fwd ? (long)it[2] * 17 : 0); }
                         const int e = it[ecol];
-                        if (e >= 0) {
-                            const int T = 1 << ((e >> kStreamTeamShift) & 15);
+                        if (nlast) {  // nothing may follow the lane's last item
+                            if (it[0] >= 0 || (e & kStreamEnd)) ++conflicts;
+                            continue;
+                        }
+                        if (it[0] >= 0) { ++c; key += (long)it[0] * 7 + (long)it[1] * 13 + (fwd ? (long)it[2] * 17 : 0); }
+                        if (e & kStreamEnd) {
+                            const int T = 1 << ((e >> kStreamTeamShift) & 7);
                             if (T * G > 64 || T > nslot) ++conflicts;
                             const int s0 = s & ~(T - 1);
                             for (int tp = 0; tp < T; ++tp)
-                                if (item(k, s0 + tp)[ecol] != e) ++conflicts;
+                                if (s0 + tp >= U || ((item(k, s0 + tp)[ecol] ^ e) & ~kStreamLast) != 0) ++conflicts;
                             const int o = e & kStreamSlotMask;
                             npair_seen[o] += c;
                             pair_sum[o] += key;
@@ -91,8 +98,9 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                             c = 0;
                             key = 0;
                         }
+                        if (e & kStreamLast) nlast = 1;
                     }
-                    if (c != 0) ++conflicts;  // pairs after the last task end
+                    if (c != 0 || nlast != 1) ++conflicts;  // pairs after the last task end / no last flag
                 }
                 for (int t = lvl[l]; t < lvl[l + 1]; ++t) {
                     const int o = tasks[4 * t];
@@ -154,37 +162,45 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
             X[sl] = (P.j_slot[q] >> 30) & 1 ? rnd(1.0 / rnd(v)) : rnd(v);
         }
         std::vector<double> acc(nslot);
-        // one level of a lane stream: every lane accumulates its items; at a task's end
-        // item the team's partial sums are added and the leader (first lane) writes
-        auto run_level = [&](const std::vector<int>& st, int base, int K, bool fwd) {
+        // one level of a lane stream: every used lane accumulates its items up to its last
+        // one; at a task's end item the team's partial sums are added and the leader writes
+        auto run_level = [&](const std::vector<int>& st, const std::vector<int>& lv, int l, bool fwd) {
+            const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
             std::fill(acc.begin(), acc.end(), 0.0);
+            std::vector<char> more(U, 1);
             const int ecol = fwd ? 3 : 2;
             for (int k = 0; k < K; ++k) {
-                for (int s = 0; s < nslot; ++s) {
-                    const int* it = &st[4 * ((size_t)base + (size_t)k * nslot + s)];
+                for (int s = 0; s < U; ++s) {
+                    if (!more[s]) continue;
+                    const int* it = &st[4 * ((size_t)base + (size_t)k * U + s)];
                     if (it[0] < 0) continue;
                     acc[s] = fwd ? rnd(acc[s] + rnd(rnd(X[it[0]] * X[it[1]]) * X[it[2]]))
                                  : rnd(acc[s] + rnd(X[it[0]] * X[it[1]]));
                 }
-                for (int s = 0; s < nslot; ++s) {
-                    const int* it = &st[4 * ((size_t)base + (size_t)k * nslot + s)];
+                for (int s = 0; s < U; ++s) {
+                    if (!more[s]) continue;
+                    const int* it = &st[4 * ((size_t)base + (size_t)k * U + s)];
                     const int e = it[ecol];
-                    if (e < 0) continue;
-                    const int T = 1 << ((e >> kStreamTeamShift) & 15);
-                    if ((s & (T - 1)) != 0) continue;
-                    double sum = 0;
-                    for (int tp = 0; tp < T; ++tp) { sum = rnd(sum + acc[s + tp]); acc[s + tp] = 0; }
-                    const int o = e & kStreamSlotMask;
-                    if (fwd) {
-                        const double v = rnd(X[o] - sum);
-                        X[o] = (e & kStreamRecip) ? rnd(1.0 / v) : v;
-                    } else {
-                        X[o] = rnd(rnd(X[o] - sum) * X[it[3]]);
+                    if (e & kStreamEnd) {
+                        const int T = 1 << ((e >> kStreamTeamShift) & 7);
+                        if ((s & (T - 1)) == 0) {
+                            double sum = 0;
+                            for (int tp = 0; tp < T; ++tp) { sum = rnd(sum + acc[s + tp]); acc[s + tp] = 0; }
+                            const int o = e & kStreamSlotMask;
+                            if (fwd) {
+                                const double v = rnd(X[o] - sum);
+                                X[o] = (e & kStreamRecip) ? rnd(1.0 / v) : v;
+                            } else {
+                                X[o] = rnd(rnd(X[o] - sum) * X[it[3]]);
+                            }
+                        }
                     }
                 }
+                for (int s = 0; s < U; ++s)
+                    if (more[s] && (st[4 * ((size_t)base + (size_t)k * U + s) + ecol] & kStreamLast)) more[s] = 0;
             }
         };
-        for (int l = 0; l < P.nlev; ++l) run_level(LS.fs, LS.f_lv[4 * l], LS.f_lv[4 * l + 1], true);
+        for (int l = 0; l < P.nlev; ++l) run_level(LS.fs, LS.f_lv, l, true);
         // dense tail: right-looking LU of the m x m tile (slots [0, m*m)), forward on its vector
         // (slots [m*m, m*m+m)), then backward; the tile diagonal ends up holding 1/d
         if (P.m > 0) {
@@ -205,7 +221,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                 for (int i = 0; i < k; ++i) yt(i) = rnd(yt(i) - T(i, k) * x);
             }
         }
-        for (int l = 0; l < P.nblev; ++l) run_level(LS.bs, LS.b_lv[4 * l], LS.b_lv[4 * l + 1], false);
+        for (int l = 0; l < P.nblev; ++l) run_level(LS.bs, LS.b_lv, l, false);
         if (single_slack)
             for (int b = 0; b < n; ++b) { Va[b] = std::arg(Vc[b]); Vm[b] = std::abs(Vc[b]); }
         for (int j = 0; j < P.N; ++j) {
