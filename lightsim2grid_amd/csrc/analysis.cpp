@@ -7,6 +7,7 @@
 #include <queue>
 #include <sstream>
 #include <stdexcept>
+#include <string>
 
 namespace pfgpu {
 
@@ -366,6 +367,8 @@ std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch
 
 namespace pfgpu {
 
+void recycle_slots(DevicePlan& P);
+
 DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const SymbolicLU& S, const Schedule& sch) {
     DevicePlan P;
     const int n = Y.n, N = J.N;
@@ -536,7 +539,152 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
             P.btasks[4 * q + 3] = nw[j];
             P.b_maxp[l] = std::max(P.b_maxp[l], S.cp[j + 1] - S.cp[j]);
         }
+    recycle_slots(P);
     return P;
+}
+
+// ---------------------------------------------------------------------------
+// Slot recycling. Time axis of one NR iteration: 0 = J fill, l + 1 = factor level
+// l, nlev + 1 = dense tail, nlev + 2 + l = backward level l, nlev + 2 + nblev =
+// update. A value lives from its first write to its last read. J entries are
+// written at 0; a fill-in entry (neither a J entry nor in the dense tile) is
+// assigned by its own task (kind 3) instead of being zeroed at 0, so it lives from
+// its task's level only. The tile and the right-hand side / solution slots (written
+// by the mismatch of the previous iteration, read by the update) keep their slots;
+// every other value takes a physical slot freed by a value whose last read came at
+// an earlier time (greedy interval colouring, optimal for interval graphs).
+// Physical slots are then ordered tile first, then by decreasing hotness.
+// ---------------------------------------------------------------------------
+void recycle_slots(DevicePlan& P) {
+    // one extra slot: the shared zero (structural zeros, and the operand of padding items)
+    const int NV = P.nvals + 1, tile = P.m * P.m + P.m;
+    const int zslot = P.nvals;
+    const int nl = P.nlev, nb = P.nblev, T_END = nl + 2 + nb;
+    std::vector<char> fixed(NV, 0), isj(NV, 0);
+    for (int s = 0; s < tile; ++s) fixed[s] = 1;
+    for (int b = 0; b < P.n; ++b) {
+        if (P.fslot_p[b] >= 0) fixed[P.fslot_p[b]] = 1;
+        if (P.fslot_q[b] >= 0) fixed[P.fslot_q[b]] = 1;
+    }
+    for (int x : P.ux_slot) fixed[x] = 1;
+    for (int q = 0; q < P.nJ; ++q) isj[P.j_slot[q] & ((1 << 30) - 1)] = 1;
+    // fill-in entries outside the tile: their task assigns them. Entries of the
+    // symmetrised pattern that are neither J entries nor written by any task are
+    // structural zeros: they all share one slot, zeroed by the J fill.
+    std::vector<char> fill(NV, 0), written(NV, 0);
+    for (size_t t = 0; t < P.tasks.size() / 4; ++t) written[P.tasks[4 * t]] = 1;
+    std::vector<char> zero(NV, 0);
+    for (int s = 0; s < NV; ++s) {
+        fill[s] = !fixed[s] && !isj[s] && written[s];
+        zero[s] = !fixed[s] && !isj[s] && !written[s];
+    }
+    for (size_t t = 0; t < P.tasks.size() / 4; ++t)
+        if (fill[P.tasks[4 * t]]) P.tasks[4 * t + 3] = 3;
+    {
+        std::vector<int> z;
+        for (int s : P.zero_slots)
+            if (!fill[s] && !zero[s]) z.push_back(s);
+        z.push_back(zslot);
+        P.zero_slots.swap(z);
+    }
+    fixed[zslot] = 1;
+    // intervals
+    std::vector<int> start(NV, 1 << 30), end(NV, -1), hot(NV, -1);
+    auto acc = [&](int s, int t, bool w) {
+        end[s] = std::max(end[s], t);
+        if (w) start[s] = std::min(start[s], t);
+        if (t < T_END) hot[s] = std::max(hot[s], t);
+    };
+    for (int q = 0; q < P.nJ; ++q) acc(P.j_slot[q] & ((1 << 30) - 1), 0, true);
+    for (int s : P.zero_slots) acc(s, 0, true);
+    for (int l = 0; l < nl; ++l)
+        for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
+            acc(P.tasks[4 * t], l + 1, true);
+            for (int q = P.tasks[4 * t + 1]; q < P.tasks[4 * t + 2]; ++q)
+                for (int k = 0; k < 3; ++k) acc(P.pairs[4 * q + k], l + 1, false);
+        }
+    for (int l = 0; l < nb; ++l)
+        for (int t = P.b_lvl[l]; t < P.b_lvl[l + 1]; ++t) {
+            const int tt = nl + 2 + l;
+            acc(P.btasks[4 * t], tt, true);
+            acc(P.btasks[4 * t + 3], tt, false);
+            for (int q = P.btasks[4 * t + 1]; q < P.btasks[4 * t + 2]; ++q) {
+                acc(P.bpairs[2 * q], tt, false);
+                acc(P.bpairs[2 * q + 1], tt, false);
+            }
+        }
+    for (int s = 0; s < NV; ++s) {
+        if (zero[s] && s != zslot) continue;  // aliases of the shared zero slot
+        if (fixed[s]) { start[s] = -1; end[s] = T_END; }
+        if (end[s] < 0) throw std::runtime_error("pfgpu: value slot never used");
+        if (start[s] > end[s])
+            throw std::runtime_error("pfgpu: value slot read before it is written (slot " + std::to_string(s) + " start " +
+                                     std::to_string(start[s]) + " end " + std::to_string(end[s]) + " fill " +
+                                     std::to_string((int)fill[s]) + " j " + std::to_string((int)isj[s]) + ")");
+    }
+    // colour the non-fixed values
+    std::vector<int> phys(NV, -1);
+    int nphys = 0;
+    for (int s = 0; s < NV; ++s)
+        if (fixed[s]) phys[s] = nphys++;  // placeholder ids, reordered below
+    for (int s = 0; s < NV; ++s)
+        if (zero[s] && s != zslot) phys[s] = phys[zslot];
+    std::vector<int> ord;
+    for (int s = 0; s < NV; ++s)
+        if (!fixed[s] && !zero[s]) ord.push_back(s);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return start[a] < start[b]; });
+    using FE = std::pair<int, int>;  // (last end, physical)
+    std::priority_queue<FE, std::vector<FE>, std::greater<FE>> freeq;
+    for (int s : ord) {
+        if (!freeq.empty() && freeq.top().first < start[s]) {
+            const int ph = freeq.top().second;
+            freeq.pop();
+            phys[s] = ph;
+        } else {
+            phys[s] = nphys++;
+        }
+        freeq.push({end[s], phys[s]});
+    }
+    // physical hotness and final order: tile first, then decreasing hotness
+    std::vector<int> phot(nphys, -1);
+    for (int s = 0; s < NV; ++s) phot[phys[s]] = std::max(phot[phys[s]], hot[s]);
+    for (int s = 0; s < NV; ++s)  // the zero slot's aliases are read-only zeros at all times
+        if (zero[s]) { start[s] = -1; end[s] = T_END; }
+    std::vector<int> pord, fin(nphys, -1);
+    for (int s = 0; s < tile; ++s) fin[phys[s]] = s;
+    for (int ph = 0; ph < nphys; ++ph)
+        if (fin[ph] < 0) pord.push_back(ph);
+    std::stable_sort(pord.begin(), pord.end(), [&](int a, int b) { return phot[a] > phot[b]; });
+    for (size_t i = 0; i < pord.size(); ++i) fin[pord[i]] = tile + (int)i;
+    std::vector<int> mp(NV);
+    for (int s = 0; s < NV; ++s) mp[s] = fin[phys[s]];
+    // self-check: the values sharing a physical slot have disjoint lifetimes
+    {
+        std::vector<std::vector<std::pair<int, int>>> by(nphys);
+        for (int s = 0; s < NV; ++s)
+            if (!zero[s] || s == zslot) by[mp[s]].push_back({start[s], end[s]});
+        for (auto& v : by) {
+            std::sort(v.begin(), v.end());
+            for (size_t i = 1; i < v.size(); ++i)
+                if (v[i].first <= v[i - 1].second) throw std::runtime_error("pfgpu: slot recycling overlap");
+        }
+    }
+    // rewrite every slot reference
+    for (int& x : P.j_slot) x = mp[x & ((1 << 30) - 1)] | (x & (1 << 30));
+    for (int& x : P.fslot_p) if (x >= 0) x = mp[x];
+    for (int& x : P.fslot_q) if (x >= 0) x = mp[x];
+    for (int& x : P.ux_slot) x = mp[x];
+    for (int& x : P.zero_slots) x = mp[x];
+    for (size_t t = 0; t < P.tasks.size() / 4; ++t) P.tasks[4 * t] = mp[P.tasks[4 * t]];
+    for (size_t q = 0; q < P.pairs.size() / 4; ++q)
+        for (int k = 0; k < 3; ++k) P.pairs[4 * q + k] = mp[P.pairs[4 * q + k]];
+    for (size_t t = 0; t < P.btasks.size() / 4; ++t) {
+        P.btasks[4 * t] = mp[P.btasks[4 * t]];
+        P.btasks[4 * t + 3] = mp[P.btasks[4 * t + 3]];
+    }
+    for (int& x : P.bpairs) x = mp[x];
+    P.nvals = nphys;
+    P.zslot = mp[zslot];
 }
 
 std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<int>& maxp, int G, int nslot) {
@@ -666,7 +814,8 @@ LevelPlacement choose_placement(const std::vector<int>& np, int G, int nslot) {
 // level then gets kStreamLast in its end word.
 template <class NPairs, class Emit>
 void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs npairs_of, Emit emit, int end_word,
-                    std::vector<int>& lv, std::vector<int>& stream) {
+                    const int (&pad)[4], int lane_round, int k_round, bool flag_last, std::vector<int>& lv,
+                    std::vector<int>& stream) {
     const int nlev = (int)lvl_ptr.size() - 1;
     lv.assign(4 * (size_t)std::max(nlev, 0), 0);
     stream.clear();
@@ -676,12 +825,13 @@ void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs np
         std::vector<int> np(nt);
         for (int i = 0; i < nt; ++i) np[i] = npairs_of(t0 + i);
         const LevelPlacement pl = choose_placement(np, G, nslot);
-        const int K = pl.K;
+        const int K = (pl.K + k_round - 1) / k_round * k_round;
         int maxlt = 0, U = 0;
         for (int i = 0; i < nt; ++i) {
             maxlt = std::max(maxlt, pl.lt[i]);
             U = std::max(U, pl.lane0[i] + (1 << pl.lt[i]));
         }
+        U = std::min(nslot, (U + lane_round - 1) / lane_round * lane_round);
         lv[4 * l] = (int)base;
         lv[4 * l + 1] = K;
         lv[4 * l + 2] = 1 << maxlt;
@@ -689,12 +839,8 @@ void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs np
         if (base + (size_t)K * U >= (size_t)1 << 31) throw std::runtime_error("lane stream too large");
         const size_t at = stream.size();
         stream.resize(4 * (base + (size_t)K * U));
-        for (size_t q = at; q < stream.size(); q += 4) {
-            stream[q] = stream[q + 1] = stream[q + 2] = -1;
-            stream[q + 3] = 0;
-        }
-        if (end_word == 2)  // backward items keep the 1/d slot in the last word
-            for (size_t q = at; q < stream.size(); q += 4) { stream[q + 2] = 0; stream[q + 3] = -1; }
+        for (size_t q = at; q < stream.size(); q += 4)
+            for (int w = 0; w < 4; ++w) stream[q + w] = pad[w];
         std::vector<int> count(U, 0);
         for (int i = 0; i < nt; ++i) {
             const int T = 1 << pl.lt[i];
@@ -710,50 +856,57 @@ void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs np
                 count[lane] = std::max(count[lane], pl.k0[i] + c);
             }
         }
-        for (int lane = 0; lane < U; ++lane) {
-            if (count[lane] == 0) throw std::runtime_error("lane streams: idle lane inside the used range");
-            stream[4 * (base + (size_t)(count[lane] - 1) * U + lane) + end_word] |= kStreamLast;
-        }
+        if (flag_last)
+            for (int lane = 0; lane < U; ++lane) {
+                if (count[lane] == 0) throw std::runtime_error("lane streams: idle lane inside the used range");
+                stream[4 * (base + (size_t)(count[lane] - 1) * U + lane) + end_word] |= kStreamLast;
+            }
         base += (size_t)K * U;
     }
 }
 
 }  // namespace
 
-LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot) {
+LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const LeanFormat* lean) {
     if (G < 1 || nslot < 1) throw std::invalid_argument("build_lane_streams: bad launch shape");
     if (P.nvals >= kStreamSlotMask) throw std::invalid_argument("build_lane_streams: too many value slots");
+    const int sc = lean ? lean->elem_bytes : 1;  // slot -> stored value
+    if (lean && (int64_t)P.nvals * sc >= kStreamSlotMask) throw std::invalid_argument("lean streams: offsets too large");
+    const int zb = P.zslot * sc;
+    const int round = lean ? std::max(1, lean->lane_round) : 1;
     LaneStreams L;
     L.G = G;
     L.nslot = nslot;
+    const int fpad[4] = {lean ? zb : -1, lean ? zb : -1, lean ? zb : -1, 0};
+    const int bpad[4] = {lean ? zb : -1, lean ? zb : -1, 0, lean ? zb : -1};
     layout_streams(
         P.f_lvl, G, nslot, [&](int t) { return P.tasks[4 * t + 2] - P.tasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {
             if (pi >= 0) {
                 const int p = P.tasks[4 * t + 1] + pi;
-                it[0] = P.pairs[4 * p];
-                it[1] = P.pairs[4 * p + 1];
-                it[2] = P.pairs[4 * p + 2];
+                it[0] = P.pairs[4 * p] * sc;
+                it[1] = P.pairs[4 * p + 1] * sc;
+                it[2] = P.pairs[4 * p + 2] * sc;
             }
             if (last)
-                it[3] = P.tasks[4 * t] | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0) |
-                        kStreamEnd;
+                it[3] = P.tasks[4 * t] * sc | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0) |
+                        (P.tasks[4 * t + 3] == 3 ? kStreamAssign : 0) | kStreamEnd;
         },
-        3, L.f_lv, L.fs);
+        3, fpad, round, lean ? std::max(1, lean->k_round) : 1, !lean, L.f_lv, L.fs);
     layout_streams(
         P.b_lvl, G, nslot, [&](int t) { return P.btasks[4 * t + 2] - P.btasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {
             if (pi >= 0) {
                 const int p = P.btasks[4 * t + 1] + pi;
-                it[0] = P.bpairs[2 * p];
-                it[1] = P.bpairs[2 * p + 1];
+                it[0] = P.bpairs[2 * p] * sc;
+                it[1] = P.bpairs[2 * p + 1] * sc;
             }
             if (last) {
-                it[2] = P.btasks[4 * t] | (lt << kStreamTeamShift) | kStreamEnd;
-                it[3] = P.btasks[4 * t + 3];
+                it[2] = P.btasks[4 * t] * sc | (lt << kStreamTeamShift) | kStreamEnd;
+                it[3] = P.btasks[4 * t + 3] * sc;
             }
         },
-        2, L.b_lv, L.bs);
+        2, bpad, round, lean ? std::max(1, lean->k_round) : 1, !lean, L.b_lv, L.bs);
     return L;
 }
 

@@ -101,6 +101,7 @@ struct DevicePlan {
     int n = 0, N = 0, off = 0, nnzY = 0, nJ = 0, nvals = 0, slack0 = 0;
     // dense tail: slots [0, m*m) hold the row-major m x m block, [m*m, m*m+m) its forward/backward vector
     int m = 0;
+    int zslot = 0;  // a slot that always holds 0 (zeroed by the J fill, never written otherwise)
     int nlev = 0, nblev = 0;
     // Ybus row view for Ibus = Y V: row r covers [yr_ptr[r], yr_ptr[r+1]) with column yr_col and CSC value pos yr_pos
     std::vector<int> yr_ptr, yr_col, yr_pos;
@@ -115,12 +116,14 @@ struct DevicePlan {
     std::vector<int> j_ypos, j_rbus, j_cbus, j_slot;
     std::vector<uint8_t> j_sel;
     std::vector<double> j_cval;
-    // LU slots that hold no J entry (fill-in): zeroed by the J-fill phase
+    // dense-tile slots that hold no J entry (fill-in): zeroed by the J-fill phase
     std::vector<int> zero_slots;
     // factor + forward tasks (level-major), 4 ints each: {out slot, pair begin, pair end, kind};
-    // kind 0 = L/U entry, 1 = pivot (stores the reciprocal), 2 = forward-solve entry.
-    // Every task starts from the value already in its slot (J entry, 0, or F) and
-    // subtracts sum X[l]*X[u]*X[d] over its pairs. Tasks without pairs are not listed.
+    // kind 0 = L/U entry, 1 = pivot (stores the reciprocal), 2 = forward-solve entry,
+    // 3 = fill-in entry outside the dense tile (assigns -sum: its slot is recycled).
+    // Other tasks start from the value already in their slot (J entry, 0 for tile
+    // fill-in, or F) and subtract sum X[l]*X[u]*X[d] over their pairs. Tasks without
+    // pairs are not listed. Value slots are shared by values with disjoint lifetimes.
     std::vector<int> f_lvl;                 // nlev + 1
     std::vector<int> f_maxp;                // per level max pair count
     std::vector<int> tasks;                 // 4 * ntask
@@ -152,10 +155,12 @@ std::vector<int> level_teams(const std::vector<int>& lvl_ptr, const std::vector<
 // takes the smallest team that holds it in K items). A lane's last item of a
 // level is flagged, so that it loads nothing after it.
 // End word (forward: item[3], backward: item[2]):
-//   bits 0-25 slot, 26-28 log2(T), 29 reciprocal (forward pivots), 30 end of
-//   task, 31 the lane's last item of the level.
+//   bits 0-24 slot, 25 assign (forward fill-in: the slot takes -sum, its old
+//   content belongs to another value), 26-28 log2(T), 29 reciprocal (forward
+//   pivots), 30 end of task, 31 the lane's last item of the level.
 constexpr int kStreamTeamShift = 26;
-constexpr int kStreamSlotMask = (1 << 26) - 1;
+constexpr int kStreamSlotMask = (1 << 25) - 1;
+constexpr int kStreamAssign = 1 << 25;
 constexpr int kStreamRecip = 1 << 29;
 constexpr int kStreamEnd = 1 << 30;
 constexpr int kStreamLast = (int)(1u << 31);
@@ -170,7 +175,18 @@ struct LaneStreams {
     std::vector<int> bs;
 };
 
-LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot);
+// Lean encoding, for launches that keep every value slot in LDS: items hold LDS byte
+// offsets (slot * elem_bytes) instead of slot ids, items without a pair point at the
+// zero slot (no per-item checks), the lanes of a level are rounded up to lane_round
+// (a whole wave), the items per lane to k_round, and no last-item flags are set (every
+// lane of a used wave walks all K items; the padding items add zeros).
+struct LeanFormat {
+    int elem_bytes = 4;  // bytes between consecutive slots of one instance (G * value size)
+    int lane_round = 1;  // lanes per wave of one instance (64 / G)
+    int k_round = 1;     // items per lane are padded to a multiple of this (the kernel's chunk)
+};
+
+LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const LeanFormat* lean = nullptr);
 
 std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch);
 

@@ -196,9 +196,10 @@ __device__ __noinline__ double rect_of(double* __restrict__ V, const double* __r
 }
 
 // Lane streams. Level l uses lanes [0, U) with K items per lane, item k of lane s
-// at base + k * U + s. End word (forward: item.w, backward: item.z): bits 0-25
-// slot, 26-28 log2(team), 29 reciprocal, 30 end of task, 31 the lane's last item
-// of the level (the lane loads nothing after it).
+// at base + k * U + s. End word (forward: item.w, backward: item.z): bits 0-24
+// slot, 25 assign (a fill-in entry: its slot held another value before), 26-28
+// log2(team), 29 reciprocal, 30 end of task, 31 the lane's last item of the level
+// (the lane loads nothing after it).
 constexpr int kSC = 2;  // items per chunk
 constexpr int kEnd = 1 << 30;
 __device__ __forceinline__ int4 no_item() { return int4{-1, -1, 0, 0}; }
@@ -226,15 +227,17 @@ __device__ __forceinline__ double lane_down(double v) {
 }
 
 // Sum over a team of 2^lt lanes (stride G, aligned to 2^lt * G) into its lowest lane.
-// All lanes of a team share lt, so each step's source lanes are active.
+// maxlt (uniform) bounds the steps; lanes of smaller teams drop the later steps by a
+// select, not a branch. All lanes of a team share lt, so the lanes a leader reads are
+// active and in its team.
 template <int G, typename FT>
-__device__ __forceinline__ FT team_sum(FT acc, int lt) {
-    if (lt > 0) acc += lane_down<1 * G>(acc);
-    if constexpr (2 * G <= 32) if (lt > 1) acc += lane_down<2 * G>(acc);
-    if constexpr (4 * G <= 32) if (lt > 2) acc += lane_down<4 * G>(acc);
-    if constexpr (8 * G <= 32) if (lt > 3) acc += lane_down<8 * G>(acc);
-    if constexpr (16 * G <= 32) if (lt > 4) acc += lane_down<16 * G>(acc);
-    if constexpr (32 * G <= 32) if (lt > 5) acc += lane_down<32 * G>(acc);
+__device__ __forceinline__ FT team_sum(FT acc, int lt, int maxlt) {
+    if (maxlt > 0) { const FT t = lane_down<1 * G>(acc); acc += lt > 0 ? t : (FT)0; }
+    if constexpr (2 * G <= 32) if (maxlt > 1) { const FT t = lane_down<2 * G>(acc); acc += lt > 1 ? t : (FT)0; }
+    if constexpr (4 * G <= 32) if (maxlt > 2) { const FT t = lane_down<4 * G>(acc); acc += lt > 2 ? t : (FT)0; }
+    if constexpr (8 * G <= 32) if (maxlt > 3) { const FT t = lane_down<8 * G>(acc); acc += lt > 3 ? t : (FT)0; }
+    if constexpr (16 * G <= 32) if (maxlt > 4) { const FT t = lane_down<16 * G>(acc); acc += lt > 4 ? t : (FT)0; }
+    if constexpr (32 * G <= 32) if (maxlt > 5) { const FT t = lane_down<32 * G>(acc); acc += lt > 5 ? t : (FT)0; }
     return acc;
 }
 
@@ -242,8 +245,11 @@ __device__ __forceinline__ FT team_sum(FT acc, int lt) {
 // the FP64 form an IEEE division.
 __device__ __forceinline__ float pivot_recip(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ double pivot_recip(double v) { return 1.0 / v; }
+// zero or non-finite pivot (klu_refactor failure)
+template <typename FT>
+__device__ __forceinline__ bool bad_pivot(FT v) { return !(fabs(v) > (FT)0 && fabs(v) < (FT)INFINITY); }
 
-template <typename FT, int G, int NT>
+template <typename FT, int G, int NT, bool LEAN>
 __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     constexpr int kWaves = NT / 64;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -276,6 +282,11 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // Operand read for the level loops: always an LDS read (clamped slot) and, only for a
     // spilled slot, a global load into a separate register. A shared destination register
     // would make every LDS read wait for all outstanding global loads (the prefetched items).
+    // lean launches (every slot in LDS): items carry byte offsets of this instance's slots
+    typedef __attribute__((address_space(3))) unsigned char lds_u8;
+    lds_u8* Xb = (lds_u8*)lds_raw + g * (int)sizeof(FT);
+    auto XB = [&](int off) -> FT { return *(lds_ft*)(Xb + off); };
+    auto XBset = [&](int off, FT v) { *(lds_ft*)(Xb + off) = v; };
     auto XLo = [&](int slot) -> FT {
         const bool in_lds = slot < nlds;
         const FT vl = Xl[min(slot, nlds - 1) * G + g];
@@ -409,30 +420,111 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // next level, across the barrier) are loading. The operands of a chunk are read
         // before its prefetch is issued, so waiting for them leaves the prefetches in flight.
         {
-            // level sizes of the load cursor (lK, lU) and of the process cursor (pK), uniform
+            bool bad = false;  // a zero or non-finite pivot met by this lane
+            if constexpr (LEAN) {
             auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
-            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
-            int lK = lvk(0, 0), lU = lvk(0, 1);
-            bool more = true;                 // the lane has items left in the load cursor's level
+            int ll = 0, lk = 0, lbase = 0;
+            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
+            // every lane loads every chunk: lanes of a wave idle at a level read a valid item
+            // of that level they do not use, and past the last level item 0
             auto prefetch = [&](int4 (&q)[kSC]) {
-                const bool on = act && more && s < lU;
+                const bool in = ll < a.nlev;
+                const int sl = min(s, lU - 1);
 #pragma unroll
-                for (int u = 0; u < kSC; ++u)
-                    q[u] = (on && lk + u < lK) ? a.fs[(size_t)lbase + (size_t)(lk + u) * lU + s] : no_item();
+                for (int u = 0; u < kSC; ++u) q[u] = a.fs[in ? (size_t)lbase + (size_t)(lk + u) * lU + sl : (size_t)0];
                 lk += kSC;
                 if (lk >= lK) {
                     lbase += lK * lU;
                     ++ll;
                     lk = 0;
                     lK = lvk(ll, 0);
-                    lU = lvk(ll, 1);
+                    lU = lvk(ll, 1) & 0xffffff;
+                }
+            };
+            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;
+            const int wave_s0 = (threadIdx.x & ~63) / G;
+            FT acc = (FT)0;
+            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+                prefetch(nxt);
+                if (wave_s0 < pU) {  // waves with no lane in use at this level only keep the barrier
+#pragma unroll
+                    for (int u = 0; u < kSC; ++u) {
+                        const int4 q = cur[u];
+                        acc = fma(XB(q.x) * XB(q.y), XB(q.z), acc);
+                        const int e = q.w;
+                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
+                            const int lt = (e >> 26) & 7;
+                            const FT sum = team_sum<G>(acc, lt, pT);
+                            if ((s & ((1 << lt) - 1)) == 0) {
+                                const int off = e & ((1 << 25) - 1);
+                                FT v = (((e >> 25) & 1) ? (FT)0 : XB(off)) - sum;  // fill-in: assign
+                                if ((e >> 29) & 1) {
+                                    bad |= bad_pivot(v);
+                                    v = pivot_recip(v);
+                                }
+                                XBset(off, v);
+                            }
+                            acc = (FT)0;
+                        }
+                    }
+                }
+                pk += kSC;
+                if (pk >= pK) {
+                    __syncthreads();
+                    ++pl;
+                    pk = 0;
+                    pK = lvk(pl, 0);
+                    const int w1 = lvk(pl, 1);
+                    pU = w1 & 0xffffff;
+                    pT = w1 >> 24;
+                }
+            };
+            int4 A[kSC], B[kSC], C[kSC];
+            prefetch(A);
+            prefetch(B);
+            while (pl < a.nlev) {
+                process(A, C);
+                if (pl >= a.nlev) break;
+                process(B, A);
+                if (pl >= a.nlev) break;
+                process(C, B);
+            }
+            } else {
+            // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
+            // log2 of its largest team), uniform
+            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
+            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
+            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
+            bool more = true;                 // the lane has items left in the load cursor's level
+            // Every lane issues every prefetch load (a lane with nothing to load reads item 0
+            // as a stand-in) and keeps a validity bit per item, applied when the item is used:
+            // a load skipped by some lanes would make the compiler wait for all loads in flight.
+            auto prefetch = [&](int4 (&q)[kSC], int& vbits) {
+                const bool on = act && more && s < lU;
+                vbits = 0;
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) {
+                    const bool ok = on && lk + u < lK;
+                    q[u] = a.fs[ok ? (size_t)lbase + (size_t)(lk + u) * lU + s : (size_t)0];
+                    vbits |= (int)ok << u;
+                }
+                lk += kSC;
+                if (lk >= lK) {
+                    lbase += lK * lU;
+                    ++ll;
+                    lk = 0;
+                    lK = lvk(ll, 0);
+                    lU = lvk(ll, 1) & 0xffffff;
                     more = true;
                 }
             };
-            int pl = 0, pk = 0, pK = lK, pU = lU;  // process cursor
+            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;  // process cursor
             const int wave_s0 = (threadIdx.x & ~63) / G;  // first instance slot of this wave
             FT acc = (FT)0;
-            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+            auto process = [&](const int4 (&cur_raw)[kSC], int cur_v, int4 (&nxt)[kSC], int& nxt_v) {
+                int4 cur[kSC];
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) cur[u] = (cur_v >> u) & 1 ? cur_raw[u] : no_item();
                 FT prod[kSC];
                 bool last = false;
                 // a wave none of whose lanes is used at this level only keeps the cursors and the barrier
@@ -448,38 +540,36 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 }
                 // the lane's last item: later chunks of the load cursor's level are skipped
                 if (last && pl == ll) more = false;
-                prefetch(nxt);
+                prefetch(nxt, nxt_v);
                 if (act && wave_on) {
 #pragma unroll
                     for (int u = 0; u < kSC; ++u) {
                         const int e = cur[u].w;
                         acc += prod[u];
-                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
-                            const int lt = (e >> 26) & 7;
-                            acc = team_sum<G>(acc, lt);
-                            if ((s & ((1 << lt) - 1)) == 0) {
-                                const int slot = e & ((1 << 26) - 1);
-                                const bool recip = (e >> 29) & 1;
+                        // end of a task: the team's lanes reduce, the leader writes
+                        const bool end = e & kEnd;
+                        if (end) {
+                        const int lt = (e >> 26) & 7;
+                        const FT sum = team_sum<G>(acc, lt, pT);
+                        const bool lead = (s & ((1 << lt) - 1)) == 0;
+                            if (lead) {
+                                const int slot = e & ((1 << 25) - 1);
+                                const bool rc = (e >> 29) & 1;
                                 // one explicit read-modify-write per address space (a merged generic
                                 // pointer for this slot trips the gfx950 backend)
+                                const bool as = (e >> 25) & 1;  // fill-in: the slot's old content is not ours
                                 if (slot < nlds) {
-                                    FT v = Xl[slot * G + g] - acc;
-                                    if (recip) {
-                                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                        v = pivot_recip(v);
-                                    }
+                                    FT v = (as ? (FT)0 : Xl[slot * G + g]) - sum;
+                                    if (rc) { bad |= bad_pivot(v); v = pivot_recip(v); }
                                     Xl[slot * G + g] = v;
                                 } else {
-                                    FT v = Xgb[(size_t)(slot - nlds) * G + g] - acc;
-                                    if (recip) {
-                                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                                        v = pivot_recip(v);
-                                    }
+                                    FT v = (as ? (FT)0 : Xgb[(size_t)(slot - nlds) * G + g]) - sum;
+                                    if (rc) { bad |= bad_pivot(v); v = pivot_recip(v); }
                                     Xgb[(size_t)(slot - nlds) * G + g] = v;
                                 }
                             }
-                            acc = (FT)0;
                         }
+                        acc = end ? (FT)0 : acc;
                     }
                 }
                 pk += kSC;
@@ -488,19 +578,24 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     ++pl;
                     pk = 0;
                     pK = lvk(pl, 0);
-                    pU = lvk(pl, 1);
+                    const int w1 = lvk(pl, 1);
+                    pU = w1 & 0xffffff;
+                    pT = w1 >> 24;
                 }
             };
             int4 A[kSC], B[kSC], C[kSC];
-            prefetch(A);
-            prefetch(B);
+            int Av = 0, Bv = 0, Cv = 0;
+            prefetch(A, Av);
+            prefetch(B, Bv);
             while (pl < a.nlev) {
-                process(A, C);
+                process(A, Av, C, Cv);
                 if (pl >= a.nlev) break;
-                process(B, A);
+                process(B, Bv, A, Av);
                 if (pl >= a.nlev) break;
-                process(C, B);
+                process(C, Cv, B, Bv);
             }
+            }
+            if (bad) s_piv[g] = 1;
         }
         // ---- dense tail: the m x m tile of every instance, by all lanes ------------------
         // Tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m), both in LDS.
@@ -564,31 +659,107 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 
         // ---- backward solve (same pipelined walk as the factor phase) -----------------
         {
+            if constexpr (LEAN) {
             const lds_int* bK = lvK + 2 * a.nlev;
-            // level sizes of the load cursor (lK, lU) and of the process cursor (pK), uniform
             auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
-            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
-            int lK = lvk(0, 0), lU = lvk(0, 1);
-            bool more = true;                 // the lane has items left in the load cursor's level
+            int ll = 0, lk = 0, lbase = 0;
+            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
+            // every lane loads every chunk: lanes of a wave idle at a level read a valid item
+            // of that level they do not use, and past the last level item 0
             auto prefetch = [&](int4 (&q)[kSC]) {
-                const bool on = act && more && s < lU;
+                const bool in = ll < a.nblev;
+                const int sl = min(s, lU - 1);
 #pragma unroll
-                for (int u = 0; u < kSC; ++u)
-                    q[u] = (on && lk + u < lK) ? a.bs[(size_t)lbase + (size_t)(lk + u) * lU + s] : no_item();
+                for (int u = 0; u < kSC; ++u) q[u] = a.bs[in ? (size_t)lbase + (size_t)(lk + u) * lU + sl : (size_t)0];
                 lk += kSC;
                 if (lk >= lK) {
                     lbase += lK * lU;
                     ++ll;
                     lk = 0;
                     lK = lvk(ll, 0);
-                    lU = lvk(ll, 1);
+                    lU = lvk(ll, 1) & 0xffffff;
+                }
+            };
+            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;
+            const int wave_s0 = (threadIdx.x & ~63) / G;
+            FT acc = (FT)0;
+            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+                prefetch(nxt);
+                if (wave_s0 < pU) {  // waves with no lane in use at this level only keep the barrier
+#pragma unroll
+                    for (int u = 0; u < kSC; ++u) {
+                        const int4 q = cur[u];
+                        acc = fma(XB(q.x), XB(q.y), acc);
+                        const int e = q.z;
+                        if (e & kEnd) {
+                            const int lt = (e >> 26) & 7;
+                            const FT sum = team_sum<G>(acc, lt, pT);
+                            if ((s & ((1 << lt) - 1)) == 0) {
+                                const int off = e & ((1 << 25) - 1);
+                                XBset(off, (XB(off) - sum) * XB(q.w));
+                            }
+                            acc = (FT)0;
+                        }
+                    }
+                }
+                pk += kSC;
+                if (pk >= pK) {
+                    __syncthreads();
+                    ++pl;
+                    pk = 0;
+                    pK = lvk(pl, 0);
+                    const int w1 = lvk(pl, 1);
+                    pU = w1 & 0xffffff;
+                    pT = w1 >> 24;
+                }
+            };
+            int4 A[kSC], B[kSC], C[kSC];
+            prefetch(A);
+            prefetch(B);
+            while (pl < a.nblev) {
+                process(A, C);
+                if (pl >= a.nblev) break;
+                process(B, A);
+                if (pl >= a.nblev) break;
+                process(C, B);
+            }
+            } else {
+            const lds_int* bK = lvK + 2 * a.nlev;
+            // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
+            // log2 of its largest team), uniform
+            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
+            int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
+            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
+            bool more = true;                 // the lane has items left in the load cursor's level
+            // Every lane issues every prefetch load (a lane with nothing to load reads item 0
+            // as a stand-in) and keeps a validity bit per item, applied when the item is used:
+            // a load skipped by some lanes would make the compiler wait for all loads in flight.
+            auto prefetch = [&](int4 (&q)[kSC], int& vbits) {
+                const bool on = act && more && s < lU;
+                vbits = 0;
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) {
+                    const bool ok = on && lk + u < lK;
+                    q[u] = a.bs[ok ? (size_t)lbase + (size_t)(lk + u) * lU + s : (size_t)0];
+                    vbits |= (int)ok << u;
+                }
+                lk += kSC;
+                if (lk >= lK) {
+                    lbase += lK * lU;
+                    ++ll;
+                    lk = 0;
+                    lK = lvk(ll, 0);
+                    lU = lvk(ll, 1) & 0xffffff;
                     more = true;
                 }
             };
-            int pl = 0, pk = 0, pK = lK, pU = lU;  // process cursor
+            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;  // process cursor
             const int wave_s0 = (threadIdx.x & ~63) / G;  // first instance slot of this wave
             FT acc = (FT)0;
-            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
+            auto process = [&](const int4 (&cur_raw)[kSC], int cur_v, int4 (&nxt)[kSC], int& nxt_v) {
+                int4 cur[kSC];
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) cur[u] = (cur_v >> u) & 1 ? cur_raw[u] : no_item();
                 FT prod[kSC];
                 bool last = false;
                 // a wave none of whose lanes is used at this level only keeps the cursors and the barrier
@@ -604,27 +775,30 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 }
                 // the lane's last item: later chunks of the load cursor's level are skipped
                 if (last && pl == ll) more = false;
-                prefetch(nxt);
+                prefetch(nxt, nxt_v);
                 if (act && wave_on) {
 #pragma unroll
                     for (int u = 0; u < kSC; ++u) {
                         const int e = cur[u].z;
                         acc += prod[u];
-                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
-                            const int lt = (e >> 26) & 7;
-                            acc = team_sum<G>(acc, lt);
-                            if ((s & ((1 << lt) - 1)) == 0) {
-                                const int slot = e & ((1 << 26) - 1);
+                        // end of a task: the team's lanes reduce, the leader writes
+                        const bool end = e & kEnd;
+                        if (end) {
+                        const int lt = (e >> 26) & 7;
+                        const FT sum = team_sum<G>(acc, lt, pT);
+                        const bool lead = (s & ((1 << lt) - 1)) == 0;
+                            if (lead) {
+                                const int slot = e & ((1 << 25) - 1);
                                 const FT rd = XL(cur[u].w);
                                 if (slot < nlds) {
-                                    Xl[slot * G + g] = (Xl[slot * G + g] - acc) * rd;
+                                    Xl[slot * G + g] = (Xl[slot * G + g] - sum) * rd;
                                 } else {
                                     FT* xp = &Xgb[(size_t)(slot - nlds) * G + g];
-                                    *xp = (*xp - acc) * rd;
+                                    *xp = (*xp - sum) * rd;
                                 }
                             }
-                            acc = (FT)0;
                         }
+                        acc = end ? (FT)0 : acc;
                     }
                 }
                 pk += kSC;
@@ -633,18 +807,22 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     ++pl;
                     pk = 0;
                     pK = lvk(pl, 0);
-                    pU = lvk(pl, 1);
+                    const int w1 = lvk(pl, 1);
+                    pU = w1 & 0xffffff;
+                    pT = w1 >> 24;
                 }
             };
             int4 A[kSC], B[kSC], C[kSC];
-            prefetch(A);
-            prefetch(B);
+            int Av = 0, Bv = 0, Cv = 0;
+            prefetch(A, Av);
+            prefetch(B, Bv);
             while (pl < a.nblev) {
-                process(A, C);
+                process(A, Av, C, Cv);
                 if (pl >= a.nblev) break;
-                process(B, A);
+                process(B, Bv, A, Av);
                 if (pl >= a.nblev) break;
-                process(C, B);
+                process(C, Cv, B, Bv);
+            }
             }
         }
 
@@ -789,38 +967,43 @@ struct pfgpu_grid {
     struct Streams {
         DevBuf<int> lv_K, fs, bs;
     };
-    std::vector<std::unique_ptr<Streams>> streams;  // index 2*log2(G) + (block == 1024)
+    std::vector<std::unique_ptr<Streams>> streams;  // by (G, block size, value size, lean)
     // autotuned launch shape per precision (G, block), 0 = not tuned
     int tuned_G[2] = {0, 0}, tuned_NT[2] = {0, 0};
 };
 
 namespace {
 
-template <typename FT, int G, int NT>
+template <typename FT, int G, int NT, bool LEAN>
 void launch_nr(const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
-    auto kfn = k_nr<FT, G, NT>;
+    auto kfn = k_nr<FT, G, NT, LEAN>;
     if (lds > 65536) HIP_OK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(NT), lds, st, a);
     HIP_OK(hipGetLastError());
 }
 
-template <typename FT, int NT>
+template <typename FT, int NT, bool LEAN>
 void launch_nr_g(int G, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
     switch (G) {
-        case 1: launch_nr<FT, 1, NT>(a, nblocks, lds, st); break;
-        case 2: launch_nr<FT, 2, NT>(a, nblocks, lds, st); break;
-        case 4: launch_nr<FT, 4, NT>(a, nblocks, lds, st); break;
-        case 8: launch_nr<FT, 8, NT>(a, nblocks, lds, st); break;
-        case 16: launch_nr<FT, 16, NT>(a, nblocks, lds, st); break;
-        case 32: launch_nr<FT, 32, NT>(a, nblocks, lds, st); break;
+        case 1: launch_nr<FT, 1, NT, LEAN>(a, nblocks, lds, st); break;
+        case 2: launch_nr<FT, 2, NT, LEAN>(a, nblocks, lds, st); break;
+        case 4: launch_nr<FT, 4, NT, LEAN>(a, nblocks, lds, st); break;
+        case 8: launch_nr<FT, 8, NT, LEAN>(a, nblocks, lds, st); break;
+        case 16: launch_nr<FT, 16, NT, LEAN>(a, nblocks, lds, st); break;
+        case 32: launch_nr<FT, 32, NT, LEAN>(a, nblocks, lds, st); break;
         default: throw ApiError(PFGPU_E_INVALID, "instances_per_wg must be 1, 2, 4, 8, 16 or 32");
     }
 }
 
 template <typename FT>
-void launch_nr_any(int G, int NT, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
-    if (NT == 1024) launch_nr_g<FT, 1024>(G, a, nblocks, lds, st);
-    else launch_nr_g<FT, 256>(G, a, nblocks, lds, st);
+void launch_nr_any(int G, int NT, bool lean, const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
+    if (lean) {
+        if (NT == 1024) launch_nr_g<FT, 1024, true>(G, a, nblocks, lds, st);
+        else launch_nr_g<FT, 256, true>(G, a, nblocks, lds, st);
+    } else {
+        if (NT == 1024) launch_nr_g<FT, 1024, false>(G, a, nblocks, lds, st);
+        else launch_nr_g<FT, 256, false>(G, a, nblocks, lds, st);
+    }
 }
 
 int log2i(int g) {
@@ -831,24 +1014,26 @@ int log2i(int g) {
 
 struct LaunchChoice {
     int G, NT, nlds, nspill, nblocks;
+    bool lean;       // every value slot in LDS: the lean kernel variant and streams
     size_t lds;      // dynamic LDS bytes: value slots, then the level table
     size_t lvk_off;  // byte offset of the level table
 };
 
-template <typename FT, int NT>
+template <typename FT, int NT, bool LEAN>
 const void* nr_kernel_g(int G) {
     switch (G) {
-        case 1: return (const void*)k_nr<FT, 1, NT>;
-        case 2: return (const void*)k_nr<FT, 2, NT>;
-        case 4: return (const void*)k_nr<FT, 4, NT>;
-        case 8: return (const void*)k_nr<FT, 8, NT>;
-        case 16: return (const void*)k_nr<FT, 16, NT>;
-        case 32: return (const void*)k_nr<FT, 32, NT>;
+        case 1: return (const void*)k_nr<FT, 1, NT, LEAN>;
+        case 2: return (const void*)k_nr<FT, 2, NT, LEAN>;
+        case 4: return (const void*)k_nr<FT, 4, NT, LEAN>;
+        case 8: return (const void*)k_nr<FT, 8, NT, LEAN>;
+        case 16: return (const void*)k_nr<FT, 16, NT, LEAN>;
+        case 32: return (const void*)k_nr<FT, 32, NT, LEAN>;
         default: throw ApiError(PFGPU_E_INVALID, "instances_per_wg must be 1, 2, 4, 8, 16 or 32");
     }
 }
 
-// Static LDS of one k_nr variant (per-instance scalars and the reduction buffer).
+// Static LDS of one k_nr variant (per-instance scalars and the reduction buffer);
+// the larger of the lean and the general variant.
 size_t static_lds(int fsize, int G, int NT) {
     static std::mutex mu;
     static std::map<int, size_t> cache;
@@ -856,12 +1041,21 @@ size_t static_lds(int fsize, int G, int NT) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    const void* fn = fsize == 4 ? (NT == 1024 ? nr_kernel_g<float, 1024>(G) : nr_kernel_g<float, 256>(G))
-                                : (NT == 1024 ? nr_kernel_g<double, 1024>(G) : nr_kernel_g<double, 256>(G));
-    hipFuncAttributes attr{};
-    HIP_OK(hipFuncGetAttributes(&attr, fn));
-    cache[key] = attr.sharedSizeBytes;
-    return attr.sharedSizeBytes;
+    size_t mx = 0;
+    for (bool lean : {false, true}) {
+        const void* fn;
+        if (fsize == 4)
+            fn = NT == 1024 ? (lean ? nr_kernel_g<float, 1024, true>(G) : nr_kernel_g<float, 1024, false>(G))
+                            : (lean ? nr_kernel_g<float, 256, true>(G) : nr_kernel_g<float, 256, false>(G));
+        else
+            fn = NT == 1024 ? (lean ? nr_kernel_g<double, 1024, true>(G) : nr_kernel_g<double, 1024, false>(G))
+                            : (lean ? nr_kernel_g<double, 256, true>(G) : nr_kernel_g<double, 256, false>(G));
+        hipFuncAttributes attr{};
+        HIP_OK(hipFuncGetAttributes(&attr, fn));
+        mx = std::max(mx, (size_t)attr.sharedSizeBytes);
+    }
+    cache[key] = mx;
+    return mx;
 }
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -906,24 +1100,39 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     if (c.nlds < std::max(1, P.m * P.m + P.m))
         throw ApiError(PFGPU_E_INVALID, "the grid's dense tail does not fit in LDS");
     c.nspill = P.nvals - c.nlds;
+    c.lean = c.nspill == 0;
     c.nblocks = (ninst + c.G - 1) / c.G;
     c.lvk_off = align16((size_t)c.nlds * c.G * fsize);
     c.lds = c.lvk_off + table;
     return c;
 }
 
-int shape_index(int G, int NT) { return 2 * log2i(G) + (NT == 1024 ? 1 : 0); }
+int shape_index(int G, int NT, int fsize, bool lean) {
+    return ((2 * log2i(G) + (NT == 1024 ? 1 : 0)) * 2 + (fsize == 8 ? 1 : 0)) * 2 + (lean ? 1 : 0);
+}
 
-void ensure_streams(pfgpu_grid* gr, int G, int NT) {
-    const int ix = shape_index(G, NT);
+void ensure_streams(pfgpu_grid* gr, int G, int NT, int fsize, bool lean) {
+    const int ix = shape_index(G, NT, fsize, lean);
     if ((int)gr->streams.size() <= ix) gr->streams.resize(ix + 1);
     if (gr->streams[ix]) return;
-    const pfgpu::LaneStreams L = pfgpu::build_lane_streams(gr->P, G, NT / G);
+    pfgpu::LeanFormat fmt;
+    fmt.elem_bytes = G * fsize;
+    fmt.lane_round = std::max(1, 64 / G);
+    fmt.k_round = kSC;
+    const pfgpu::LaneStreams L = pfgpu::build_lane_streams(gr->P, G, NT / G, lean ? &fmt : nullptr);
     auto t = std::make_unique<pfgpu_grid::Streams>();
     auto up = [](DevBuf<int>& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>(4, -1) : v); };
     std::vector<int> K;
-    for (size_t l = 0; l < L.f_lv.size() / 4; ++l) { K.push_back(L.f_lv[4 * l + 1]); K.push_back(L.f_lv[4 * l + 3]); }
-    for (size_t l = 0; l < L.b_lv.size() / 4; ++l) { K.push_back(L.b_lv[4 * l + 1]); K.push_back(L.b_lv[4 * l + 3]); }
+    // per level {K, U | log2(largest team) << 24}
+    auto lg = [](int t) { int l = 0; while ((1 << l) < t) ++l; return l; };
+    for (size_t l = 0; l < L.f_lv.size() / 4; ++l) {
+        K.push_back(L.f_lv[4 * l + 1]);
+        K.push_back(L.f_lv[4 * l + 3] | (lg(L.f_lv[4 * l + 2]) << 24));
+    }
+    for (size_t l = 0; l < L.b_lv.size() / 4; ++l) {
+        K.push_back(L.b_lv[4 * l + 1]);
+        K.push_back(L.b_lv[4 * l + 3] | (lg(L.b_lv[4 * l + 2]) << 24));
+    }
     up(t->lv_K, K);
     up(t->fs, L.fs);
     up(t->bs, L.bs);
@@ -963,7 +1172,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
         NT_req = gr->tuned_NT[precision];
     }
     LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req, NT_req);
-    ensure_streams(gr, lc.G, lc.NT);
+    ensure_streams(gr, lc.G, lc.NT, fsize, lc.lean);
     const bool deltas = d_dptr != nullptr;
     size_t off = 0;
     const size_t o_y = off;
@@ -1003,7 +1212,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
     a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
-    const auto& ls = *gr->streams[shape_index(lc.G, lc.NT)];
+    const auto& ls = *gr->streams[shape_index(lc.G, lc.NT, fsize, lc.lean)];
     a.lv_K = ls.lv_K.p; a.lvk_off = (int)lc.lvk_off;
     a.fs = reinterpret_cast<const int4*>(ls.fs.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
     a.w = gr->w.p;
@@ -1013,8 +1222,8 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
     a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
     HIP_OK(hipEventRecord(ctx->ev0, stream));
-    if (precision == PFGPU_FP32) launch_nr_any<float>(lc.G, lc.NT, a, lc.nblocks, lc.lds, stream);
-    else launch_nr_any<double>(lc.G, lc.NT, a, lc.nblocks, lc.lds, stream);
+    if (precision == PFGPU_FP32) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
+    else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
     HIP_OK(hipEventRecord(ctx->ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     if (st) {

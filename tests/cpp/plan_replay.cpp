@@ -44,31 +44,32 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     for (int b = 0; b < n; ++b) slack_abs += Sc[b].real();
     auto rnd = [&](double v) { return use_fp32 ? (double)(float)v : v; };
 
-    // schedule conflict check: no slot written twice by the factor tasks, every slot a task
-    // reads is written at an earlier level (or by the J fill / mismatch before the levels)
+    // schedule conflict check: within one level no slot is written twice and no slot a
+    // task reads is written (slots are recycled across levels; the analysis checks that
+    // values sharing a slot have disjoint lifetimes)
     {
-        std::vector<int> wlev(P.nvals, -1);
         int conflicts = 0;
-        for (int l = 0; l < P.nlev; ++l)
+        for (int l = 0; l < P.nlev; ++l) {
+            std::set<int> w, r;
             for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t) {
-                if (wlev[P.tasks[4 * t]] >= 0) ++conflicts;
-                wlev[P.tasks[4 * t]] = l;
-            }
-        for (int l = 0; l < P.nlev; ++l)
-            for (int t = P.f_lvl[l]; t < P.f_lvl[l + 1]; ++t)
+                if (!w.insert(P.tasks[4 * t]).second) ++conflicts;
                 for (int p = P.tasks[4 * t + 1]; p < P.tasks[4 * t + 2]; ++p)
-                    for (int k = 0; k < 3; ++k)
-                        if (wlev[P.pairs[4 * p + k]] >= l) ++conflicts;
+                    for (int k = 0; k < 3; ++k) r.insert(P.pairs[4 * p + k]);
+            }
+            for (int x : w)
+                if (r.count(x)) ++conflicts;
+        }
         // lane streams: all lanes of a team (aligned group of T lanes, T from the end
         // word) end each task at the same item with the same end word, every task is
         // ended by T lanes, the items cover each task's pairs exactly once, and every
         // used lane has one "last item" flag after which it holds nothing
         auto check_streams = [&](const std::vector<int>& lv, const std::vector<int>& st, int nl, bool fwd,
                                  const std::vector<int>& tasks, const std::vector<int>& lvl) {
-            std::vector<long> npair_seen(P.nvals, 0), ends(P.nvals, 0), pair_sum(P.nvals, 0), team_of(P.nvals, 0),
-                recip_of(P.nvals, 0);
             const int ecol = fwd ? 3 : 2;
             for (int l = 0; l < nl; ++l) {
+                // per level: slots are recycled across levels, not within one
+                std::vector<long> npair_seen(P.nvals, 0), ends(P.nvals, 0), pair_sum(P.nvals, 0), team_of(P.nvals, 0),
+                    recip_of(P.nvals, 0);
                 const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
                 if (U < 1 || U > nslot || K < 1) ++conflicts;
                 auto item = [&](int k, int s) { return &st[4 * ((size_t)base + (size_t)k * U + s)]; };
@@ -116,6 +117,48 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
         };
         check_streams(LS.f_lv, LS.fs, P.nlev, true, P.tasks, P.f_lvl);
         check_streams(LS.b_lv, LS.bs, P.nblev, false, P.btasks, P.b_lvl);
+        // the lean encoding (LDS byte offsets, zero-slot padding, lanes rounded to a wave)
+        // must hold exactly the items of the checked one
+        for (int esz : {4, 8}) {
+            LeanFormat fmt;
+            fmt.elem_bytes = G * esz;
+            fmt.lane_round = std::max(1, 64 / G);
+            fmt.k_round = 2;
+            const LaneStreams LL = build_lane_streams(P, G, nslot, &fmt);
+            const int sc = fmt.elem_bytes, zb = P.zslot * sc;
+            auto cmp = [&](const std::vector<int>& lv, const std::vector<int>& st, const std::vector<int>& llv,
+                           const std::vector<int>& lst, int nl, bool fwd) {
+                const int ecol = fwd ? 3 : 2;
+                for (int l = 0; l < nl; ++l) {
+                    const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
+                    const int lb = llv[4 * l], lK = llv[4 * l + 1], lU = llv[4 * l + 3];
+                    const int want = std::min(nslot, (U + fmt.lane_round - 1) / fmt.lane_round * fmt.lane_round);
+                    if (lK != (K + 1) / 2 * 2 || lU != want || llv[4 * l + 2] != lv[4 * l + 2]) { ++conflicts; continue; }
+                    for (int s = 0; s < lU; ++s) {
+                        bool after_last = s >= U;
+                        for (int k = 0; k < lK; ++k) {
+                            if (k >= K) after_last = true;
+                            const int* e = &lst[4 * ((size_t)lb + (size_t)k * lU + s)];
+                            if (after_last) {
+                                if (e[0] != zb || e[1] != zb || e[ecol] != 0) ++conflicts;
+                                continue;
+                            }
+                            const int* g = &st[4 * ((size_t)base + (size_t)k * U + s)];
+                            const int ge = g[ecol] & ~kStreamLast;
+                            const int np = fwd ? 3 : 2;
+                            for (int w = 0; w < np; ++w)
+                                if (e[w] != (g[0] >= 0 ? g[w] * sc : zb)) ++conflicts;
+                            const int want_e = (ge & kStreamEnd) ? (((ge & kStreamSlotMask) * sc) | (ge & ~kStreamSlotMask)) : 0;
+                            if (e[ecol] != want_e) ++conflicts;
+                            if (!fwd && (ge & kStreamEnd) && e[3] != g[3] * sc) ++conflicts;
+                            if (g[ecol] & kStreamLast) after_last = true;
+                        }
+                    }
+                }
+            };
+            cmp(LS.f_lv, LS.fs, LL.f_lv, LL.fs, P.nlev, true);
+            cmp(LS.b_lv, LS.bs, LL.b_lv, LL.bs, P.nblev, false);
+        }
         *conflicts_out = conflicts;
     }
 
@@ -188,7 +231,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                             for (int tp = 0; tp < T; ++tp) { sum = rnd(sum + acc[s + tp]); acc[s + tp] = 0; }
                             const int o = e & kStreamSlotMask;
                             if (fwd) {
-                                const double v = rnd(X[o] - sum);
+                                const double v = rnd(((e & kStreamAssign) ? 0.0 : X[o]) - sum);
                                 X[o] = (e & kStreamRecip) ? rnd(1.0 / v) : v;
                             } else {
                                 X[o] = rnd(rnd(X[o] - sum) * X[it[3]]);
