@@ -818,8 +818,7 @@ void layout_streams(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs np
                     std::vector<int>& stream) {
     const int nlev = (int)lvl_ptr.size() - 1;
     lv.assign(4 * (size_t)std::max(nlev, 0), 0);
-    stream.clear();
-    size_t base = 0;
+    size_t base = stream.size() / 4;  // items already in the stream (a leading padding item)
     for (int l = 0; l < nlev; ++l) {
         const int t0 = lvl_ptr[l], nt = lvl_ptr[l + 1] - t0;
         std::vector<int> np(nt);
@@ -879,6 +878,11 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
     L.nslot = nslot;
     const int fpad[4] = {lean ? zb : -1, lean ? zb : -1, lean ? zb : -1, 0};
     const int bpad[4] = {lean ? zb : -1, lean ? zb : -1, 0, lean ? zb : -1};
+    // lean streams start with one padding item (item 0), read in place of items past a level's end
+    if (lean) {
+        L.fs.assign(fpad, fpad + 4);
+        L.bs.assign(bpad, bpad + 4);
+    }
     layout_streams(
         P.f_lvl, G, nslot, [&](int t) { return P.tasks[4 * t + 2] - P.tasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {

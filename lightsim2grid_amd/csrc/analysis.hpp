@@ -179,7 +179,8 @@ struct LaneStreams {
 // offsets (slot * elem_bytes) instead of slot ids, items without a pair point at the
 // zero slot (no per-item checks), the lanes of a level are rounded up to lane_round
 // (a whole wave), the items per lane to k_round, and no last-item flags are set (every
-// lane of a used wave walks all K items; the padding items add zeros).
+// lane of a used wave walks all K items; the padding items add zeros). Item 0 of a lean
+// stream is a padding item, read in place of items past a level's end.
 struct LeanFormat {
     int elem_bytes = 4;  // bytes between consecutive slots of one instance (G * value size)
     int lane_round = 1;  // lanes per wave of one instance (64 / G)

@@ -201,6 +201,7 @@ __device__ __noinline__ double rect_of(double* __restrict__ V, const double* __r
 // log2(team), 29 reciprocal, 30 end of task, 31 the lane's last item of the level
 // (the lane loads nothing after it).
 constexpr int kSC = 2;  // items per chunk
+constexpr int kHB = 4;  // items of a level loaded ahead, during the level before it (lean walk)
 constexpr int kEnd = 1 << 30;
 __device__ __forceinline__ int4 no_item() { return int4{-1, -1, 0, 0}; }
 
@@ -422,72 +423,85 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         {
             bool bad = false;  // a zero or non-finite pivot met by this lane
             if constexpr (LEAN) {
+            // The first kHB items of a level are loaded during the level before it (two
+            // register buffers, by level parity); the rest of a longer level is loaded in the
+            // level. Every lane issues every load: items past a level's end read the padding
+            // item 0, lanes of a wave idle at a level read a valid item of it they do not use.
             auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
-            int ll = 0, lk = 0, lbase = 0;
-            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
-            // every lane loads every chunk: lanes of a wave idle at a level read a valid item
-            // of that level they do not use, and past the last level item 0
-            auto prefetch = [&](int4 (&q)[kSC]) {
-                const bool in = ll < a.nlev;
-                const int sl = min(s, lU - 1);
+            auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
+                const int sl = min(s, U - 1);
 #pragma unroll
-                for (int u = 0; u < kSC; ++u) q[u] = a.fs[in ? (size_t)lbase + (size_t)(lk + u) * lU + sl : (size_t)0];
-                lk += kSC;
-                if (lk >= lK) {
-                    lbase += lK * lU;
-                    ++ll;
-                    lk = 0;
-                    lK = lvk(ll, 0);
-                    lU = lvk(ll, 1) & 0xffffff;
+                for (int u = 0; u < kHB; ++u) q[u] = a.fs[u < K ? (size_t)base + (size_t)u * U + sl : (size_t)0];
+            };
+            FT acc = (FT)0;
+            const int wave_s0 = (threadIdx.x & ~63) / G;
+            // Within a level no item reads a slot written in it, so the operands of all of a
+            // lane's items (and the old content of their output slots) are read first.
+            struct Ops {
+                FT p, old;
+            };
+            auto ops = [&](const int4 q) -> Ops { return {XB(q.x) * XB(q.y) * XB(q.z), XB(q.w & ((1 << 25) - 1))}; };
+            auto item = [&](const int4 q, const Ops o, int T) {
+                acc += o.p;
+                const int e = q.w;
+                if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
+                    const int lt = (e >> 26) & 7;
+                    const FT sum = team_sum<G>(acc, lt, T);
+                    if ((s & ((1 << lt) - 1)) == 0) {
+                        const int off = e & ((1 << 25) - 1);
+                        FT v = (((e >> 25) & 1) ? (FT)0 : o.old) - sum;  // fill-in: assign
+                        if ((e >> 29) & 1) {
+                            bad |= bad_pivot(v);
+                            v = pivot_recip(v);
+                        }
+                        XBset(off, v);
+                    }
+                    acc = (FT)0;
                 }
             };
-            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;
-            const int wave_s0 = (threadIdx.x & ~63) / G;
-            FT acc = (FT)0;
-            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
-                prefetch(nxt);
-                if (wave_s0 < pU) {  // waves with no lane in use at this level only keep the barrier
+            auto level = [&](int base, int K, int U, int T, const int4 (&h)[kHB]) {
+                if (wave_s0 < U) {  // waves with no lane in use at this level only keep the barrier
+                    Ops o[kHB];
 #pragma unroll
-                    for (int u = 0; u < kSC; ++u) {
-                        const int4 q = cur[u];
-                        acc = fma(XB(q.x) * XB(q.y), XB(q.z), acc);
-                        const int e = q.w;
-                        if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
-                            const int lt = (e >> 26) & 7;
-                            const FT sum = team_sum<G>(acc, lt, pT);
-                            if ((s & ((1 << lt) - 1)) == 0) {
-                                const int off = e & ((1 << 25) - 1);
-                                FT v = (((e >> 25) & 1) ? (FT)0 : XB(off)) - sum;  // fill-in: assign
-                                if ((e >> 29) & 1) {
-                                    bad |= bad_pivot(v);
-                                    v = pivot_recip(v);
-                                }
-                                XBset(off, v);
-                            }
-                            acc = (FT)0;
-                        }
+                    for (int u = 0; u < kHB; ++u) o[u] = ops(h[u]);
+#pragma unroll
+                    for (int u = 0; u < kHB; ++u)
+                        if (u < K) item(h[u], o[u], T);
+                    const int sl = min(s, U - 1);
+                    for (int k = kHB; k < K; k += 2) {
+                        const int4 r0 = a.fs[(size_t)base + (size_t)k * U + sl];
+                        const int4 r1 = a.fs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
+                        const Ops o0 = ops(r0), o1 = ops(r1);
+                        item(r0, o0, T);
+                        item(r1, o1, T);
                     }
                 }
-                pk += kSC;
-                if (pk >= pK) {
-                    __syncthreads();
-                    ++pl;
-                    pk = 0;
-                    pK = lvk(pl, 0);
-                    const int w1 = lvk(pl, 1);
-                    pU = w1 & 0xffffff;
-                    pT = w1 >> 24;
-                }
+                __syncthreads();
             };
-            int4 A[kSC], B[kSC], C[kSC];
-            prefetch(A);
-            prefetch(B);
-            while (pl < a.nlev) {
-                process(A, C);
-                if (pl >= a.nlev) break;
-                process(B, A);
-                if (pl >= a.nlev) break;
-                process(C, B);
+            int4 H0[kHB], H1[kHB];
+            int base = 1;  // level 0 follows the leading padding item
+            int K = lvk(0, 0), w = lvk(0, 1);
+            load_head(base, K, w & 0xffffff, H0);
+            for (int l = 0; l < a.nlev; l += 2) {
+                int U = w & 0xffffff, T = w >> 24;
+                int nb = base + K * U;
+                int Kn = lvk(l + 1, 0), wn = lvk(l + 1, 1);
+                load_head(nb, Kn, wn & 0xffffff, H1);
+                level(base, K, U, T, H0);
+                if (l + 1 >= a.nlev) break;
+                base = nb;
+                K = Kn;
+                w = wn;
+                U = w & 0xffffff;
+                T = w >> 24;
+                nb = base + K * U;
+                Kn = lvk(l + 2, 0);
+                wn = lvk(l + 2, 1);
+                load_head(nb, Kn, wn & 0xffffff, H0);
+                level(base, K, U, T, H1);
+                base = nb;
+                K = Kn;
+                w = wn;
             }
             } else {
             // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
@@ -661,67 +675,77 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         {
             if constexpr (LEAN) {
             const lds_int* bK = lvK + 2 * a.nlev;
+            // The first kHB items of a level are loaded during the level before it (two
+            // register buffers, by level parity); the rest of a longer level is loaded in the
+            // level. Every lane issues every load: items past a level's end read the padding
+            // item 0, lanes of a wave idle at a level read a valid item of it they do not use.
             auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
-            int ll = 0, lk = 0, lbase = 0;
-            int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
-            // every lane loads every chunk: lanes of a wave idle at a level read a valid item
-            // of that level they do not use, and past the last level item 0
-            auto prefetch = [&](int4 (&q)[kSC]) {
-                const bool in = ll < a.nblev;
-                const int sl = min(s, lU - 1);
+            auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
+                const int sl = min(s, U - 1);
 #pragma unroll
-                for (int u = 0; u < kSC; ++u) q[u] = a.bs[in ? (size_t)lbase + (size_t)(lk + u) * lU + sl : (size_t)0];
-                lk += kSC;
-                if (lk >= lK) {
-                    lbase += lK * lU;
-                    ++ll;
-                    lk = 0;
-                    lK = lvk(ll, 0);
-                    lU = lvk(ll, 1) & 0xffffff;
+                for (int u = 0; u < kHB; ++u) q[u] = a.bs[u < K ? (size_t)base + (size_t)u * U + sl : (size_t)0];
+            };
+            FT acc = (FT)0;
+            const int wave_s0 = (threadIdx.x & ~63) / G;
+            struct Ops {
+                FT p, old, rd;
+            };
+            auto ops = [&](const int4 q) -> Ops {
+                return {XB(q.x) * XB(q.y), XB(q.z & ((1 << 25) - 1)), XB(q.w)};
+            };
+            auto item = [&](const int4 q, const Ops o, int T) {
+                acc += o.p;
+                const int e = q.z;
+                if (e & kEnd) {
+                    const int lt = (e >> 26) & 7;
+                    const FT sum = team_sum<G>(acc, lt, T);
+                    if ((s & ((1 << lt) - 1)) == 0) XBset(e & ((1 << 25) - 1), (o.old - sum) * o.rd);
+                    acc = (FT)0;
                 }
             };
-            int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;
-            const int wave_s0 = (threadIdx.x & ~63) / G;
-            FT acc = (FT)0;
-            auto process = [&](const int4 (&cur)[kSC], int4 (&nxt)[kSC]) {
-                prefetch(nxt);
-                if (wave_s0 < pU) {  // waves with no lane in use at this level only keep the barrier
+            auto level = [&](int base, int K, int U, int T, const int4 (&h)[kHB]) {
+                if (wave_s0 < U) {  // waves with no lane in use at this level only keep the barrier
+                    Ops o[kHB];
 #pragma unroll
-                    for (int u = 0; u < kSC; ++u) {
-                        const int4 q = cur[u];
-                        acc = fma(XB(q.x), XB(q.y), acc);
-                        const int e = q.z;
-                        if (e & kEnd) {
-                            const int lt = (e >> 26) & 7;
-                            const FT sum = team_sum<G>(acc, lt, pT);
-                            if ((s & ((1 << lt) - 1)) == 0) {
-                                const int off = e & ((1 << 25) - 1);
-                                XBset(off, (XB(off) - sum) * XB(q.w));
-                            }
-                            acc = (FT)0;
-                        }
+                    for (int u = 0; u < kHB; ++u) o[u] = ops(h[u]);
+#pragma unroll
+                    for (int u = 0; u < kHB; ++u)
+                        if (u < K) item(h[u], o[u], T);
+                    const int sl = min(s, U - 1);
+                    for (int k = kHB; k < K; k += 2) {
+                        const int4 r0 = a.bs[(size_t)base + (size_t)k * U + sl];
+                        const int4 r1 = a.bs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
+                        const Ops o0 = ops(r0), o1 = ops(r1);
+                        item(r0, o0, T);
+                        item(r1, o1, T);
                     }
                 }
-                pk += kSC;
-                if (pk >= pK) {
-                    __syncthreads();
-                    ++pl;
-                    pk = 0;
-                    pK = lvk(pl, 0);
-                    const int w1 = lvk(pl, 1);
-                    pU = w1 & 0xffffff;
-                    pT = w1 >> 24;
-                }
+                __syncthreads();
             };
-            int4 A[kSC], B[kSC], C[kSC];
-            prefetch(A);
-            prefetch(B);
-            while (pl < a.nblev) {
-                process(A, C);
-                if (pl >= a.nblev) break;
-                process(B, A);
-                if (pl >= a.nblev) break;
-                process(C, B);
+            int4 H0[kHB], H1[kHB];
+            int base = 1;  // level 0 follows the leading padding item
+            int K = lvk(0, 0), w = lvk(0, 1);
+            load_head(base, K, w & 0xffffff, H0);
+            for (int l = 0; l < a.nblev; l += 2) {
+                int U = w & 0xffffff, T = w >> 24;
+                int nb = base + K * U;
+                int Kn = lvk(l + 1, 0), wn = lvk(l + 1, 1);
+                load_head(nb, Kn, wn & 0xffffff, H1);
+                level(base, K, U, T, H0);
+                if (l + 1 >= a.nblev) break;
+                base = nb;
+                K = Kn;
+                w = wn;
+                U = w & 0xffffff;
+                T = w >> 24;
+                nb = base + K * U;
+                Kn = lvk(l + 2, 0);
+                wn = lvk(l + 2, 1);
+                load_head(nb, Kn, wn & 0xffffff, H0);
+                level(base, K, U, T, H1);
+                base = nb;
+                K = Kn;
+                w = wn;
             }
             } else {
             const lds_int* bK = lvK + 2 * a.nlev;
