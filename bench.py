@@ -25,6 +25,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_nr_case2000_n1_hbm_traffic.md")
+
+
+def measured_traffic(form: str):
+    """HBM bytes per sweep launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, each scaled by
+    the same counter's reading for a copy kernel of known size), from the committed profile of
+    this workload; None when the profile is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {form}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def algorithmic_bytes(info, iters, w):
     """Bytes per launch: sum over solves of it * [w (nnzJ + 3 nnzLU) + 16*4n + 8*2N]."""
     n, N = info["n_bus"], info["n_jac"]
@@ -322,8 +339,11 @@ def main():
                    "n_bus": g.n, "n_branch": g.nbr, "n_jac": info["n_jac"], "nnz_jac": info["nnz_jac"],
                    "nnz_lu": info["nnz_lu"], "instances_per_step": ninst},
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_nr<float,G>", "kernel_ms": r32["kernel_ms"]},
+                     "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("fp32") if args.grid == "case2000_synth" else None,
+                     "traffic_source": "profiles/k_nr_case2000_n1_hbm_traffic.md",
+                     "bytes_per_launch": r32["bytes_per_launch"],
+                     "kernel": "k_nr<float,G,block,lean>", "kernel_ms": r32["kernel_ms"]},
         "cpu_baseline": cpu,
         "fp64_form": {"value": r64["solves_per_s"], "ms_per_step": r64["ms_per_step"], "kernel_ms": r64["kernel_ms"],
                       "achieved_gbs": r64["achieved_gbs"], "iters_mean": r64["iters_mean"],
