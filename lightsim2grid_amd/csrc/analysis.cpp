@@ -368,6 +368,7 @@ std::string describe(const JPattern& J, const SymbolicLU& S, const Schedule& sch
 namespace pfgpu {
 
 void recycle_slots(DevicePlan& P);
+void build_jfill_table(DevicePlan& P);
 
 DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const SymbolicLU& S, const Schedule& sch) {
     DevicePlan P;
@@ -540,7 +541,37 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
             P.b_maxp[l] = std::max(P.b_maxp[l], S.cp[j + 1] - S.cp[j]);
         }
     recycle_slots(P);
+    build_jfill_table(P);
     return P;
+}
+
+// Groups the J entries by the Ybus entry they are computed from (after the slots are final).
+void build_jfill_table(DevicePlan& P) {
+    std::vector<int> ent(P.nnzY, -1);
+    for (int q = 0; q < P.nJ; ++q) {
+        const int sel = P.j_sel[q];
+        if (sel == SEL_CONST) {
+            P.jc_slot.push_back(P.j_slot[q]);
+            P.jc_jidx.push_back(q);
+            P.jc_val.push_back(P.j_cval[q]);
+            continue;
+        }
+        const int yp = P.j_ypos[q];
+        if (ent[yp] < 0) {
+            ent[yp] = (int)P.ye_pos.size();
+            P.ye_pos.push_back(yp);
+            P.ye_rc.push_back(P.j_rbus[q]);
+            P.ye_rc.push_back(P.j_cbus[q]);
+            P.ye_slot.insert(P.ye_slot.end(), {-1, -1, -1, -1});
+            P.ye_jidx.insert(P.ye_jidx.end(), {-1, -1, -1, -1});
+        }
+        const int e = ent[yp];
+        if (P.ye_rc[2 * e] != P.j_rbus[q] || P.ye_rc[2 * e + 1] != P.j_cbus[q])
+            throw std::runtime_error("pfgpu: J entries of one Ybus entry disagree on its buses");
+        if (P.ye_slot[4 * e + sel] != -1) throw std::runtime_error("pfgpu: two J entries of one Ybus entry and selector");
+        P.ye_slot[4 * e + sel] = P.j_slot[q];
+        P.ye_jidx[4 * e + sel] = q;
+    }
 }
 
 // ---------------------------------------------------------------------------

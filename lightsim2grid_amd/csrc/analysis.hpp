@@ -116,6 +116,14 @@ struct DevicePlan {
     std::vector<int> j_ypos, j_rbus, j_cbus, j_slot;
     std::vector<uint8_t> j_sel;
     std::vector<double> j_cval;
+    // J fill by Ybus entry: each listed Ybus entry (CSC position ye_pos, row bus / column
+    // bus in ye_rc) feeds up to 4 J entries: Re and Im of dS/dVa, Re and Im of dS/dVm.
+    // ye_slot holds their value slots (-1: absent; bit 30: store the reciprocal) and
+    // ye_jidx their J entry indices. The constant J entries (slack-weight column) are
+    // listed apart in jc_slot / jc_val.
+    std::vector<int> ye_pos, ye_rc, ye_slot, ye_jidx;
+    std::vector<int> jc_slot, jc_jidx;
+    std::vector<double> jc_val;
     // dense-tile slots that hold no J entry (fill-in): zeroed by the J-fill phase
     std::vector<int> zero_slots;
     // factor + forward tasks (level-major), 4 ints each: {out slot, pair begin, pair end, kind};

@@ -40,6 +40,7 @@
 namespace {
 
 constexpr int kMaxBlock = 1024;
+constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
 constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
 
@@ -83,6 +84,21 @@ struct KArgs {
     int n, N, nJ, nzero, nlev, nblev, nlds, nspill, single_slack, ninst, max_iter, m;
     double tol;
     const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
+    const int2* yr_pc;     // per Ybus row entry: {CSC position, column bus}
+    // J fill by Ybus entry: position, {row bus, column bus}, 4 target slots, 4 J indices
+    const int* ye_pos;
+    const int2* ye_rc;
+    const int4* ye_slot;
+    const int4* ye_jidx;
+    int nye;
+    const int *jc_slot, *jc_jidx;  // constant J entries
+    const double* jc_val;
+    int njc;
+    // per-instance Ybus edits applied on the fly: Y_inst = Y - dval at dpos, entries
+    // [dptr[i], dptr[i+1]) of instance i (dptr == nullptr: no edits)
+    const int *dptr, *dpos;
+    const double* dval;
+    double* Vn;          // [ninst][n] complex: V / |V|, refreshed before each J fill
     const int8_t* ux_kind;
     const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
@@ -305,6 +321,42 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     double* Va = a.Va + ii * (long long)n;
     double* Vm = a.Vm + ii * (long long)n;
     double* Ib = a.Ib + ii * 2 * (long long)n;
+    double* Vn = a.Vn + ii * 2 * (long long)n;
+    // this instance's Ybus edits: the first kMaxEdits in LDS, any further ones read from memory
+    __shared__ int s_dpos[G][kMaxEdits], s_nd[G], s_d0[G];
+    __shared__ double s_dval[G][kMaxEdits][2];
+    if (s == 0) {
+        s_nd[g] = 0;
+        s_d0[g] = 0;
+        if (a.dptr != nullptr && valid) {
+            s_d0[g] = a.dptr[ii];
+            s_nd[g] = a.dptr[ii + 1] - a.dptr[ii];
+        }
+    }
+    __syncthreads();
+    if (s < kMaxEdits && s < s_nd[g]) {
+        const int k = s_d0[g] + s;
+        s_dpos[g][s] = a.dpos[k];
+        s_dval[g][s][0] = a.dval[2 * k];
+        s_dval[g][s][1] = a.dval[2 * k + 1];
+    }
+    const int nd = s_nd[g], d0 = s_d0[g];
+    // Ybus value at CSC position pos for this instance
+    auto Yat = [&](int pos) -> c64 {
+        c64 y = ld_c(Y, pos);
+        const int nl = min(nd, kMaxEdits);
+        for (int d = 0; d < nl; ++d)
+            if (s_dpos[g][d] == pos) {
+                y.re -= s_dval[g][d][0];
+                y.im -= s_dval[g][d][1];
+            }
+        for (int d = kMaxEdits; d < nd; ++d)
+            if (a.dpos[d0 + d] == pos) {
+                y.re -= a.dval[2 * (d0 + d)];
+                y.im -= a.dval[2 * (d0 + d) + 1];
+            }
+        return y;
+    };
 
     // ---- init: Va/Vm from V, slack_absorbed = Re(sum Sbus) (BaseNRAlgo.tpp:60,77-79)
     double ssum = 0.0;
@@ -338,8 +390,22 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             const double sa = s_slack[g];
             for (int r = s; r < n; r += nslot) {
                 c64 acc = {0.0, 0.0};
-                for (int q = a.yr_ptr[r]; q < a.yr_ptr[r + 1]; ++q)
-                    acc = cadd(acc, cmul(ld_c(Y, a.yr_pos[q]), ld_c(V, a.yr_col[q])));
+                const int q1 = a.yr_ptr[r + 1];
+                // 4 entries' loads at a time, summed in row order
+                for (int q = a.yr_ptr[r]; q < q1; q += 4) {
+                    int2 pc[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) pc[u] = a.yr_pc[q + u < q1 ? q + u : q];
+                    c64 yv[4], vv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        yv[u] = Yat(pc[u].x);
+                        vv[u] = ld_c(V, pc[u].y);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (q + u < q1) acc = cadd(acc, cmul(yv[u], vv[u]));
+                }
                 Ib[2 * r] = acc.re;
                 Ib[2 * r + 1] = acc.im;
                 c64 mis = csub(cmul(ld_c(V, r), cconj(acc)), ld_c(S, r));
@@ -370,47 +436,50 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         bool act = valid && !s_done[g];
         if (s == 0 && act) s_it[g] += 1;
 
-        // ---- J fill: zero the fill-in slots, write every J entry --------------------
-        // (kJU entries per lane per step: their loads are issued together; a deeper
-        // unroll costs more registers than it saves in latency)
-        constexpr int kJU = 2;
+        // ---- J fill: zero the tile fill-in slots, write every J entry ------------------
+        // (dS_dVa / dS_dVm, BaseNRAlgo.tpp:202-260, per Ybus entry: one set of loads feeds the
+        // up to 4 J entries built on it; V / |V| per bus first)
+        if (act)
+            for (int b = s; b < n; b += nslot) {
+                const double vr = V[2 * b], vi = V[2 * b + 1];
+                const double ic = 1.0 / sqrt(vr * vr + vi * vi);
+                Vn[2 * b] = vr * ic;
+                Vn[2 * b + 1] = vi * ic;
+            }
+        __syncthreads();
         if (act) {
+            auto put = [&](int slot, int jidx, double jv) {
+                if (slot < 0) return;
+                if (a.Jx_last != nullptr && inst == 0) a.Jx_last[jidx] = jv;
+                FT v = (FT)jv;
+                if (slot & (1 << 30)) {  // pivot with nothing to eliminate
+                    if (bad_pivot(v)) s_piv[g] = 1;
+                    v = (FT)1 / v;
+                }
+                XS(slot & ((1 << 30) - 1), v);
+            };
             for (int i = s; i < a.nzero; i += nslot) XS(a.zero_slots[i], (FT)0);
-            for (int q0 = s; q0 < a.nJ; q0 += kJU * nslot) {
-                int qq[kJU], sel[kJU], rb[kJU], cb[kJU], yp[kJU], js[kJU];
-#pragma unroll
-                for (int u = 0; u < kJU; ++u) {
-                    const int q = q0 + u * nslot;
-                    qq[u] = q < a.nJ ? q : -1;
-                    const int qc = q < a.nJ ? q : 0;
-                    sel[u] = a.j_sel[qc];
-                    rb[u] = a.j_rbus[qc];
-                    cb[u] = a.j_cbus[qc];
-                    yp[u] = a.j_ypos[qc];
-                    js[u] = a.j_slot[qc];
+            for (int i = s; i < a.njc; i += nslot) put(a.jc_slot[i], a.jc_jidx[i], a.jc_val[i]);
+            for (int e = s; e < a.nye; e += nslot) {
+                const int pos = a.ye_pos[e];
+                const int2 rc = a.ye_rc[e];
+                const int4 sl = a.ye_slot[e];
+                const c64 y = Yat(pos), vr = ld_c(V, rc.x), vc = ld_c(V, rc.y), vnc = ld_c(Vn, rc.y);
+                const bool diag = rc.x == rc.y;
+                c64 t = cmul(y, vc);
+                c64 dvm = cmul(cconj(cmul(y, vnc)), vr);
+                if (diag) {
+                    const c64 ib = ld_c(Ib, rc.x);
+                    t = csub(t, ib);
+                    dvm = cadd(dvm, cmul(cconj(ib), vnc));
                 }
-                c64 y[kJU], vr[kJU], vc[kJU], ib[kJU];
-                double cv[kJU];
-#pragma unroll
-                for (int u = 0; u < kJU; ++u) {
-                    y[u] = ld_c(Y, yp[u] >= 0 ? yp[u] : 0);
-                    vr[u] = ld_c(V, rb[u]);
-                    vc[u] = ld_c(V, cb[u]);
-                    ib[u] = ld_c(Ib, rb[u]);
-                    cv[u] = a.j_cval[qq[u] >= 0 ? qq[u] : 0];
-                }
-#pragma unroll
-                for (int u = 0; u < kJU; ++u) {
-                    if (qq[u] < 0) continue;
-                    const double jv = jvalue_of(sel[u], rb[u] == cb[u], y[u], vr[u], vc[u], ib[u], cv[u]);
-                    if (a.Jx_last != nullptr && inst == 0) a.Jx_last[qq[u]] = jv;
-                    FT v = (FT)jv;
-                    if (js[u] & (1 << 30)) {  // pivot with nothing to eliminate
-                        if (!(fabs((double)v) > 0.0) || !isfinite((double)v)) s_piv[g] = 1;
-                        v = (FT)1 / v;
-                    }
-                    XS(js[u] & ((1 << 30) - 1), v);
-                }
+                const c64 dva = cmul(cconj({-t.re, -t.im}), {-vr.im, vr.re});  // * (j V_r)
+                int4 jx = {0, 0, 0, 0};
+                if (a.Jx_last != nullptr && inst == 0) jx = a.ye_jidx[e];
+                put(sl.x, jx.x, dva.re);
+                put(sl.y, jx.y, dva.im);
+                put(sl.z, jx.z, dvm.re);
+                put(sl.w, jx.w, dvm.im);
             }
         }
         __syncthreads();
@@ -883,28 +952,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     }
 }
 
-// Per-instance setup: Ybus values with deltas and broadcast of V0.
-__global__ void k_apply_deltas(const double* Ybase, double* Yout, int nnzY, int ninst,
-                               const int* dptr, const int* dpos, const double* dval) {
-    const long long total = (long long)ninst * nnzY;
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
-         k += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(k % nnzY);
-        Yout[2 * k] = Ybase[2 * q];
-        Yout[2 * k + 1] = Ybase[2 * q + 1];
-    }
-}
-__global__ void k_apply_deltas2(double* Yout, int nnzY, int ninst, const int* dptr, const int* dpos,
-                                const double* dval) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ninst) return;
-    double* Y = Yout + 2 * (long long)i * nnzY;
-    for (int k = dptr[i]; k < dptr[i + 1]; ++k) {
-        const int q = dpos[k];
-        Y[2 * q] -= dval[2 * k];
-        Y[2 * q + 1] -= dval[2 * k + 1];
-    }
-}
+// Per-instance initial V (broadcast or per instance).
 __global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n, int ninst) {
     const long long total = (long long)ninst * n;
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
@@ -984,6 +1032,8 @@ struct pfgpu_grid {
     pfgpu::DevicePlan P;
     pfgpu_grid_info info{};
     DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots;
+    DevBuf<int> yr_pc, ye_pos, ye_rc, ye_slot, ye_jidx, jc_slot, jc_jidx;
+    DevBuf<double> jc_val;
     DevBuf<int8_t> ux_kind;
     DevBuf<uint8_t> j_sel;
     DevBuf<double> j_cval, w;
@@ -1177,7 +1227,7 @@ void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Core solve on device buffers. Workspace layout: [Yinst?][Va][Vm][Ib][Xg].
+// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Vn][Xg].
 void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
                   const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
                   int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
@@ -1198,28 +1248,18 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req, NT_req);
     ensure_streams(gr, lc.G, lc.NT, fsize, lc.lean);
     const bool deltas = d_dptr != nullptr;
+    if (deltas && Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
     size_t off = 0;
-    const size_t o_y = off;
-    if (deltas) off += align_up((size_t)ninst * P.nnzY * 16);
     const size_t o_va = off; off += align_up((size_t)ninst * n * 8);
     const size_t o_vm = off; off += align_up((size_t)ninst * n * 8);
     const size_t o_ib = off; off += align_up((size_t)ninst * n * 16);
+    const size_t o_vn = off; off += align_up((size_t)ninst * n * 16);
     const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
     char* ws = static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
     hipStream_t stream = ctx->stream;
+    // the kernel applies the per-instance Ybus edits (deltas) to the shared base itself
     const double* Ypt = d_Yx;
-    long long ystride = (long long)Yx_stride * 2;
-    if (deltas) {
-        double* Yi = reinterpret_cast<double*>(ws + o_y);
-        if (Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
-        hipLaunchKernelGGL(k_apply_deltas, dim3(1024), dim3(256), 0, stream, d_Yx, Yi, P.nnzY, ninst, d_dptr, d_dpos,
-                           d_dval);
-        hipLaunchKernelGGL(k_apply_deltas2, dim3((ninst + 255) / 256), dim3(256), 0, stream, Yi, P.nnzY, ninst, d_dptr,
-                           d_dpos, d_dval);
-        HIP_OK(hipGetLastError());
-        Ypt = Yi;
-        ystride = (long long)P.nnzY * 2;
-    }
+    const long long ystride = (long long)Yx_stride * 2;
     if (d_V0 != d_V || v0_stride == 0) {
         // d_V0 == d_V with a shared row is allowed: row 0 is only read by the threads writing row 0 itself
         hipLaunchKernelGGL(k_init_v, dim3(1024), dim3(256), 0, stream, d_V0, (long long)v0_stride * 2, d_V, n, ninst);
@@ -1232,6 +1272,13 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.nlds = lc.nlds; a.nspill = lc.nspill;
     a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
     a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
+    a.yr_pc = reinterpret_cast<const int2*>(gr->yr_pc.p);
+    a.ye_pos = gr->ye_pos.p; a.ye_rc = reinterpret_cast<const int2*>(gr->ye_rc.p);
+    a.ye_slot = reinterpret_cast<const int4*>(gr->ye_slot.p); a.ye_jidx = reinterpret_cast<const int4*>(gr->ye_jidx.p);
+    a.nye = (int)P.ye_pos.size();
+    a.jc_slot = gr->jc_slot.p; a.jc_jidx = gr->jc_jidx.p; a.jc_val = gr->jc_val.p; a.njc = (int)P.jc_slot.size();
+    a.dptr = deltas ? d_dptr : nullptr; a.dpos = d_dpos; a.dval = d_dval;
+    a.Vn = reinterpret_cast<double*>(ws + o_vn);
     a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
@@ -1369,6 +1416,19 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         g->info.flops = P.flops;
         g->info.single_slack = single_slack ? 1 : 0;
         g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
+        {
+            std::vector<int> pc(2 * P.yr_pos.size());
+            for (size_t k = 0; k < P.yr_pos.size(); ++k) { pc[2 * k] = P.yr_pos[k]; pc[2 * k + 1] = P.yr_col[k]; }
+            g->yr_pc.upload(pc.empty() ? std::vector<int>(2, 0) : pc);
+        }
+        auto up_or = [](auto& d, const auto& v, size_t minsz) {
+            auto w = v;
+            if (w.size() < minsz) w.resize(minsz);
+            d.upload(w);
+        };
+        up_or(g->ye_pos, P.ye_pos, 1); up_or(g->ye_rc, P.ye_rc, 2); up_or(g->ye_slot, P.ye_slot, 4);
+        up_or(g->ye_jidx, P.ye_jidx, 4); up_or(g->jc_slot, P.jc_slot, 1); up_or(g->jc_jidx, P.jc_jidx, 1);
+        up_or(g->jc_val, P.jc_val, 1);
         g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
         g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
         g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
