@@ -401,8 +401,8 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
             hot[j] = std::max(hot[j], lv);
             for (int a = S.cp[j]; a < S.cp[j + 1]; ++a) hot[S.off_U + a] = std::max(hot[S.off_U + a], lv);
         }
-    // dense tail first: tile [0, m*m) row-major, then its vector [m*m, m*m+m); every other
-    // slot follows by decreasing hotness
+    // dense tail first: tile [0, m*(m+1)) row-major with row stride m + 1, its vector as
+    // column m; every other slot follows by decreasing hotness
     const int m = S.m, k0 = S.k0;
     P.m = m;
     std::vector<int> colof(S.nnzL);
@@ -410,12 +410,12 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
         for (int q = S.cp[k]; q < S.cp[k + 1]; ++q) colof[q] = k;
     std::vector<int> nw(S.nvals, -1);
     for (int j = k0; j < N; ++j) {
-        nw[j] = (j - k0) * (m + 1);
-        nw[S.off_y + j] = m * m + (j - k0);
+        nw[j] = (j - k0) * (m + 2);
+        nw[S.off_y + j] = (j - k0) * (m + 1) + m;
         for (int q = S.cp[j]; q < S.cp[j + 1]; ++q) {
             const int r = S.ci[q];  // > j >= k0
-            nw[S.off_L + q] = (r - k0) * m + (j - k0);
-            nw[S.off_U + q] = (j - k0) * m + (r - k0);
+            nw[S.off_L + q] = (r - k0) * (m + 1) + (j - k0);
+            nw[S.off_U + q] = (j - k0) * (m + 1) + (r - k0);
         }
     }
     std::vector<int> ord;

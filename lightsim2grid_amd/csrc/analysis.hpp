@@ -99,7 +99,8 @@ Schedule build_schedule(const SymbolicLU& S);
 // in LDS and spill the rest to a per-instance global buffer.
 struct DevicePlan {
     int n = 0, N = 0, off = 0, nnzY = 0, nJ = 0, nvals = 0, slack0 = 0;
-    // dense tail: slots [0, m*m) hold the row-major m x m block, [m*m, m*m+m) its forward/backward vector
+    // dense tail: slots [0, m*(m+1)) hold the m x m block row-major with row stride m + 1,
+    // its forward/backward vector as column m
     int m = 0;
     int zslot = 0;  // a slot that always holds 0 (zeroed by the J fill, never written otherwise)
     int nlev = 0, nblev = 0;

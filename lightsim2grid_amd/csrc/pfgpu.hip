@@ -680,58 +680,82 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
             if (bad) s_piv[g] = 1;
         }
-        // ---- dense tail: the m x m tile of every instance, by all lanes ------------------
-        // Tile at slots [0, m*m) row-major, its vector at [m*m, m*m+m), both in LDS.
-        // Right-looking LU with diagonal pivots, one pass (and one barrier) per column k:
-        // pass k updates the trailing block and the vector with L(i,k) = T(i,k) / T(k,k);
-        // column k itself is scaled, and its pivot replaced by the reciprocal, in pass
-        // k + 1 (so that no pass writes what it reads). Then the backward sweep,
-        // one pass per column from the last: x_k = y_k / d_k, y_i -= U(i,k) x_k (i < k),
-        // x_k stored one pass later.
+        // ---- dense tail: the m x m tile of every instance ---------------------------------
+        // Tile at slots [0, m*(m+1)): row-major with row stride m + 1, the right-hand side
+        // held as column m. Right-looking LU with diagonal pivots, two columns per pass (one
+        // barrier per pass): pass k eliminates columns k and k+1 from the trailing block
+        // (rows >= k+2, columns >= k+2, the vector included) with the rank-2 update
+        //   T(i,j) -= L(i,k) U(k,j) + L(i,k+1) U(k+1,j),
+        //   L(i,k) = T(i,k) / d_k,  U(k+1,j) = T(k+1,j) - L(k+1,k) U(k,j),
+        //   L(i,k+1) = (T(i,k+1) - L(i,k) U(k,k+1)) / d_k+1,
+        // each lane recomputing the L and U factors of its entry from the tile. Row k+1 and
+        // the two reciprocal pivots are stored one pass later (so that no pass writes what
+        // it reads); L itself is not stored, nothing after the forward sweep reads it.
+        // Then the backward sweep on the first wave alone (its lanes hold s < 64 / G of
+        // every instance; LDS accesses of one wave complete in issue order, so its passes
+        // need no barrier): x_k = y_k / d_k, y_i -= U(i,k) x_k for i < k.
         if (a.m > 0) {
-            const int m = a.m, yb = m * m;
-            auto T = [&](int i, int j) -> int { return (i * m + j) * G + g; };  // LDS index
-            FT rd_prev = (FT)0;
-            for (int k = 0; k <= m; ++k) {
+            const int m = a.m, ld = m + 1;
+            auto T = [&](int i, int j) -> int { return (i * ld + j) * G + g; };  // LDS index
+            FT rp0 = (FT)0, rp1 = (FT)0, lp = (FT)0;  // the previous pass's 1/d_k, 1/d_k+1, L(k+1,k)
+            bool bad = false;
+            for (int k = 0;; k += 2) {
                 if (act) {
-                    FT rd = (FT)0;
+                    FT rd0 = (FT)0, rd1 = (FT)0, l10 = (FT)0;
                     if (k < m) {
-                        const FT d = Xl[T(k, k)];
-                        if (s == 0 && (!(fabs((double)d) > 0.0) || !isfinite((double)d))) s_piv[g] = 1;
-                        rd = pivot_recip(d);
-                        const int w = m - k - 1;  // trailing block w x w plus the vector column
-                        for (int idx = s; idx < w * (w + 1); idx += nslot) {
-                            const int i = k + 1 + idx / (w + 1), jj = idx % (w + 1);
-                            const FT lik = Xl[T(i, k)] * rd;
-                            if (jj < w) {
-                                const int j = k + 1 + jj;
-                                Xl[T(i, j)] -= lik * Xl[T(k, j)];
-                            } else {
-                                Xl[(yb + i) * G + g] -= lik * Xl[(yb + k) * G + g];
+                        const FT d0 = Xl[T(k, k)];
+                        bad |= bad_pivot(d0);
+                        rd0 = pivot_recip(d0);
+                        if (k + 1 < m) {
+                            const FT u01 = Xl[T(k, k + 1)];
+                            l10 = Xl[T(k + 1, k)] * rd0;
+                            const FT d1 = Xl[T(k + 1, k + 1)] - l10 * u01;
+                            bad |= bad_pivot(d1);
+                            rd1 = pivot_recip(d1);
+                            // entries (i, j), i in [k+2, m), j in [k+2, m]: w rows of w + 1
+                            const int w = m - k - 2, wc = w + 1;
+                            // row = floor((idx + 1/2) / wc): the fractional part of the exact
+                            // quotient stays >= 1/(2 wc) from an integer, far above the float error
+                            const float inv = __builtin_amdgcn_rcpf((float)wc);
+                            for (int idx = s; idx < w * wc; idx += nslot) {
+                                const int r = (int)(((float)idx + 0.5f) * inv);
+                                const int i = k + 2 + r, j = k + 2 + (idx - r * wc);
+                                const FT li0 = Xl[T(i, k)] * rd0;
+                                const FT li1 = (Xl[T(i, k + 1)] - li0 * u01) * rd1;
+                                const FT u0j = Xl[T(k, j)];
+                                const FT u1j = Xl[T(k + 1, j)] - l10 * u0j;
+                                Xl[T(i, j)] -= li0 * u0j + li1 * u1j;
                             }
                         }
                     }
-                    if (k > 0) {  // finish column k - 1
-                        for (int i = k + s; i < m; i += nslot) Xl[T(i, k - 1)] *= rd_prev;
-                        if (s == 0) Xl[T(k - 1, k - 1)] = rd_prev;
+                    if (k >= 2) {  // store row k-1 of U and the pivots k-2, k-1 of the previous pass
+                        const int p = k - 2;
+                        if (p + 1 < m) {
+                            for (int j = p + 2 + s; j <= m; j += nslot) Xl[T(p + 1, j)] -= lp * Xl[T(p, j)];
+                            if (s == 0) Xl[T(p + 1, p + 1)] = rp1;
+                        }
+                        if (s == 0) Xl[T(p, p)] = rp0;
                     }
-                    rd_prev = rd;
+                    rp0 = rd0;
+                    rp1 = rd1;
+                    lp = l10;
                 }
                 __syncthreads();
+                if (k >= m) break;
             }
-            FT x_prev = (FT)0;
-            for (int k = m - 1; k >= -1; --k) {
-                if (act) {
-                    FT xk = (FT)0;
-                    if (k >= 0) {
-                        xk = Xl[(yb + k) * G + g] * Xl[T(k, k)];
-                        for (int i = s; i < k; i += nslot) Xl[(yb + i) * G + g] -= Xl[T(i, k)] * xk;
-                    }
-                    if (k + 1 < m && s == 0) Xl[(yb + k + 1) * G + g] = x_prev;
-                    x_prev = xk;
+            if (bad && s == 0) s_piv[g] = 1;
+            if (threadIdx.x < 64 && act) {
+                constexpr int ws = 64 / G;  // lanes of one instance in the first wave
+                for (int k = m - 1; k >= 0; --k) {
+                    const FT xk = Xl[T(k, m)] * Xl[T(k, k)];
+                    for (int i = s; i < k; i += ws) Xl[T(i, m)] -= Xl[T(i, k)] * xk;
+                    if (s == 0) Xl[T(k, m)] = xk;
+                    // keep this pass's LDS writes ahead of the next pass's reads in the code
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
         if (s == 0 && act && s_piv[g]) {  // zero / non-finite pivot: klu_refactor failure
             s_status[g] = 6;

@@ -244,12 +244,12 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
             }
         };
         for (int l = 0; l < P.nlev; ++l) run_level(LS.fs, LS.f_lv, l, true);
-        // dense tail: right-looking LU of the m x m tile (slots [0, m*m)), forward on its vector
-        // (slots [m*m, m*m+m)), then backward; the tile diagonal ends up holding 1/d
+        // dense tail: right-looking LU of the m x m tile (slots [0, m*(m+1)), row stride m + 1),
+        // forward on its vector (column m), then backward; the tile diagonal ends up holding 1/d
         if (P.m > 0) {
             const int m = P.m;
-            auto T = [&](int i, int j) -> double& { return X[i * m + j]; };
-            auto yt = [&](int i) -> double& { return X[m * m + i]; };
+            auto T = [&](int i, int j) -> double& { return X[i * (m + 1) + j]; };
+            auto yt = [&](int i) -> double& { return X[i * (m + 1) + m]; };
             for (int k = 0; k < m; ++k) {
                 const double rd = rnd(1.0 / T(k, k));
                 for (int i = k + 1; i < m; ++i) T(i, k) = rnd(T(i, k) * rd);
