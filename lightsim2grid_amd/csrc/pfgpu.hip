@@ -43,6 +43,17 @@ constexpr int kMaxBlock = 1024;
 constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
 constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
+typedef __attribute__((address_space(3))) double lds_dbl;
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+
+// Per-workgroup state of the NR kernel, in dynamic LDS after the value slots and the level
+// table. The kernel declares no static LDS, so its value slots start at LDS address 0.
+// Layout: part[kWaves][G], slack[G], dval[G][kMaxEdits][2] (doubles), then done, it, status,
+// piv [G], dpos[G][kMaxEdits], nd, d0 [G], ndone (ints).
+__host__ __device__ constexpr int wg_state_bytes(int G, int NT) {
+    return 8 * G * (NT / 64 + 1 + 2 * kMaxEdits) + 4 * (G * (6 + kMaxEdits) + 1);
+}
 
 thread_local std::string g_last_error;
 
@@ -109,6 +120,7 @@ struct KArgs {
     const int4* fs;      // {l, u, 1/d, end}: l < 0 no pair; end >= 0: slot | log2(team) << 26 | recip << 30
     const int4* bs;      // {u, x', end, 1/d}: u < 0 no pair; end >= 0: slot | log2(team) << 26
     int lvk_off;         // byte offset of the level table in dynamic LDS
+    int sh_off;          // byte offset of the per-workgroup state in dynamic LDS
     const double* w;
     const double* Yx;
     long long y_stride;  // doubles between instances (0 = shared)
@@ -136,7 +148,7 @@ __device__ __forceinline__ c64 ld_c(const double* p, int i) { return {p[2 * i], 
 // Reduce over the nslot lanes of one instance: shuffles inside the wave
 // (lanes g, g+G, ... of a wave), then the kWaves partials through LDS.
 template <int G, int NT, bool IS_MAX>
-__device__ __forceinline__ double inst_reduce(double v, double (*part)[G]) {
+__device__ __forceinline__ double inst_reduce(double v, lds_dbl* part) {
     constexpr int kWaves = NT / 64;
 #pragma unroll
     for (int o = G; o < 64; o <<= 1) {
@@ -145,12 +157,12 @@ __device__ __forceinline__ double inst_reduce(double v, double (*part)[G]) {
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
-    if (lane < G) part[wave][lane] = v;
+    if (lane < G) part[wave * G + lane] = v;
     __syncthreads();
     const int g = threadIdx.x % G;
-    double r = part[0][g];
+    double r = part[g];
 #pragma unroll
-    for (int k = 1; k < kWaves; ++k) r = IS_MAX ? fmax(r, part[k][g]) : fmin(r, part[k][g]);
+    for (int k = 1; k < kWaves; ++k) r = IS_MAX ? fmax(r, part[k * G + g]) : fmin(r, part[k * G + g]);
     return r;
 }
 
@@ -274,13 +286,20 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // pointer into LDS (flat pointers into LDS mixed with global ones trip the gfx950 backend)
     typedef __attribute__((address_space(3))) FT lds_ft;
     lds_ft* Xl = (lds_ft*)(lds_raw);
-    __shared__ double s_part[kWaves][G];
-    __shared__ double s_slack[G];
-    __shared__ int s_done[G], s_it[G], s_status[G], s_piv[G];
-    __shared__ int s_ndone;
+    lds_u8* const shb = (lds_u8*)lds_raw + a.sh_off;
+    lds_dbl* const s_part = (lds_dbl*)shb;  // [kWaves][G]
+    lds_dbl* const s_slack = s_part + kWaves * G;
+    lds_dbl* const s_dval = s_slack + G;  // [G][kMaxEdits][2]: this instance's Ybus edits
+    lds_int* const s_done = (lds_int*)(s_dval + G * kMaxEdits * 2);
+    lds_int* const s_it = s_done + G;
+    lds_int* const s_status = s_it + G;
+    lds_int* const s_piv = s_status + G;
+    lds_int* const s_dpos = s_piv + G;  // [G][kMaxEdits]
+    lds_int* const s_nd = s_dpos + G * kMaxEdits;
+    lds_int* const s_d0 = s_nd + G;
+    lds_int* const s_ndone = s_d0 + G;
     // {items per lane, used lanes} of every factor level, then of every backward level
     // (in LDS after the value slots)
-    typedef __attribute__((address_space(3))) int lds_int;
     lds_int* lvK = (lds_int*)(lds_raw + a.lvk_off);
     for (int i = threadIdx.x; i < 2 * (a.nlev + a.nblev); i += NT) lvK[i] = a.lv_K[i];
 
@@ -299,11 +318,12 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // Operand read for the level loops: always an LDS read (clamped slot) and, only for a
     // spilled slot, a global load into a separate register. A shared destination register
     // would make every LDS read wait for all outstanding global loads (the prefetched items).
-    // lean launches (every slot in LDS): items carry byte offsets of this instance's slots
-    typedef __attribute__((address_space(3))) unsigned char lds_u8;
-    lds_u8* Xb = (lds_u8*)lds_raw + g * (int)sizeof(FT);
-    auto XB = [&](int off) -> FT { return *(lds_ft*)(Xb + off); };
-    auto XBset = [&](int off, FT v) { *(lds_ft*)(Xb + off) = v; };
+    // lean launches (every slot in LDS): items carry byte offsets of instance 0's slots, which
+    // are their LDS addresses (the value slots start at LDS address 0: the kernel has no static
+    // LDS, checked by the host before a lean launch), so an operand read takes no address add
+    const int xg = g * (int)sizeof(FT);
+    auto XB = [&](int off) -> FT { return *(lds_ft*)(size_t)(unsigned)(off + xg); };
+    auto XBset = [&](int off, FT v) { *(lds_ft*)(size_t)(unsigned)(off + xg) = v; };
     auto XLo = [&](int slot) -> FT {
         const bool in_lds = slot < nlds;
         const FT vl = Xl[min(slot, nlds - 1) * G + g];
@@ -323,8 +343,6 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     double* Ib = a.Ib + ii * 2 * (long long)n;
     double* Vn = a.Vn + ii * 2 * (long long)n;
     // this instance's Ybus edits: the first kMaxEdits in LDS, any further ones read from memory
-    __shared__ int s_dpos[G][kMaxEdits], s_nd[G], s_d0[G];
-    __shared__ double s_dval[G][kMaxEdits][2];
     if (s == 0) {
         s_nd[g] = 0;
         s_d0[g] = 0;
@@ -336,9 +354,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     __syncthreads();
     if (s < kMaxEdits && s < s_nd[g]) {
         const int k = s_d0[g] + s;
-        s_dpos[g][s] = a.dpos[k];
-        s_dval[g][s][0] = a.dval[2 * k];
-        s_dval[g][s][1] = a.dval[2 * k + 1];
+        s_dpos[g * kMaxEdits + s] = a.dpos[k];
+        s_dval[(g * kMaxEdits + s) * 2] = a.dval[2 * k];
+        s_dval[(g * kMaxEdits + s) * 2 + 1] = a.dval[2 * k + 1];
     }
     const int nd = s_nd[g], d0 = s_d0[g];
     // Ybus value at CSC position pos for this instance
@@ -346,9 +364,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         c64 y = ld_c(Y, pos);
         const int nl = min(nd, kMaxEdits);
         for (int d = 0; d < nl; ++d)
-            if (s_dpos[g][d] == pos) {
-                y.re -= s_dval[g][d][0];
-                y.im -= s_dval[g][d][1];
+            if (s_dpos[g * kMaxEdits + d] == pos) {
+                y.re -= s_dval[(g * kMaxEdits + d) * 2];
+                y.im -= s_dval[(g * kMaxEdits + d) * 2 + 1];
             }
         for (int d = kMaxEdits; d < nd; ++d)
             if (a.dpos[d0 + d] == pos) {
@@ -369,11 +387,11 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     for (int o = G; o < 64; o <<= 1) ssum += __shfl_xor(ssum, o);
     {
         const int lane = tid & 63, wave = tid >> 6;
-        if (lane < G) s_part[wave][lane] = ssum;
+        if (lane < G) s_part[wave * G + lane] = ssum;
         __syncthreads();
         if (s == 0) {
             double t = 0;
-            for (int k = 0; k < kWaves; ++k) t += s_part[k][g];
+            for (int k = 0; k < kWaves; ++k) t += s_part[k * G + g];
             s_slack[g] = t;
             s_it[g] = 0;
             s_status[g] = 0;
@@ -429,10 +447,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (tid == 0) {
             int nd = 0;
             for (int k = 0; k < G; ++k) nd += s_done[k];
-            s_ndone = nd;
+            *s_ndone = nd;
         }
         __syncthreads();
-        if (s_ndone == G) break;
+        if (*s_ndone == G) break;
         bool act = valid && !s_done[g];
         if (s == 0 && act) s_it[g] += 1;
 
@@ -1113,8 +1131,9 @@ int log2i(int g) {
 struct LaunchChoice {
     int G, NT, nlds, nspill, nblocks;
     bool lean;       // every value slot in LDS: the lean kernel variant and streams
-    size_t lds;      // dynamic LDS bytes: value slots, then the level table
+    size_t lds;      // dynamic LDS bytes: value slots, the level table, the workgroup state
     size_t lvk_off;  // byte offset of the level table
+    size_t sh_off;   // byte offset of the workgroup state
 };
 
 template <typename FT, int NT, bool LEAN>
@@ -1166,7 +1185,7 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     // per workgroup: static LDS of the variant + value slots + the level table
     const size_t table = align16((size_t)(P.nlev + P.nblev) * 8);
     auto avail = [&](int G, int NT) -> size_t {
-        const size_t fixed = static_lds(fsize, G, NT) + table + 16;
+        const size_t fixed = static_lds(fsize, G, NT) + table + align16((size_t)wg_state_bytes(G, NT)) + 32;
         return (size_t)ctx->lds_max > fixed ? (size_t)ctx->lds_max - fixed : 0;
     };
     LaunchChoice c{};
@@ -1199,9 +1218,13 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
         throw ApiError(PFGPU_E_INVALID, "the grid's dense tail does not fit in LDS");
     c.nspill = P.nvals - c.nlds;
     c.lean = c.nspill == 0;
+    // lean items carry LDS addresses: the value slots must start at LDS address 0
+    if (c.lean && static_lds(fsize, c.G, c.NT) != 0)
+        throw ApiError(PFGPU_E_INTERNAL, "the NR kernel has static LDS: lean item offsets would not be LDS addresses");
     c.nblocks = (ninst + c.G - 1) / c.G;
     c.lvk_off = align16((size_t)c.nlds * c.G * fsize);
-    c.lds = c.lvk_off + table;
+    c.sh_off = c.lvk_off + table;
+    c.lds = c.sh_off + align16((size_t)wg_state_bytes(c.G, c.NT));
     return c;
 }
 
@@ -1308,7 +1331,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
     a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
     const auto& ls = *gr->streams[shape_index(lc.G, lc.NT, fsize, lc.lean)];
-    a.lv_K = ls.lv_K.p; a.lvk_off = (int)lc.lvk_off;
+    a.lv_K = ls.lv_K.p; a.lvk_off = (int)lc.lvk_off; a.sh_off = (int)lc.sh_off;
     a.fs = reinterpret_cast<const int4*>(ls.fs.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
     a.w = gr->w.p;
     a.Yx = Ypt; a.y_stride = ystride;
