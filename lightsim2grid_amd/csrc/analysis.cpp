@@ -838,6 +838,107 @@ LevelPlacement choose_placement(const std::vector<int>& np, int G, int nslot) {
     return best;
 }
 
+// Chunked placement of one level (lean encoding): a lane's items form c chunks of Lc
+// items, a task takes one chunk of a team of T = 2^lt lanes (the smallest power of two
+// with ceil(np / T) <= Lc). Teams are laid out in decreasing size, column after column of
+// nslot lanes, so that each is aligned to its size. Lc minimises an estimate of the level's
+// time in item units: the issue work of a lane (c chunks of Lc items plus an end each: a
+// reduction of lt steps, the read of the old value and the leader's write) times the waves
+// that share a SIMD, plus the latency of the item loads past the kHead items per lane that
+// are loaded during the previous level (two items per round trip).
+struct ChunkPlacement {
+    int Lc = 0, c = 0, U = 0, maxlt = 0;
+    std::vector<int> col, lane0, lt;  // per task: chunk column, first lane, log2 team size
+};
+
+ChunkPlacement place_chunked(const std::vector<int>& np, int G, int nslot, int lane_round, int kHead) {
+    const int tmax = std::max(1, std::min(64 / G, nslot));
+    const int tmax_log = log2_ceil(tmax + 1) - 1;  // floor(log2(tmax))
+    const int nt = (int)np.size();
+    int maxnp = 1;
+    for (int x : np) maxnp = std::max(maxnp, x);
+    const int lc_lo = std::max(1, ceil_div(maxnp, 1 << tmax_log));
+    ChunkPlacement best;
+    double best_cost = 1e300;
+    std::vector<int> lt(nt);
+    for (int Lc = lc_lo; Lc <= std::max(lc_lo, maxnp); Lc = Lc < 64 ? Lc + 1 : Lc + Lc / 8) {
+        long total = 0;
+        int maxlt = 0;
+        for (int i = 0; i < nt; ++i) {
+            lt[i] = log2_ceil(std::max(1, ceil_div(std::max(np[i], 1), Lc)));
+            total += 1L << lt[i];
+            maxlt = std::max(maxlt, lt[i]);
+        }
+        const int c = (int)((total + nslot - 1) / nslot);
+        const int U = c == 1 ? std::min(nslot, (int)((total + lane_round - 1) / lane_round * lane_round)) : nslot;
+        const double simd_load = std::max(1.0, std::ceil((double)U * G / 64.0) / 4.0);
+        const int K = c * Lc;
+        const double cost = simd_load * (K + c * (7.0 + 1.5 * maxlt)) + 12.0 * std::max(0, (K - kHead + 1) / 2);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best.Lc = Lc;
+            best.c = c;
+            best.U = U;
+            best.maxlt = maxlt;
+            best.lt = lt;
+        }
+    }
+    std::vector<int> ord(nt);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return best.lt[x] > best.lt[y]; });
+    best.col.assign(nt, 0);
+    best.lane0.assign(nt, 0);
+    long p = 0;
+    for (int i : ord) {
+        best.col[i] = (int)(p / nslot);
+        best.lane0[i] = (int)(p % nslot);
+        p += 1L << best.lt[i];
+    }
+    return best;
+}
+
+// Lean layout of the levels of a task list: K = c * Lc items per lane, lanes [0, U), item k
+// of lane s at base + k * U + s. emit(task, pair index or -1, end, log2 team size, item)
+// fills one item pre-set to pad.
+template <class NPairs, class Emit>
+void layout_chunked(const std::vector<int>& lvl_ptr, int G, int nslot, NPairs npairs_of, Emit emit,
+                    const int (&pad)[4], int lane_round, int k_head, std::vector<int>& lv, std::vector<int>& lcs,
+                    std::vector<int>& stream) {
+    const int nlev = (int)lvl_ptr.size() - 1;
+    lv.assign(4 * (size_t)std::max(nlev, 0), 0);
+    lcs.assign(std::max(nlev, 0), 0);
+    size_t base = stream.size() / 4;
+    for (int l = 0; l < nlev; ++l) {
+        const int t0 = lvl_ptr[l], nt = lvl_ptr[l + 1] - t0;
+        std::vector<int> np(nt);
+        for (int i = 0; i < nt; ++i) np[i] = npairs_of(t0 + i);
+        const ChunkPlacement pl = place_chunked(np, G, nslot, lane_round, k_head);
+        const int K = pl.c * pl.Lc, U = pl.U;
+        lv[4 * l] = (int)base;
+        lv[4 * l + 1] = K;
+        lv[4 * l + 2] = 1 << pl.maxlt;
+        lv[4 * l + 3] = U;
+        lcs[l] = pl.Lc;
+        if (K >= (1 << 16) || pl.Lc >= (1 << 15)) throw std::runtime_error("lane streams: level too long");
+        if (base + (size_t)K * U >= (size_t)1 << 31) throw std::runtime_error("lane stream too large");
+        const size_t at = stream.size();
+        stream.resize(4 * (base + (size_t)K * U));
+        for (size_t q = at; q < stream.size(); q += 4)
+            for (int w = 0; w < 4; ++w) stream[q + w] = pad[w];
+        for (int i = 0; i < nt; ++i) {
+            const int T = 1 << pl.lt[i];
+            for (int tp = 0; tp < T; ++tp)
+                for (int jj = 0; jj < pl.Lc; ++jj) {
+                    const int pi = jj * T + tp;
+                    const size_t k = (size_t)pl.col[i] * pl.Lc + jj;
+                    int* it = &stream[4 * (base + k * U + pl.lane0[i] + tp)];
+                    emit(t0 + i, pi < np[i] ? pi : -1, jj == pl.Lc - 1, pl.lt[i], it);
+                }
+        }
+        base += (size_t)K * U;
+    }
+}
+
 // Lays out the levels [lvl_ptr[l], lvl_ptr[l+1]) of a task list. Level l uses lanes
 // [0, U) and K items per lane, stored lane-minor at base + k * U + s. emit(task, pair
 // index within the task or -1 for none, end (is the task's last item), log2 team size,
@@ -914,8 +1015,15 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
         L.fs.assign(fpad, fpad + 4);
         L.bs.assign(bpad, bpad + 4);
     }
-    layout_streams(
-        P.f_lvl, G, nslot, [&](int t) { return P.tasks[4 * t + 2] - P.tasks[4 * t + 1]; },
+    auto lay = [&](const std::vector<int>& lvl_ptr, auto npairs_of, auto emit, int end_word, const int(&pad)[4],
+                   std::vector<int>& lv, std::vector<int>& lcs, std::vector<int>& stream) {
+        if (lean)
+            layout_chunked(lvl_ptr, G, nslot, npairs_of, emit, pad, round, std::max(0, lean->head_items), lv, lcs, stream);
+        else
+            layout_streams(lvl_ptr, G, nslot, npairs_of, emit, end_word, pad, 1, 1, true, lv, stream);
+    };
+    lay(
+        P.f_lvl, [&](int t) { return P.tasks[4 * t + 2] - P.tasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {
             if (pi >= 0) {
                 const int p = P.tasks[4 * t + 1] + pi;
@@ -927,9 +1035,9 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
                 it[3] = P.tasks[4 * t] * sc | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0) |
                         (P.tasks[4 * t + 3] == 3 ? kStreamAssign : 0) | kStreamEnd;
         },
-        3, fpad, round, lean ? std::max(1, lean->k_round) : 1, !lean, L.f_lv, L.fs);
-    layout_streams(
-        P.b_lvl, G, nslot, [&](int t) { return P.btasks[4 * t + 2] - P.btasks[4 * t + 1]; },
+        3, fpad, L.f_lv, L.f_lc, L.fs);
+    lay(
+        P.b_lvl, [&](int t) { return P.btasks[4 * t + 2] - P.btasks[4 * t + 1]; },
         [&](int t, int pi, bool last, int lt, int* it) {
             if (pi >= 0) {
                 const int p = P.btasks[4 * t + 1] + pi;
@@ -941,7 +1049,7 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
                 it[3] = P.btasks[4 * t + 3] * sc;
             }
         },
-        2, bpad, round, lean ? std::max(1, lean->k_round) : 1, !lean, L.b_lv, L.bs);
+        2, bpad, L.b_lv, L.b_lc, L.bs);
     return L;
 }
 

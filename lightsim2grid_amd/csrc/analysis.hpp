@@ -178,6 +178,8 @@ struct LaneStreams {
     int G = 0, nslot = 0;
     // per level: {base item, K items per lane, largest team size, U used lanes}
     std::vector<int> f_lv, b_lv;
+    // lean streams only, per level: the chunk length Lc (see LeanFormat)
+    std::vector<int> f_lc, b_lc;
     // forward items {l, u, d, end word}: l < 0 = no pair
     std::vector<int> fs;
     // backward items {u, x', end word, 1/d slot}: u < 0 = no pair
@@ -187,13 +189,17 @@ struct LaneStreams {
 // Lean encoding, for launches that keep every value slot in LDS: items hold LDS byte
 // offsets (slot * elem_bytes) instead of slot ids, items without a pair point at the
 // zero slot (no per-item checks), the lanes of a level are rounded up to lane_round
-// (a whole wave), the items per lane to k_round, and no last-item flags are set (every
-// lane of a used wave walks all K items; the padding items add zeros). Item 0 of a lean
-// stream is a padding item, read in place of items past a level's end.
+// (a whole wave), and no last-item flags are set (every lane of a used wave walks all K
+// items; the padding items add zeros). Task ends are aligned: a level's K items per lane
+// form K / Lc chunks of Lc items, each task takes one chunk of a team of T lanes (T the
+// smallest power of two with ceil(pairs / T) <= Lc), so every lane's task ends fall on
+// the items Lc - 1, 2 Lc - 1, ..., the same for all lanes, and the kernel accumulates
+// the other items without any test. The end word of a chunk of a lane without a task
+// is 0. Item 0 of a lean stream is a padding item, read in place of items past a level's end.
 struct LeanFormat {
     int elem_bytes = 4;  // bytes between consecutive slots of one instance (G * value size)
     int lane_round = 1;  // lanes per wave of one instance (64 / G)
-    int k_round = 1;     // items per lane are padded to a multiple of this (the kernel's chunk)
+    int head_items = 4;  // items per lane of a level that the kernel loads during the level before
 };
 
 LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const LeanFormat* lean = nullptr);

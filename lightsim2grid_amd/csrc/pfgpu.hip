@@ -510,10 +510,14 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         {
             bool bad = false;  // a zero or non-finite pivot met by this lane
             if constexpr (LEAN) {
-            // The first kHB items of a level are loaded during the level before it (two
-            // register buffers, by level parity); the rest of a longer level is loaded in the
-            // level. Every lane issues every load: items past a level's end read the padding
-            // item 0, lanes of a wave idle at a level read a valid item of it they do not use.
+            // Level table word 0: K | Lc << 16. A lane's K items of a level form K / Lc
+            // chunks, and every task ends at the last item of a chunk, at item indices that are
+            // the same for all lanes: the items are summed with no test, and the team reduction
+            // and the leader's update run once per chunk. The first kHB items of a level are
+            // loaded during the level before it (two register buffers, by level parity); the
+            // rest of a longer level is loaded in the level. Every lane issues every load: items
+            // past a level's end read the padding item 0, lanes of a wave idle at a level read a
+            // valid item of it they do not use.
             auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
             auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
                 const int sl = min(s, U - 1);
@@ -522,73 +526,77 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             };
             FT acc = (FT)0;
             const int wave_s0 = (threadIdx.x & ~63) / G;
-            // Within a level no item reads a slot written in it, so the operands of all of a
-            // lane's items (and the old content of their output slots) are read first.
-            struct Ops {
-                FT p, old;
-            };
-            auto ops = [&](const int4 q) -> Ops { return {XB(q.x) * XB(q.y) * XB(q.z), XB(q.w & ((1 << 25) - 1))}; };
-            auto item = [&](const int4 q, const Ops o, int T) {
-                acc += o.p;
-                const int e = q.w;
-                if (e & kEnd) {  // end of a task: the team's lanes reduce, the leader writes
-                    const int lt = (e >> 26) & 7;
-                    const FT sum = team_sum<G>(acc, lt, T);
-                    if ((s & ((1 << lt) - 1)) == 0) {
-                        const int off = e & ((1 << 25) - 1);
-                        FT v = (((e >> 25) & 1) ? (FT)0 : o.old) - sum;  // fill-in: assign
-                        if ((e >> 29) & 1) {
-                            bad |= bad_pivot(v);
-                            v = pivot_recip(v);
-                        }
-                        XBset(off, v);
+            // end of a chunk: the team's lanes reduce, the leader updates the task's slot
+            auto chunk_end = [&](const int e, const int T) {
+                const int lt = (e >> 26) & 7;
+                const FT sum = team_sum<G>(acc, lt, T);
+                if ((e & kEnd) && (s & ((1 << lt) - 1)) == 0) {
+                    const int off = e & ((1 << 25) - 1);
+                    FT v = (((e >> 25) & 1) ? (FT)0 : XB(off)) - sum;  // fill-in: assign
+                    if ((e >> 29) & 1) {
+                        bad |= bad_pivot(v);
+                        v = pivot_recip(v);
                     }
-                    acc = (FT)0;
+                    XBset(off, v);
                 }
+                acc = (FT)0;
             };
-            auto level = [&](int base, int K, int U, int T, const int4 (&h)[kHB]) {
+            auto prod = [&](const int4 q) -> FT { return XB(q.x) * XB(q.y) * XB(q.z); };
+            auto level = [&](int base, int K, int Lc, int U, int T, const int4 (&h)[kHB]) {
                 if (wave_s0 < U) {  // waves with no lane in use at this level only keep the barrier
-                    Ops o[kHB];
+                    FT p[kHB];
 #pragma unroll
-                    for (int u = 0; u < kHB; ++u) o[u] = ops(h[u]);
+                    for (int u = 0; u < kHB; ++u) p[u] = prod(h[u]);
+                    int kend = Lc - 1;
 #pragma unroll
                     for (int u = 0; u < kHB; ++u)
-                        if (u < K) item(h[u], o[u], T);
+                        if (u < K) {
+                            acc += p[u];
+                            if (u == kend) {
+                                chunk_end(h[u].w, T);
+                                kend += Lc;
+                            }
+                        }
                     const int sl = min(s, U - 1);
                     for (int k = kHB; k < K; k += 2) {
                         const int4 r0 = a.fs[(size_t)base + (size_t)k * U + sl];
                         const int4 r1 = a.fs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
-                        const Ops o0 = ops(r0), o1 = ops(r1);
-                        item(r0, o0, T);
-                        item(r1, o1, T);
+                        const FT p0 = prod(r0), p1 = prod(r1);
+                        acc += p0;
+                        if (k == kend) {
+                            chunk_end(r0.w, T);
+                            kend += Lc;
+                        }
+                        acc += p1;
+                        if (k + 1 == kend) {
+                            chunk_end(r1.w, T);
+                            kend += Lc;
+                        }
                     }
                 }
                 __syncthreads();
             };
             int4 H0[kHB], H1[kHB];
             int base = 1;  // level 0 follows the leading padding item
-            int K = lvk(0, 0), w = lvk(0, 1);
-            load_head(base, K, w & 0xffffff, H0);
+            int w0 = lvk(0, 0), w1 = lvk(0, 1);
+            load_head(base, w0 & 0xffff, w1 & 0xffffff, H0);
             for (int l = 0; l < a.nlev; l += 2) {
-                int U = w & 0xffffff, T = w >> 24;
-                int nb = base + K * U;
-                int Kn = lvk(l + 1, 0), wn = lvk(l + 1, 1);
-                load_head(nb, Kn, wn & 0xffffff, H1);
-                level(base, K, U, T, H0);
+                int nb = base + (w0 & 0xffff) * (w1 & 0xffffff);
+                int n0 = lvk(l + 1, 0), n1 = lvk(l + 1, 1);
+                load_head(nb, n0 & 0xffff, n1 & 0xffffff, H1);
+                level(base, w0 & 0xffff, w0 >> 16, w1 & 0xffffff, w1 >> 24, H0);
                 if (l + 1 >= a.nlev) break;
                 base = nb;
-                K = Kn;
-                w = wn;
-                U = w & 0xffffff;
-                T = w >> 24;
-                nb = base + K * U;
-                Kn = lvk(l + 2, 0);
-                wn = lvk(l + 2, 1);
-                load_head(nb, Kn, wn & 0xffffff, H0);
-                level(base, K, U, T, H1);
+                w0 = n0;
+                w1 = n1;
+                nb = base + (w0 & 0xffff) * (w1 & 0xffffff);
+                n0 = lvk(l + 2, 0);
+                n1 = lvk(l + 2, 1);
+                load_head(nb, n0 & 0xffff, n1 & 0xffffff, H0);
+                level(base, w0 & 0xffff, w0 >> 16, w1 & 0xffffff, w1 >> 24, H1);
                 base = nb;
-                K = Kn;
-                w = wn;
+                w0 = n0;
+                w1 = n1;
             }
             } else {
             // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
@@ -785,11 +793,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // ---- backward solve (same pipelined walk as the factor phase) -----------------
         {
             if constexpr (LEAN) {
+            // chunked items as in the factor walk; x = (x - sum u x') / d at a chunk's end
             const lds_int* bK = lvK + 2 * a.nlev;
-            // The first kHB items of a level are loaded during the level before it (two
-            // register buffers, by level parity); the rest of a longer level is loaded in the
-            // level. Every lane issues every load: items past a level's end read the padding
-            // item 0, lanes of a wave idle at a level read a valid item of it they do not use.
             auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
             auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
                 const int sl = min(s, U - 1);
@@ -798,65 +803,72 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             };
             FT acc = (FT)0;
             const int wave_s0 = (threadIdx.x & ~63) / G;
-            struct Ops {
-                FT p, old, rd;
-            };
-            auto ops = [&](const int4 q) -> Ops {
-                return {XB(q.x) * XB(q.y), XB(q.z & ((1 << 25) - 1)), XB(q.w)};
-            };
-            auto item = [&](const int4 q, const Ops o, int T) {
-                acc += o.p;
+            auto chunk_end = [&](const int4 q, const int T) {
                 const int e = q.z;
-                if (e & kEnd) {
-                    const int lt = (e >> 26) & 7;
-                    const FT sum = team_sum<G>(acc, lt, T);
-                    if ((s & ((1 << lt) - 1)) == 0) XBset(e & ((1 << 25) - 1), (o.old - sum) * o.rd);
-                    acc = (FT)0;
+                const int lt = (e >> 26) & 7;
+                const FT sum = team_sum<G>(acc, lt, T);
+                if ((e & kEnd) && (s & ((1 << lt) - 1)) == 0) {
+                    const int off = e & ((1 << 25) - 1);
+                    XBset(off, (XB(off) - sum) * XB(q.w));
                 }
+                acc = (FT)0;
             };
-            auto level = [&](int base, int K, int U, int T, const int4 (&h)[kHB]) {
+            auto prod = [&](const int4 q) -> FT { return XB(q.x) * XB(q.y); };
+            auto level = [&](int base, int K, int Lc, int U, int T, const int4 (&h)[kHB]) {
                 if (wave_s0 < U) {  // waves with no lane in use at this level only keep the barrier
-                    Ops o[kHB];
+                    FT p[kHB];
 #pragma unroll
-                    for (int u = 0; u < kHB; ++u) o[u] = ops(h[u]);
+                    for (int u = 0; u < kHB; ++u) p[u] = prod(h[u]);
+                    int kend = Lc - 1;
 #pragma unroll
                     for (int u = 0; u < kHB; ++u)
-                        if (u < K) item(h[u], o[u], T);
+                        if (u < K) {
+                            acc += p[u];
+                            if (u == kend) {
+                                chunk_end(h[u], T);
+                                kend += Lc;
+                            }
+                        }
                     const int sl = min(s, U - 1);
                     for (int k = kHB; k < K; k += 2) {
                         const int4 r0 = a.bs[(size_t)base + (size_t)k * U + sl];
                         const int4 r1 = a.bs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
-                        const Ops o0 = ops(r0), o1 = ops(r1);
-                        item(r0, o0, T);
-                        item(r1, o1, T);
+                        const FT p0 = prod(r0), p1 = prod(r1);
+                        acc += p0;
+                        if (k == kend) {
+                            chunk_end(r0, T);
+                            kend += Lc;
+                        }
+                        acc += p1;
+                        if (k + 1 == kend) {
+                            chunk_end(r1, T);
+                            kend += Lc;
+                        }
                     }
                 }
                 __syncthreads();
             };
             int4 H0[kHB], H1[kHB];
             int base = 1;  // level 0 follows the leading padding item
-            int K = lvk(0, 0), w = lvk(0, 1);
-            load_head(base, K, w & 0xffffff, H0);
+            int w0 = lvk(0, 0), w1 = lvk(0, 1);
+            load_head(base, w0 & 0xffff, w1 & 0xffffff, H0);
             for (int l = 0; l < a.nblev; l += 2) {
-                int U = w & 0xffffff, T = w >> 24;
-                int nb = base + K * U;
-                int Kn = lvk(l + 1, 0), wn = lvk(l + 1, 1);
-                load_head(nb, Kn, wn & 0xffffff, H1);
-                level(base, K, U, T, H0);
+                int nb = base + (w0 & 0xffff) * (w1 & 0xffffff);
+                int n0 = lvk(l + 1, 0), n1 = lvk(l + 1, 1);
+                load_head(nb, n0 & 0xffff, n1 & 0xffffff, H1);
+                level(base, w0 & 0xffff, w0 >> 16, w1 & 0xffffff, w1 >> 24, H0);
                 if (l + 1 >= a.nblev) break;
                 base = nb;
-                K = Kn;
-                w = wn;
-                U = w & 0xffffff;
-                T = w >> 24;
-                nb = base + K * U;
-                Kn = lvk(l + 2, 0);
-                wn = lvk(l + 2, 1);
-                load_head(nb, Kn, wn & 0xffffff, H0);
-                level(base, K, U, T, H1);
+                w0 = n0;
+                w1 = n1;
+                nb = base + (w0 & 0xffff) * (w1 & 0xffffff);
+                n0 = lvk(l + 2, 0);
+                n1 = lvk(l + 2, 1);
+                load_head(nb, n0 & 0xffff, n1 & 0xffffff, H0);
+                level(base, w0 & 0xffff, w0 >> 16, w1 & 0xffffff, w1 >> 24, H1);
                 base = nb;
-                K = Kn;
-                w = wn;
+                w0 = n0;
+                w1 = n1;
             }
             } else {
             const lds_int* bK = lvK + 2 * a.nlev;
@@ -1239,19 +1251,18 @@ void ensure_streams(pfgpu_grid* gr, int G, int NT, int fsize, bool lean) {
     pfgpu::LeanFormat fmt;
     fmt.elem_bytes = G * fsize;
     fmt.lane_round = std::max(1, 64 / G);
-    fmt.k_round = kSC;
     const pfgpu::LaneStreams L = pfgpu::build_lane_streams(gr->P, G, NT / G, lean ? &fmt : nullptr);
     auto t = std::make_unique<pfgpu_grid::Streams>();
     auto up = [](DevBuf<int>& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>(4, -1) : v); };
     std::vector<int> K;
-    // per level {K, U | log2(largest team) << 24}
+    // per level {K | Lc << 16 (lean streams: chunk length), U | log2(largest team) << 24}
     auto lg = [](int t) { int l = 0; while ((1 << l) < t) ++l; return l; };
     for (size_t l = 0; l < L.f_lv.size() / 4; ++l) {
-        K.push_back(L.f_lv[4 * l + 1]);
+        K.push_back(L.f_lv[4 * l + 1] | (lean ? L.f_lc[l] << 16 : 0));
         K.push_back(L.f_lv[4 * l + 3] | (lg(L.f_lv[4 * l + 2]) << 24));
     }
     for (size_t l = 0; l < L.b_lv.size() / 4; ++l) {
-        K.push_back(L.b_lv[4 * l + 1]);
+        K.push_back(L.b_lv[4 * l + 1] | (lean ? L.b_lc[l] << 16 : 0));
         K.push_back(L.b_lv[4 * l + 3] | (lg(L.b_lv[4 * l + 2]) << 24));
     }
     up(t->lv_K, K);

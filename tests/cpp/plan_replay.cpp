@@ -22,7 +22,7 @@ extern "C" {
 // Also reports write-write / read-before-write conflicts found in the level schedule (conflicts_out).
 int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const double* Sbus,
                  const int* slack_ids, int nslack, const double* w, const int* pv, int npv, const int* pq, int npq,
-                 int single_slack, double* V, int max_iter, double tol, int use_fp32, int G, int nslot,
+                 int single_slack, double* V, int max_iter, double tol, int use_fp32, int G, int nslot, int lean,
                  int* status_out, int* conflicts_out, double* Jx_out) {
     YbusPattern Y;
     Y.n = n;
@@ -32,7 +32,13 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     SymbolicLU S = symbolic_lu(J);
     Schedule sch = build_schedule(S);
     DevicePlan P = build_device_plan(Y, J, S, sch);
-    const LaneStreams LS = build_lane_streams(P, G, nslot);
+    // lean != 0: replay the lean streams (LDS byte offsets, aligned task ends) of the kernel's
+    // value size instead of the general ones
+    LeanFormat fmt;
+    fmt.elem_bytes = G * (use_fp32 ? 4 : 8);
+    fmt.lane_round = std::max(1, 64 / G);
+    const LaneStreams LS = lean ? build_lane_streams(P, G, nslot, &fmt) : build_lane_streams(P, G, nslot);
+    const int sc = lean ? fmt.elem_bytes : 1, zb = lean ? P.zslot * sc : -1;
     const cplx* Yc = reinterpret_cast<const cplx*>(Yx);
     const cplx* Sc = reinterpret_cast<const cplx*>(Sbus);
     std::vector<cplx> Vc(reinterpret_cast<cplx*>(V), reinterpret_cast<cplx*>(V) + n);
@@ -62,16 +68,21 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
         // lane streams: all lanes of a team (aligned group of T lanes, T from the end
         // word) end each task at the same item with the same end word, every task is
         // ended by T lanes, the items cover each task's pairs exactly once, and every
-        // used lane has one "last item" flag after which it holds nothing
-        auto check_streams = [&](const std::vector<int>& lv, const std::vector<int>& st, int nl, bool fwd,
-                                 const std::vector<int>& tasks, const std::vector<int>& lvl) {
+        // used lane has one "last item" flag after which it holds nothing. Lean streams:
+        // no last flags, padding items point at the zero slot, task ends only at the last
+        // item of a chunk (items Lc - 1, 2 Lc - 1, ...), the end word of other items is 0,
+        // and the lanes are a whole number of waves.
+        auto check_streams = [&](const std::vector<int>& lv, const std::vector<int>& lcs, const std::vector<int>& st,
+                                 int nl, bool fwd, const std::vector<int>& tasks, const std::vector<int>& lvl) {
             const int ecol = fwd ? 3 : 2;
             for (int l = 0; l < nl; ++l) {
                 // per level: slots are recycled across levels, not within one
                 std::vector<long> npair_seen(P.nvals, 0), ends(P.nvals, 0), pair_sum(P.nvals, 0), team_of(P.nvals, 0),
                     recip_of(P.nvals, 0);
                 const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
-                if (U < 1 || U > nslot || K < 1) ++conflicts;
+                const int Lc = lean ? lcs[l] : 1;
+                if (U < 1 || U > nslot || K < 1 || Lc < 1 || K % Lc != 0) { ++conflicts; continue; }
+                if (lean && U % fmt.lane_round != 0 && U != nslot) ++conflicts;
                 auto item = [&](int k, int s) { return &st[4 * ((size_t)base + (size_t)k * U + s)]; };
                 for (int s = 0; s < U; ++s) {
                     long c = 0, key = 0;
@@ -83,14 +94,18 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                             if (it[0] >= 0 || (e & kStreamEnd)) ++conflicts;
                             continue;
                         }
-                        if (it[0] >= 0) { ++c; key += (long)it[0] * 7 + (long)it[1] * 13 + (fwd ? (long)it[2] * 17 : 0); }
+                        const bool pad = lean ? (it[0] == zb && it[1] == zb && (!fwd || it[2] == zb)) : it[0] < 0;
+                        if (lean && !pad && (it[0] % sc || it[1] % sc || (fwd && it[2] % sc))) ++conflicts;
+                        if (!pad) { ++c; key += (long)(it[0] / sc) * 7 + (long)(it[1] / sc) * 13 + (fwd ? (long)(it[2] / sc) * 17 : 0); }
+                        if (lean && (k + 1) % Lc != 0 && e != 0) ++conflicts;  // an end off a chunk boundary
                         if (e & kStreamEnd) {
                             const int T = 1 << ((e >> kStreamTeamShift) & 7);
                             if (T * G > 64 || T > nslot) ++conflicts;
                             const int s0 = s & ~(T - 1);
                             for (int tp = 0; tp < T; ++tp)
                                 if (s0 + tp >= U || ((item(k, s0 + tp)[ecol] ^ e) & ~kStreamLast) != 0) ++conflicts;
-                            const int o = e & kStreamSlotMask;
+                            if (lean && (((e & kStreamSlotMask) % sc) || (!fwd && it[3] % sc))) ++conflicts;
+                            const int o = (e & kStreamSlotMask) / sc;
                             npair_seen[o] += c;
                             pair_sum[o] += key;
                             ends[o] += 1;
@@ -99,9 +114,11 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                             c = 0;
                             key = 0;
                         }
+                        if (lean && (k + 1) % Lc == 0 && !(e & kStreamEnd) && c != 0) ++conflicts;  // pairs of no task
+                        if (lean && (k + 1) % Lc == 0) c = 0, key = 0;
                         if (e & kStreamLast) nlast = 1;
                     }
-                    if (c != 0 || nlast != 1) ++conflicts;  // pairs after the last task end / no last flag
+                    if (c != 0 || nlast != (lean ? 0 : 1)) ++conflicts;  // pairs after the last task end / last flag
                 }
                 for (int t = lvl[l]; t < lvl[l + 1]; ++t) {
                     const int o = tasks[4 * t];
@@ -115,50 +132,8 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                 }
             }
         };
-        check_streams(LS.f_lv, LS.fs, P.nlev, true, P.tasks, P.f_lvl);
-        check_streams(LS.b_lv, LS.bs, P.nblev, false, P.btasks, P.b_lvl);
-        // the lean encoding (LDS byte offsets, zero-slot padding, lanes rounded to a wave)
-        // must hold exactly the items of the checked one
-        for (int esz : {4, 8}) {
-            LeanFormat fmt;
-            fmt.elem_bytes = G * esz;
-            fmt.lane_round = std::max(1, 64 / G);
-            fmt.k_round = 2;
-            const LaneStreams LL = build_lane_streams(P, G, nslot, &fmt);
-            const int sc = fmt.elem_bytes, zb = P.zslot * sc;
-            auto cmp = [&](const std::vector<int>& lv, const std::vector<int>& st, const std::vector<int>& llv,
-                           const std::vector<int>& lst, int nl, bool fwd) {
-                const int ecol = fwd ? 3 : 2;
-                for (int l = 0; l < nl; ++l) {
-                    const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
-                    const int lb = llv[4 * l], lK = llv[4 * l + 1], lU = llv[4 * l + 3];
-                    const int want = std::min(nslot, (U + fmt.lane_round - 1) / fmt.lane_round * fmt.lane_round);
-                    if (lK != (K + 1) / 2 * 2 || lU != want || llv[4 * l + 2] != lv[4 * l + 2]) { ++conflicts; continue; }
-                    for (int s = 0; s < lU; ++s) {
-                        bool after_last = s >= U;
-                        for (int k = 0; k < lK; ++k) {
-                            if (k >= K) after_last = true;
-                            const int* e = &lst[4 * ((size_t)lb + (size_t)k * lU + s)];
-                            if (after_last) {
-                                if (e[0] != zb || e[1] != zb || e[ecol] != 0) ++conflicts;
-                                continue;
-                            }
-                            const int* g = &st[4 * ((size_t)base + (size_t)k * U + s)];
-                            const int ge = g[ecol] & ~kStreamLast;
-                            const int np = fwd ? 3 : 2;
-                            for (int w = 0; w < np; ++w)
-                                if (e[w] != (g[0] >= 0 ? g[w] * sc : zb)) ++conflicts;
-                            const int want_e = (ge & kStreamEnd) ? (((ge & kStreamSlotMask) * sc) | (ge & ~kStreamSlotMask)) : 0;
-                            if (e[ecol] != want_e) ++conflicts;
-                            if (!fwd && (ge & kStreamEnd) && e[3] != g[3] * sc) ++conflicts;
-                            if (g[ecol] & kStreamLast) after_last = true;
-                        }
-                    }
-                }
-            };
-            cmp(LS.f_lv, LS.fs, LL.f_lv, LL.fs, P.nlev, true);
-            cmp(LS.b_lv, LS.bs, LL.b_lv, LL.bs, P.nblev, false);
-        }
+        check_streams(LS.f_lv, LS.f_lc, LS.fs, P.nlev, true, P.tasks, P.f_lvl);
+        check_streams(LS.b_lv, LS.b_lc, LS.bs, P.nblev, false, P.btasks, P.b_lvl);
         *conflicts_out = conflicts;
     }
 
@@ -208,6 +183,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
         // one level of a lane stream: every used lane accumulates its items up to its last
         // one; at a task's end item the team's partial sums are added and the leader writes
         auto run_level = [&](const std::vector<int>& st, const std::vector<int>& lv, int l, bool fwd) {
+            auto Xs = [&](int v) -> double& { return X[v / sc]; };  // slot of a stored operand
             const int base = lv[4 * l], K = lv[4 * l + 1], U = lv[4 * l + 3];
             std::fill(acc.begin(), acc.end(), 0.0);
             std::vector<char> more(U, 1);
@@ -217,8 +193,8 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                     if (!more[s]) continue;
                     const int* it = &st[4 * ((size_t)base + (size_t)k * U + s)];
                     if (it[0] < 0) continue;
-                    acc[s] = fwd ? rnd(acc[s] + rnd(rnd(X[it[0]] * X[it[1]]) * X[it[2]]))
-                                 : rnd(acc[s] + rnd(X[it[0]] * X[it[1]]));
+                    acc[s] = fwd ? rnd(acc[s] + rnd(rnd(Xs(it[0]) * Xs(it[1])) * Xs(it[2])))
+                                 : rnd(acc[s] + rnd(Xs(it[0]) * Xs(it[1])));
                 }
                 for (int s = 0; s < U; ++s) {
                     if (!more[s]) continue;
@@ -229,12 +205,12 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                         if ((s & (T - 1)) == 0) {
                             double sum = 0;
                             for (int tp = 0; tp < T; ++tp) { sum = rnd(sum + acc[s + tp]); acc[s + tp] = 0; }
-                            const int o = e & kStreamSlotMask;
+                            const int o = (e & kStreamSlotMask) / sc;
                             if (fwd) {
                                 const double v = rnd(((e & kStreamAssign) ? 0.0 : X[o]) - sum);
                                 X[o] = (e & kStreamRecip) ? rnd(1.0 / v) : v;
                             } else {
-                                X[o] = rnd(rnd(X[o] - sum) * X[it[3]]);
+                                X[o] = rnd(rnd(X[o] - sum) * Xs(it[3]));
                             }
                         }
                     }

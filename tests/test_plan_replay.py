@@ -13,7 +13,7 @@ _i32 = ctypes.POINTER(ctypes.c_int)
 _f64 = ctypes.POINTER(ctypes.c_double)
 
 
-def replay(d, single, fp32, max_iter=10, tol=1e-8, want_J=False, G=1, nslot=1024):
+def replay(d, single, fp32, max_iter=10, tol=1e-8, want_J=False, G=1, nslot=1024, lean=0):
     L = ctypes.CDLL(build_replay())
     Y = sp.csc_matrix(d["Ybus"])
     Y.sort_indices()
@@ -30,7 +30,7 @@ def replay(d, single, fp32, max_iter=10, tol=1e-8, want_J=False, G=1, nslot=1024
     it = L.replay_solve(n, Yp.ctypes.data_as(_i32), Yi.ctypes.data_as(_i32), Yx.ctypes.data_as(_f64),
                         S.ctypes.data_as(_f64), sl.ctypes.data_as(_i32), len(sl), w.ctypes.data_as(_f64),
                         pv.ctypes.data_as(_i32), len(pv), pq.ctypes.data_as(_i32), len(pq), int(single),
-                        V.ctypes.data_as(_f64), int(max_iter), ctypes.c_double(tol), int(fp32), int(G), int(nslot),
+                        V.ctypes.data_as(_f64), int(max_iter), ctypes.c_double(tol), int(fp32), int(G), int(nslot), int(lean),
                         ctypes.byref(st),
                         ctypes.byref(cf), None)
     return it, st.value, cf.value, V
@@ -53,14 +53,15 @@ SHAPES = [(1, 1024), (8, 128), (32, 8)]
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("fp32", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"G{s[0]}x{s[1]}")
-def test_replay_matches_oracle(oracle_mod, name, fp32, shape):
+@pytest.mark.parametrize("lean", [0, 1], ids=["general", "lean"])
+def test_replay_matches_oracle(oracle_mod, name, fp32, shape, lean):
     d = _data(name)
     singles = [False, True] if len(np.atleast_1d(d["slack_ids"])) == 1 else [False]
     for single in singles:
         o = oracle_mod.OracleNR(single)
         conv = o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"],
                             10, 1e-8)
-        it, st, conflicts, V = replay(d, single, fp32, G=shape[0], nslot=shape[1])
+        it, st, conflicts, V = replay(d, single, fp32, G=shape[0], nslot=shape[1], lean=lean)
         assert conflicts == 0
         assert (st == 0) == conv
         assert abs(it - o.get_nb_iter()) <= (0 if not fp32 else 1)
