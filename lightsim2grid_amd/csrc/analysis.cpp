@@ -1050,6 +1050,14 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
             }
         },
         2, bpad, L.b_lv, L.b_lc, L.bs);
+    // lean streams end with nslot padding items: a lane reads item base + k * U + s of a level
+    // without clamping s to the level's lanes (the waves of lanes s >= U skip the level), and
+    // the levels past the last one read at the end of the stream
+    if (lean)
+        for (int i = 0; i < nslot; ++i) {
+            L.fs.insert(L.fs.end(), fpad, fpad + 4);
+            L.bs.insert(L.bs.end(), bpad, bpad + 4);
+        }
     return L;
 }
 

@@ -195,7 +195,8 @@ struct LaneStreams {
 // smallest power of two with ceil(pairs / T) <= Lc), so every lane's task ends fall on
 // the items Lc - 1, 2 Lc - 1, ..., the same for all lanes, and the kernel accumulates
 // the other items without any test. The end word of a chunk of a lane without a task
-// is 0. Item 0 of a lean stream is a padding item, read in place of items past a level's end.
+// is 0. A lean stream starts with one padding item and ends with nslot of them, so that
+// lanes may read past their level's lanes and levels.
 struct LeanFormat {
     int elem_bytes = 4;  // bytes between consecutive slots of one instance (G * value size)
     int lane_round = 1;  // lanes per wave of one instance (64 / G)

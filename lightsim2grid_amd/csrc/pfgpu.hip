@@ -301,6 +301,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // {items per lane, used lanes} of every factor level, then of every backward level
     // (in LDS after the value slots)
     lds_int* lvK = (lds_int*)(lds_raw + a.lvk_off);
+    // the same table read through the scalar cache (uniform loads from the constant address space)
+    typedef __attribute__((address_space(4))) const int cst_int;
+    const cst_int* const lv_s = (const cst_int*)a.lv_K;
     for (int i = threadIdx.x; i < 2 * (a.nlev + a.nblev); i += NT) lvK[i] = a.lv_K[i];
 
     const int tid = threadIdx.x;
@@ -518,14 +521,17 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             // rest of a longer level is loaded in the level. Every lane issues every load: items
             // past a level's end read the padding item 0, lanes of a wave idle at a level read a
             // valid item of it they do not use.
-            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
+            // Level sizes come from scalar loads of the table in memory, and item addresses are
+            // a scalar base plus the lane (lanes past U read items they do not use; the stream
+            // ends with nslot padding items): the loop computes no address in a vector register,
+            // so no prefetch in flight is waited for before its level.
+            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? lv_s[2 * l + w] : 0; };
+            const int wave_s0 = (threadIdx.x & ~63) / G;
             auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
-                const int sl = min(s, U - 1);
 #pragma unroll
-                for (int u = 0; u < kHB; ++u) q[u] = a.fs[u < K ? (size_t)base + (size_t)u * U + sl : (size_t)0];
+                for (int u = 0; u < kHB; ++u) q[u] = a.fs[(size_t)base + (size_t)min(u, K - 1) * U + s];
             };
             FT acc = (FT)0;
-            const int wave_s0 = (threadIdx.x & ~63) / G;
             // end of a chunk: the team's lanes reduce, the leader updates the task's slot
             auto chunk_end = [&](const int e, const int T) {
                 const int lt = (e >> 26) & 7;
@@ -557,17 +563,16 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                                 kend += Lc;
                             }
                         }
-                    const int sl = min(s, U - 1);
                     for (int k = kHB; k < K; k += 2) {
-                        const int4 r0 = a.fs[(size_t)base + (size_t)k * U + sl];
-                        const int4 r1 = a.fs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
+                        const int4 r0 = a.fs[(size_t)base + (size_t)k * U + s];
+                        const int4 r1 = a.fs[(size_t)base + (size_t)min(k + 1, K - 1) * U + s];
                         const FT p0 = prod(r0), p1 = prod(r1);
                         acc += p0;
                         if (k == kend) {
                             chunk_end(r0.w, T);
                             kend += Lc;
                         }
-                        acc += p1;
+                        acc += k + 1 < K ? p1 : (FT)0;  // r1 repeats item K - 1 past the level's end
                         if (k + 1 == kend) {
                             chunk_end(r1.w, T);
                             kend += Lc;
@@ -794,15 +799,13 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         {
             if constexpr (LEAN) {
             // chunked items as in the factor walk; x = (x - sum u x') / d at a chunk's end
-            const lds_int* bK = lvK + 2 * a.nlev;
-            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
+            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? lv_s[2 * (a.nlev + l) + w] : 0; };
+            const int wave_s0 = (threadIdx.x & ~63) / G;
             auto load_head = [&](int base, int K, int U, int4 (&q)[kHB]) {
-                const int sl = min(s, U - 1);
 #pragma unroll
-                for (int u = 0; u < kHB; ++u) q[u] = a.bs[u < K ? (size_t)base + (size_t)u * U + sl : (size_t)0];
+                for (int u = 0; u < kHB; ++u) q[u] = a.bs[(size_t)base + (size_t)min(u, K - 1) * U + s];
             };
             FT acc = (FT)0;
-            const int wave_s0 = (threadIdx.x & ~63) / G;
             auto chunk_end = [&](const int4 q, const int T) {
                 const int e = q.z;
                 const int lt = (e >> 26) & 7;
@@ -829,17 +832,16 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                                 kend += Lc;
                             }
                         }
-                    const int sl = min(s, U - 1);
                     for (int k = kHB; k < K; k += 2) {
-                        const int4 r0 = a.bs[(size_t)base + (size_t)k * U + sl];
-                        const int4 r1 = a.bs[k + 1 < K ? (size_t)base + (size_t)(k + 1) * U + sl : (size_t)0];
+                        const int4 r0 = a.bs[(size_t)base + (size_t)k * U + s];
+                        const int4 r1 = a.bs[(size_t)base + (size_t)min(k + 1, K - 1) * U + s];
                         const FT p0 = prod(r0), p1 = prod(r1);
                         acc += p0;
                         if (k == kend) {
                             chunk_end(r0, T);
                             kend += Lc;
                         }
-                        acc += p1;
+                        acc += k + 1 < K ? p1 : (FT)0;  // r1 repeats item K - 1 past the level's end
                         if (k + 1 == kend) {
                             chunk_end(r1, T);
                             kend += Lc;
