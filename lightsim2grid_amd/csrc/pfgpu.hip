@@ -43,6 +43,8 @@ constexpr int kMaxBlock = 1024;
 constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
 constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
+constexpr int kEll = 4;  // Ybus row entries per row in the column-major mismatch table
+constexpr int kMR = 1;   // mismatch rows per lane at a time
 typedef __attribute__((address_space(3))) double lds_dbl;
 typedef __attribute__((address_space(3))) int lds_int;
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
@@ -96,6 +98,7 @@ struct KArgs {
     double tol;
     const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
     const int2* yr_pc;     // per Ybus row entry: {CSC position, column bus}
+    const int2* yr_ell;    // [kEll][n]: the first kEll entries of every row as in yr_pc, {-1, row} past its end
     // J fill by Ybus entry: position, {row bus, column bus}, 4 target slots, 4 J indices
     const int* ye_pos;
     const int2* ye_rc;
@@ -405,35 +408,52 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     }
 
     // ---- mismatch phase ------------------------------------------------------
+    // Ibus = Y V by rows. The first kEll entries of every row come from a column-major table
+    // (no row-pointer load before them, coalesced), the rest of a longer row from the CSR
+    // row view; kMR rows per lane at a time, all their loads issued before the arithmetic.
     auto mismatch = [&]() -> double {
         double fm = 0.0;
         if (valid && !s_done[g]) {
             const double sa = s_slack[g];
-            for (int r = s; r < n; r += nslot) {
-                c64 acc = {0.0, 0.0};
-                const int q1 = a.yr_ptr[r + 1];
-                // 4 entries' loads at a time, summed in row order
-                for (int q = a.yr_ptr[r]; q < q1; q += 4) {
-                    int2 pc[4];
+            for (int r0 = s; r0 < n; r0 += kMR * nslot) {
+                int2 pc[kMR][kEll];
+                int q0[kMR], q1[kMR];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) pc[u] = a.yr_pc[q + u < q1 ? q + u : q];
-                    c64 yv[4], vv[4];
+                for (int j = 0; j < kMR; ++j) {
+                    const int r = min(r0 + j * nslot, n - 1);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        yv[u] = Yat(pc[u].x);
-                        vv[u] = ld_c(V, pc[u].y);
+                    for (int k = 0; k < kEll; ++k) pc[j][k] = a.yr_ell[k * n + r];
+                    q0[j] = a.yr_ptr[r] + kEll;
+                    q1[j] = a.yr_ptr[r + 1];
+                }
+                c64 yv[kMR][kEll], vv[kMR][kEll];
+#pragma unroll
+                for (int j = 0; j < kMR; ++j)
+#pragma unroll
+                    for (int k = 0; k < kEll; ++k) {
+                        yv[j][k] = Yat(max(pc[j][k].x, 0));
+                        vv[j][k] = ld_c(V, pc[j][k].y);
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (q + u < q1) acc = cadd(acc, cmul(yv[u], vv[u]));
+                for (int j = 0; j < kMR; ++j) {
+                    const int r = r0 + j * nslot;
+                    if (r >= n) break;
+                    c64 acc = {0.0, 0.0};  // summed in row order
+#pragma unroll
+                    for (int k = 0; k < kEll; ++k)
+                        if (pc[j][k].x >= 0) acc = cadd(acc, cmul(yv[j][k], vv[j][k]));
+                    for (int q = q0[j]; q < q1[j]; ++q) {
+                        const int2 e = a.yr_pc[q];
+                        acc = cadd(acc, cmul(Yat(e.x), ld_c(V, e.y)));
+                    }
+                    Ib[2 * r] = acc.re;
+                    Ib[2 * r + 1] = acc.im;
+                    c64 mis = csub(cmul(ld_c(V, r), cconj(acc)), ld_c(S, r));
+                    if (!a.single_slack) mis.re += sa * a.w[r];
+                    const int fp = a.fslot_p[r], fq = a.fslot_q[r];
+                    if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
+                    if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
                 }
-                Ib[2 * r] = acc.re;
-                Ib[2 * r + 1] = acc.im;
-                c64 mis = csub(cmul(ld_c(V, r), cconj(acc)), ld_c(S, r));
-                if (!a.single_slack) mis.re += sa * a.w[r];
-                const int fp = a.fslot_p[r], fq = a.fslot_q[r];
-                if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
-                if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
             }
         }
         return inst_reduce<G, NT, true>(fm, s_part);
@@ -481,26 +501,48 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             };
             for (int i = s; i < a.nzero; i += nslot) XS(a.zero_slots[i], (FT)0);
             for (int i = s; i < a.njc; i += nslot) put(a.jc_slot[i], a.jc_jidx[i], a.jc_val[i]);
-            for (int e = s; e < a.nye; e += nslot) {
-                const int pos = a.ye_pos[e];
-                const int2 rc = a.ye_rc[e];
-                const int4 sl = a.ye_slot[e];
-                const c64 y = Yat(pos), vr = ld_c(V, rc.x), vc = ld_c(V, rc.y), vnc = ld_c(Vn, rc.y);
-                const bool diag = rc.x == rc.y;
-                c64 t = cmul(y, vc);
-                c64 dvm = cmul(cconj(cmul(y, vnc)), vr);
-                if (diag) {
-                    const c64 ib = ld_c(Ib, rc.x);
-                    t = csub(t, ib);
-                    dvm = cadd(dvm, cmul(cconj(ib), vnc));
+            // kJE entries per lane at a time: the table loads of all of them, then their Y, V,
+            // V/|V| and Ibus loads, then the arithmetic (two memory round trips per batch)
+            constexpr int kJE = 2;
+            for (int e0 = s; e0 < a.nye; e0 += kJE * nslot) {
+                int pos[kJE];
+                int2 rc[kJE];
+                int4 sl[kJE];
+#pragma unroll
+                for (int u = 0; u < kJE; ++u) {
+                    const int e = min(e0 + u * nslot, a.nye - 1);
+                    pos[u] = a.ye_pos[e];
+                    rc[u] = a.ye_rc[e];
+                    sl[u] = a.ye_slot[e];
                 }
-                const c64 dva = cmul(cconj({-t.re, -t.im}), {-vr.im, vr.re});  // * (j V_r)
-                int4 jx = {0, 0, 0, 0};
-                if (a.Jx_last != nullptr && inst == 0) jx = a.ye_jidx[e];
-                put(sl.x, jx.x, dva.re);
-                put(sl.y, jx.y, dva.im);
-                put(sl.z, jx.z, dvm.re);
-                put(sl.w, jx.w, dvm.im);
+                c64 y[kJE], vr[kJE], vc[kJE], vnc[kJE], ib[kJE];
+#pragma unroll
+                for (int u = 0; u < kJE; ++u) {
+                    y[u] = Yat(pos[u]);
+                    vr[u] = ld_c(V, rc[u].x);
+                    vc[u] = ld_c(V, rc[u].y);
+                    vnc[u] = ld_c(Vn, rc[u].y);
+                    ib[u] = rc[u].x == rc[u].y ? ld_c(Ib, rc[u].x) : c64{0.0, 0.0};
+                }
+#pragma unroll
+                for (int u = 0; u < kJE; ++u) {
+                    const int e = e0 + u * nslot;
+                    if (e >= a.nye) break;
+                    const bool diag = rc[u].x == rc[u].y;
+                    c64 t = cmul(y[u], vc[u]);
+                    c64 dvm = cmul(cconj(cmul(y[u], vnc[u])), vr[u]);
+                    if (diag) {
+                        t = csub(t, ib[u]);
+                        dvm = cadd(dvm, cmul(cconj(ib[u]), vnc[u]));
+                    }
+                    const c64 dva = cmul(cconj({-t.re, -t.im}), {-vr[u].im, vr[u].re});  // * (j V_r)
+                    int4 jx = {0, 0, 0, 0};
+                    if (a.Jx_last != nullptr && inst == 0) jx = a.ye_jidx[e];
+                    put(sl[u].x, jx.x, dva.re);
+                    put(sl[u].y, jx.y, dva.im);
+                    put(sl[u].z, jx.z, dvm.re);
+                    put(sl[u].w, jx.w, dvm.im);
+                }
             }
         }
         __syncthreads();
@@ -1088,7 +1130,7 @@ struct pfgpu_grid {
     pfgpu::DevicePlan P;
     pfgpu_grid_info info{};
     DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots;
-    DevBuf<int> yr_pc, ye_pos, ye_rc, ye_slot, ye_jidx, jc_slot, jc_jidx;
+    DevBuf<int> yr_pc, yr_ell, ye_pos, ye_rc, ye_slot, ye_jidx, jc_slot, jc_jidx;
     DevBuf<double> jc_val;
     DevBuf<int8_t> ux_kind;
     DevBuf<uint8_t> j_sel;
@@ -1333,6 +1375,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
     a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
     a.yr_pc = reinterpret_cast<const int2*>(gr->yr_pc.p);
+    a.yr_ell = reinterpret_cast<const int2*>(gr->yr_ell.p);
     a.ye_pos = gr->ye_pos.p; a.ye_rc = reinterpret_cast<const int2*>(gr->ye_rc.p);
     a.ye_slot = reinterpret_cast<const int4*>(gr->ye_slot.p); a.ye_jidx = reinterpret_cast<const int4*>(gr->ye_jidx.p);
     a.nye = (int)P.ye_pos.size();
@@ -1480,6 +1523,16 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
             std::vector<int> pc(2 * P.yr_pos.size());
             for (size_t k = 0; k < P.yr_pos.size(); ++k) { pc[2 * k] = P.yr_pos[k]; pc[2 * k + 1] = P.yr_col[k]; }
             g->yr_pc.upload(pc.empty() ? std::vector<int>(2, 0) : pc);
+            const int n = (int)P.yr_ptr.size() - 1;
+            std::vector<int> ell(2 * (size_t)kEll * std::max(n, 1), 0);
+            for (int r = 0; r < n; ++r)
+                for (int k = 0; k < kEll; ++k) {
+                    const int q = P.yr_ptr[r] + k;
+                    const bool in = q < P.yr_ptr[r + 1];
+                    ell[2 * ((size_t)k * n + r)] = in ? P.yr_pos[q] : -1;
+                    ell[2 * ((size_t)k * n + r) + 1] = in ? P.yr_col[q] : r;
+                }
+            g->yr_ell.upload(ell);
         }
         auto up_or = [](auto& d, const auto& v, size_t minsz) {
             auto w = v;
