@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -43,8 +44,7 @@ constexpr int kMaxBlock = 1024;
 constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
 constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
-constexpr int kEll = 4;  // Ybus row entries per row in the column-major mismatch table
-constexpr int kMR = 1;   // mismatch rows per lane at a time
+constexpr int kEll = 4;  // Ybus row entries read at a time in the mismatch (column groups of its table)
 typedef __attribute__((address_space(3))) double lds_dbl;
 typedef __attribute__((address_space(3))) int lds_int;
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
@@ -98,7 +98,7 @@ struct KArgs {
     double tol;
     const int *yr_ptr, *yr_col, *yr_pos, *fslot_p, *fslot_q, *ux_bus, *ux_slot;
     const int2* yr_pc;     // per Ybus row entry: {CSC position, column bus}
-    const int2* yr_ell;    // [kEll][n]: the first kEll entries of every row as in yr_pc, {-1, row} past its end
+    const int2* yr_ell;    // [ell_w][n]: the entries of every row as in yr_pc, {-1, row} past its end
     // J fill by Ybus entry: position, {row bus, column bus}, 4 target slots, 4 J indices
     const int* ye_pos;
     const int2* ye_rc;
@@ -124,6 +124,9 @@ struct KArgs {
     const int4* bs;      // {u, x', end, 1/d}: u < 0 no pair; end >= 0: slot | log2(team) << 26
     int lvk_off;         // byte offset of the level table in dynamic LDS
     int sh_off;          // byte offset of the per-workgroup state in dynamic LDS
+    int v_off;           // lean launches: byte offset of the instances' V in dynamic LDS ([bus][G] complex)
+    const double* Yell;  // Ybus values in yr_ell order ([ell_w][n] complex, 0 past a row's end), or null
+    const double* Yye;   // Ybus values in ye_pos order, or null
     const double* w;
     const double* Yx;
     long long y_stride;  // doubles between instances (0 = shared)
@@ -203,24 +206,27 @@ __device__ __forceinline__ double inf_if_nonfinite(double a) { return (a <= 1.79
 // count past 128 VGPRs, which at 1024 lanes per workgroup means scratch spills
 // in the factor loop.
 // Va = arg(V), Vm = |V| (BaseNRAlgo.tpp:77-79, BaseNRSingleSlackAlgo.tpp:131-132)
-__device__ __noinline__ void polar_of(const double* __restrict__ V, double* __restrict__ Va,
-                                      double* __restrict__ Vm, int b0, int n, int step) {
+// V holds bus b at V[2 b vs] (vs = 1 in global memory, G in LDS).
+template <typename VP>
+__device__ __noinline__ void polar_of(VP V, int vs, double* __restrict__ Va, double* __restrict__ Vm, int b0, int n,
+                                      int step) {
     for (int b = b0; b < n; b += step) {
-        const double vr = V[2 * b], vi = V[2 * b + 1];
+        const double vr = V[2 * b * vs], vi = V[2 * b * vs + 1];
         Va[b] = atan2(vi, vr);
         Vm[b] = sqrt(vr * vr + vi * vi);
     }
 }
 // V = Vm * exp(j Va) (BaseNRAlgo.tpp:150-151); returns min Vm over the lane's buses
-__device__ __noinline__ double rect_of(double* __restrict__ V, const double* __restrict__ Va,
-                                       const double* __restrict__ Vm, int b0, int n, int step) {
+template <typename VP>
+__device__ __noinline__ double rect_of(VP V, int vs, const double* __restrict__ Va, const double* __restrict__ Vm,
+                                       int b0, int n, int step) {
     double vmin = INFINITY;
     for (int b = b0; b < n; b += step) {
         double sn, cs;
         sincos(Va[b], &sn, &cs);
         const double m = Vm[b];
-        V[2 * b] = m * cs;
-        V[2 * b + 1] = m * sn;
+        V[2 * b * vs] = m * cs;
+        V[2 * b * vs + 1] = m * sn;
         vmin = fmin(vmin, m);
     }
     return vmin;
@@ -348,6 +354,18 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     double* Vm = a.Vm + ii * (long long)n;
     double* Ib = a.Ib + ii * 2 * (long long)n;
     double* Vn = a.Vn + ii * 2 * (long long)n;
+    // V of this instance: in LDS for lean launches (loaded here, stored back at the end), else global
+    lds_dbl* const Vl = (lds_dbl*)(lds_raw + a.v_off) + 2 * g;
+    auto ldV = [&](int b) -> c64 {
+        if constexpr (LEAN) return {Vl[2 * b * G], Vl[2 * b * G + 1]};
+        else return ld_c(V, b);
+    };
+    if constexpr (LEAN)
+        if (valid)
+            for (int b = s; b < n; b += nslot) {
+                Vl[2 * b * G] = V[2 * b];
+                Vl[2 * b * G + 1] = V[2 * b + 1];
+            }
     // this instance's Ybus edits: the first kMaxEdits in LDS, any further ones read from memory
     if (s == 0) {
         s_nd[g] = 0;
@@ -366,14 +384,24 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     }
     const int nd = s_nd[g], d0 = s_d0[g];
     // Ybus value at CSC position pos for this instance
-    auto Yat = [&](int pos) -> c64 {
-        c64 y = ld_c(Y, pos);
-        const int nl = min(nd, kMaxEdits);
-        for (int d = 0; d < nl; ++d)
-            if (s_dpos[g * kMaxEdits + d] == pos) {
-                y.re -= s_dval[(g * kMaxEdits + d) * 2];
-                y.im -= s_dval[(g * kMaxEdits + d) * 2 + 1];
-            }
+    // The first kMaxEdits edit positions are read into registers at the start of a phase
+    // (edit_pos), so that an entry is checked against them with compares only.
+    typedef int EditPos[kMaxEdits];
+    auto edit_pos = [&](EditPos& dp) {
+#pragma unroll
+        for (int d = 0; d < kMaxEdits; ++d) dp[d] = d < nd ? s_dpos[g * kMaxEdits + d] : INT_MIN;
+    };
+    auto edits = [&](c64 y, int pos, const EditPos& dp) -> c64 {
+        bool hit = false;
+#pragma unroll
+        for (int d = 0; d < kMaxEdits; ++d) hit |= dp[d] == pos;
+        if (hit)
+#pragma unroll
+            for (int d = 0; d < kMaxEdits; ++d)
+                if (dp[d] == pos) {
+                    y.re -= s_dval[(g * kMaxEdits + d) * 2];
+                    y.im -= s_dval[(g * kMaxEdits + d) * 2 + 1];
+                }
         for (int d = kMaxEdits; d < nd; ++d)
             if (a.dpos[d0 + d] == pos) {
                 y.re -= a.dval[2 * (d0 + d)];
@@ -381,11 +409,17 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         return y;
     };
+    auto Yat = [&](int pos, const EditPos& dp) -> c64 { return edits(ld_c(Y, pos), pos, dp); };
+    // Ybus values gathered in the order of the mismatch / J-fill tables (shared Ybus only):
+    // loaded alongside the table entries instead of after them
+    const double* const Yell = a.Yell;
+    const double* const Yye = a.Yye;
 
     // ---- init: Va/Vm from V, slack_absorbed = Re(sum Sbus) (BaseNRAlgo.tpp:60,77-79)
     double ssum = 0.0;
     if (valid) {
-        polar_of(V, Va, Vm, s, n, nslot);
+        if constexpr (LEAN) polar_of(Vl, G, Va, Vm, s, n, nslot);
+        else polar_of(V, 1, Va, Vm, s, n, nslot);
         for (int b = s; b < n; b += nslot) ssum += S[2 * b];
     }
     // sum-reduce (as max of a single value is not a sum, do a dedicated pass)
@@ -408,52 +442,39 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     }
 
     // ---- mismatch phase ------------------------------------------------------
-    // Ibus = Y V by rows. The first kEll entries of every row come from a column-major table
-    // (no row-pointer load before them, coalesced), the rest of a longer row from the CSR
-    // row view; kMR rows per lane at a time, all their loads issued before the arithmetic.
+    // Ibus = Y V by rows, from a column-major table of every row's entries (ell_w columns,
+    // padded with {-1, row}): a row's entries are read kEll at a time with no row-pointer
+    // load before them, coalesced across lanes, and summed in row order.
     auto mismatch = [&]() -> double {
         double fm = 0.0;
         if (valid && !s_done[g]) {
             const double sa = s_slack[g];
-            for (int r0 = s; r0 < n; r0 += kMR * nslot) {
-                int2 pc[kMR][kEll];
-                int q0[kMR], q1[kMR];
+            EditPos dp;
+            edit_pos(dp);
+            for (int r = s; r < n; r += nslot) {
+                c64 acc = {0.0, 0.0};
+                const int len = a.yr_ptr[r + 1] - a.yr_ptr[r];
+                for (int k0 = 0; k0 == 0 || k0 < len; k0 += kEll) {
+                    int2 pc[kEll];
 #pragma unroll
-                for (int j = 0; j < kMR; ++j) {
-                    const int r = min(r0 + j * nslot, n - 1);
-#pragma unroll
-                    for (int k = 0; k < kEll; ++k) pc[j][k] = a.yr_ell[k * n + r];
-                    q0[j] = a.yr_ptr[r] + kEll;
-                    q1[j] = a.yr_ptr[r + 1];
-                }
-                c64 yv[kMR][kEll], vv[kMR][kEll];
-#pragma unroll
-                for (int j = 0; j < kMR; ++j)
+                    for (int k = 0; k < kEll; ++k) pc[k] = a.yr_ell[(size_t)(k0 + k) * n + r];
+                    c64 yv[kEll], vv[kEll];
 #pragma unroll
                     for (int k = 0; k < kEll; ++k) {
-                        yv[j][k] = Yat(max(pc[j][k].x, 0));
-                        vv[j][k] = ld_c(V, pc[j][k].y);
+                        yv[k] = Yell ? edits(ld_c(Yell, (k0 + k) * n + r), pc[k].x, dp) : Yat(max(pc[k].x, 0), dp);
+                        vv[k] = ldV(pc[k].y);
                     }
-#pragma unroll
-                for (int j = 0; j < kMR; ++j) {
-                    const int r = r0 + j * nslot;
-                    if (r >= n) break;
-                    c64 acc = {0.0, 0.0};  // summed in row order
 #pragma unroll
                     for (int k = 0; k < kEll; ++k)
-                        if (pc[j][k].x >= 0) acc = cadd(acc, cmul(yv[j][k], vv[j][k]));
-                    for (int q = q0[j]; q < q1[j]; ++q) {
-                        const int2 e = a.yr_pc[q];
-                        acc = cadd(acc, cmul(Yat(e.x), ld_c(V, e.y)));
-                    }
-                    Ib[2 * r] = acc.re;
-                    Ib[2 * r + 1] = acc.im;
-                    c64 mis = csub(cmul(ld_c(V, r), cconj(acc)), ld_c(S, r));
-                    if (!a.single_slack) mis.re += sa * a.w[r];
-                    const int fp = a.fslot_p[r], fq = a.fslot_q[r];
-                    if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
-                    if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
+                        if (pc[k].x >= 0) acc = cadd(acc, cmul(yv[k], vv[k]));
                 }
+                Ib[2 * r] = acc.re;
+                Ib[2 * r + 1] = acc.im;
+                c64 mis = csub(cmul(ldV(r), cconj(acc)), ld_c(S, r));
+                if (!a.single_slack) mis.re += sa * a.w[r];
+                const int fp = a.fslot_p[r], fq = a.fslot_q[r];
+                if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
+                if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
             }
         }
         return inst_reduce<G, NT, true>(fm, s_part);
@@ -482,7 +503,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // up to 4 J entries built on it; V / |V| per bus first)
         if (act)
             for (int b = s; b < n; b += nslot) {
-                const double vr = V[2 * b], vi = V[2 * b + 1];
+                const c64 v = ldV(b);
+                const double vr = v.re, vi = v.im;
                 const double ic = 1.0 / sqrt(vr * vr + vi * vi);
                 Vn[2 * b] = vr * ic;
                 Vn[2 * b + 1] = vi * ic;
@@ -504,6 +526,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             // kJE entries per lane at a time: the table loads of all of them, then their Y, V,
             // V/|V| and Ibus loads, then the arithmetic (two memory round trips per batch)
             constexpr int kJE = 2;
+            EditPos dp;
+            edit_pos(dp);
             for (int e0 = s; e0 < a.nye; e0 += kJE * nslot) {
                 int pos[kJE];
                 int2 rc[kJE];
@@ -518,9 +542,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 c64 y[kJE], vr[kJE], vc[kJE], vnc[kJE], ib[kJE];
 #pragma unroll
                 for (int u = 0; u < kJE; ++u) {
-                    y[u] = Yat(pos[u]);
-                    vr[u] = ld_c(V, rc[u].x);
-                    vc[u] = ld_c(V, rc[u].y);
+                    y[u] = Yye ? edits(ld_c(Yye, min(e0 + u * nslot, a.nye - 1)), pos[u], dp) : Yat(pos[u], dp);
+                    vr[u] = ldV(rc[u].x);
+                    vc[u] = ldV(rc[u].y);
                     vnc[u] = ld_c(Vn, rc[u].y);
                     ib[u] = rc[u].x == rc[u].y ? ld_c(Ib, rc[u].x) : c64{0.0, 0.0};
                 }
@@ -1018,7 +1042,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         }
 
         // ---- voltage update ------------------------------------------------------------
-        if (act && a.single_slack) polar_of(V, Va, Vm, s, n, nslot);  // BaseNRSingleSlackAlgo.tpp:131-132
+        if (act && a.single_slack) {  // BaseNRSingleSlackAlgo.tpp:131-132
+            if constexpr (LEAN) polar_of(Vl, G, Va, Vm, s, n, nslot);
+            else polar_of(V, 1, Va, Vm, s, n, nslot);
+        }
         __syncthreads();
         if (act)
             for (int j = s; j < a.N; j += nslot) {
@@ -1030,9 +1057,15 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         __syncthreads();
         double vmin = INFINITY;
-        if (act) vmin = rect_of(V, Va, Vm, s, n, nslot);
+        if (act) {
+            if constexpr (LEAN) vmin = rect_of(Vl, G, Va, Vm, s, n, nslot);
+            else vmin = rect_of(V, 1, Va, Vm, s, n, nslot);
+        }
         vmin = inst_reduce<G, NT, false>(vmin, s_part);
-        if (act && vmin < 0.0) polar_of(V, Va, Vm, s, n, nslot);
+        if (act && vmin < 0.0) {
+            if constexpr (LEAN) polar_of(Vl, G, Va, Vm, s, n, nslot);
+            else polar_of(V, 1, Va, Vm, s, n, nslot);
+        }
         __syncthreads();
 
         // ---- mismatch + convergence ----------------------------------------------------
@@ -1044,6 +1077,12 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         }
         __syncthreads();
     }
+    if constexpr (LEAN)
+        if (valid)
+            for (int b = s; b < n; b += nslot) {
+                V[2 * b] = Vl[2 * b * G];
+                V[2 * b + 1] = Vl[2 * b * G + 1];
+            }
     if (s == 0 && valid) {
         a.iters[inst] = s_it[g];
         a.status[inst] = s_status[g];
@@ -1051,6 +1090,21 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 }
 
 // Per-instance initial V (broadcast or per instance).
+// Ybus values in the order of the mismatch table (0 past a row's end) and of the J-fill table.
+__global__ void k_gather_y(const double* __restrict__ Y, const int2* __restrict__ ell, int nell,
+                           const int* __restrict__ ye_pos, int nye, double* __restrict__ Yell, double* __restrict__ Yye) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nell) {
+        const int p = ell[i].x;
+        Yell[2 * i] = p >= 0 ? Y[2 * p] : 0.0;
+        Yell[2 * i + 1] = p >= 0 ? Y[2 * p + 1] : 0.0;
+    } else if (i < nell + nye) {
+        const int e = i - nell, p = ye_pos[e];
+        Yye[2 * e] = Y[2 * p];
+        Yye[2 * e + 1] = Y[2 * p + 1];
+    }
+}
+
 __global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n, int ninst) {
     const long long total = (long long)ninst * n;
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
@@ -1131,6 +1185,7 @@ struct pfgpu_grid {
     pfgpu_grid_info info{};
     DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots;
     DevBuf<int> yr_pc, yr_ell, ye_pos, ye_rc, ye_slot, ye_jidx, jc_slot, jc_jidx;
+    int ell_w = 0;  // columns of yr_ell
     DevBuf<double> jc_val;
     DevBuf<int8_t> ux_kind;
     DevBuf<uint8_t> j_sel;
@@ -1190,6 +1245,7 @@ struct LaunchChoice {
     size_t lds;      // dynamic LDS bytes: value slots, the level table, the workgroup state
     size_t lvk_off;  // byte offset of the level table
     size_t sh_off;   // byte offset of the workgroup state
+    size_t v_off;    // lean launches: byte offset of the instances' V
 };
 
 template <typename FT, int NT, bool LEAN>
@@ -1250,14 +1306,14 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     } else {
         c.G = 1;
         for (int G = kMaxG; G >= 2; G >>= 1) {
-            const size_t need = (size_t)P.nvals * G * fsize;
+            const size_t need = (size_t)P.nvals * G * fsize + (size_t)P.n * G * 16;
             const long long blocks = (ninst + G - 1) / G;
             if (need <= avail(G, 1024) && blocks >= 2LL * std::max(1, ctx->n_cu)) { c.G = G; break; }
         }
         if (c.G == 1) {
             // small batches: still allow G>1 if it fits and there is more than one instance
             for (int G = std::min(kMaxG, 8); G >= 2; G >>= 1) {
-                const size_t need = (size_t)P.nvals * G * fsize;
+                const size_t need = (size_t)P.nvals * G * fsize + (size_t)P.n * G * 16;
                 if (need <= avail(G, 1024) && ninst >= 4 * G * std::max(1, ctx->n_cu)) { c.G = G; break; }
             }
         }
@@ -1273,14 +1329,16 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
     if (c.nlds < std::max(1, P.m * P.m + P.m))
         throw ApiError(PFGPU_E_INVALID, "the grid's dense tail does not fit in LDS");
     c.nspill = P.nvals - c.nlds;
-    c.lean = c.nspill == 0;
+    c.lvk_off = align16((size_t)c.nlds * c.G * fsize);
+    c.sh_off = c.lvk_off + table;
+    c.v_off = c.sh_off + align16((size_t)wg_state_bytes(c.G, c.NT));
+    // lean: every value slot in LDS, and the instances' V beside them
+    c.lean = c.nspill == 0 && c.v_off + (size_t)P.n * c.G * 16 <= (size_t)ctx->lds_max;
     // lean items carry LDS addresses: the value slots must start at LDS address 0
     if (c.lean && static_lds(fsize, c.G, c.NT) != 0)
         throw ApiError(PFGPU_E_INTERNAL, "the NR kernel has static LDS: lean item offsets would not be LDS addresses");
     c.nblocks = (ninst + c.G - 1) / c.G;
-    c.lvk_off = align16((size_t)c.nlds * c.G * fsize);
-    c.sh_off = c.lvk_off + table;
-    c.lds = c.sh_off + align16((size_t)wg_state_bytes(c.G, c.NT));
+    c.lds = c.lean ? c.v_off + (size_t)P.n * c.G * 16 : c.v_off;
     return c;
 }
 
@@ -1329,7 +1387,7 @@ void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Vn][Xg].
+// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Vn][Xg][Ybus gathered].
 void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
                   const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
                   int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
@@ -1357,6 +1415,10 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     const size_t o_ib = off; off += align_up((size_t)ninst * n * 16);
     const size_t o_vn = off; off += align_up((size_t)ninst * n * 16);
     const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
+    // shared Ybus: its values gathered in the order of the mismatch and J-fill tables
+    const bool gather = Yx_stride == 0;
+    const size_t o_yell = off; off += gather ? align_up((size_t)gr->ell_w * n * 16) : 0;
+    const size_t o_yye = off; off += gather ? align_up(P.ye_pos.size() * 16) : 0;
     char* ws = static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
     hipStream_t stream = ctx->stream;
     // the kernel applies the per-instance Ybus edits (deltas) to the shared base itself
@@ -1395,7 +1457,16 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.V = d_V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
     a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
+    a.v_off = (int)lc.v_off;
+    a.Yell = gather ? reinterpret_cast<const double*>(ws + o_yell) : nullptr;
+    a.Yye = gather ? reinterpret_cast<const double*>(ws + o_yye) : nullptr;
     HIP_OK(hipEventRecord(ctx->ev0, stream));
+    if (gather) {
+        const int nell = gr->ell_w * n, nye = (int)P.ye_pos.size();
+        hipLaunchKernelGGL(k_gather_y, dim3((nell + nye + 255) / 256), dim3(256), 0, stream, d_Yx, a.yr_ell, nell,
+                           a.ye_pos, nye, reinterpret_cast<double*>(ws + o_yell), reinterpret_cast<double*>(ws + o_yye));
+        HIP_OK(hipGetLastError());
+    }
     if (precision == PFGPU_FP32) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
     else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
     HIP_OK(hipEventRecord(ctx->ev1, stream));
@@ -1523,10 +1594,15 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
             std::vector<int> pc(2 * P.yr_pos.size());
             for (size_t k = 0; k < P.yr_pos.size(); ++k) { pc[2 * k] = P.yr_pos[k]; pc[2 * k + 1] = P.yr_col[k]; }
             g->yr_pc.upload(pc.empty() ? std::vector<int>(2, 0) : pc);
+            // column-major row table, width the longest row rounded up to kEll columns
             const int n = (int)P.yr_ptr.size() - 1;
-            std::vector<int> ell(2 * (size_t)kEll * std::max(n, 1), 0);
+            int w = 1;
+            for (int r = 0; r < n; ++r) w = std::max(w, P.yr_ptr[r + 1] - P.yr_ptr[r]);
+            w = (w + kEll - 1) / kEll * kEll;
+            g->ell_w = w;
+            std::vector<int> ell(2 * (size_t)w * std::max(n, 1), 0);
             for (int r = 0; r < n; ++r)
-                for (int k = 0; k < kEll; ++k) {
+                for (int k = 0; k < w; ++k) {
                     const int q = P.yr_ptr[r] + k;
                     const bool in = q < P.yr_ptr[r + 1];
                     ell[2 * ((size_t)k * n + r)] = in ? P.yr_pos[q] : -1;

@@ -98,6 +98,29 @@ def test_contingency_batch_matches_oracle(gpu_ctx, oracle_mod, grid_name, max_co
     assert np.max(np.abs(r["V"][convr] - Vr[convr])) <= vtol(prec)
 
 
+@pytest.mark.parametrize("grid_name,max_cont", [("case118_synth", 64), ("case2000_synth", 48)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_per_instance_ybus_matches_deltas(gpu_ctx, grid_name, max_cont, prec):
+    """The same outages given as per-instance Ybus values (Yx of shape (ninst, nnz), the
+    values read in place) and as edits of a shared Ybus (read through its gathered copies):
+    identical statuses, iterations and voltages."""
+    from lightsim2grid_amd import abi
+    g, conts, (dptr, dpos, dval) = _n1(grid_name, max_cont)
+    d = g.solver_inputs()
+    _say("per-instance Ybus", grid_name, len(conts), "fp32" if prec else "fp64")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    ref = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8, precision=prec, deltas=(dptr, dpos, dval))
+    Yb = np.repeat(Yx[None, :], len(conts), axis=0)
+    for i in range(len(conts)):
+        for k in range(dptr[i], dptr[i + 1]):
+            Yb[i, dpos[k]] -= dval[k]
+    r = gr.solve(Yb, d["Sbus"], d["V0"], 10, 1e-8, precision=prec)
+    assert np.array_equal(r["status"], ref["status"])
+    assert np.array_equal(r["iters"], ref["iters"])
+    assert np.max(np.abs(r["V"] - ref["V"])) <= 1e-12
+
+
 @pytest.mark.parametrize("G", [1, 2, 4, 8, 16, 32])
 def test_instances_per_workgroup_do_not_change_results(gpu_ctx, G):
     from lightsim2grid_amd import abi
