@@ -112,7 +112,6 @@ struct KArgs {
     // [dptr[i], dptr[i+1]) of instance i (dptr == nullptr: no edits)
     const int *dptr, *dpos;
     const double* dval;
-    double* Vn;          // [ninst][n] complex: V / |V|, refreshed before each J fill
     const int8_t* ux_kind;
     const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
@@ -353,7 +352,6 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     double* Va = a.Va + ii * (long long)n;
     double* Vm = a.Vm + ii * (long long)n;
     double* Ib = a.Ib + ii * 2 * (long long)n;
-    double* Vn = a.Vn + ii * 2 * (long long)n;
     // V of this instance: in LDS for lean launches (loaded here, stored back at the end), else global
     lds_dbl* const Vl = (lds_dbl*)(lds_raw + a.v_off) + 2 * g;
     auto ldV = [&](int b) -> c64 {
@@ -500,16 +498,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 
         // ---- J fill: zero the tile fill-in slots, write every J entry ------------------
         // (dS_dVa / dS_dVm, BaseNRAlgo.tpp:202-260, per Ybus entry: one set of loads feeds the
-        // up to 4 J entries built on it; V / |V| per bus first)
-        if (act)
-            for (int b = s; b < n; b += nslot) {
-                const c64 v = ldV(b);
-                const double vr = v.re, vi = v.im;
-                const double ic = 1.0 / sqrt(vr * vr + vi * vi);
-                Vn[2 * b] = vr * ic;
-                Vn[2 * b + 1] = vi * ic;
-            }
-        __syncthreads();
+        // up to 4 J entries built on it; V / |V| of the column bus computed on the spot)
         if (act) {
             auto put = [&](int slot, int jidx, double jv) {
                 if (slot < 0) return;
@@ -545,7 +534,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     y[u] = Yye ? edits(ld_c(Yye, min(e0 + u * nslot, a.nye - 1)), pos[u], dp) : Yat(pos[u], dp);
                     vr[u] = ldV(rc[u].x);
                     vc[u] = ldV(rc[u].y);
-                    vnc[u] = ld_c(Vn, rc[u].y);
+                    const double ic = 1.0 / sqrt(vc[u].re * vc[u].re + vc[u].im * vc[u].im);
+                    vnc[u] = {vc[u].re * ic, vc[u].im * ic};
                     ib[u] = rc[u].x == rc[u].y ? ld_c(Ib, rc[u].x) : c64{0.0, 0.0};
                 }
 #pragma unroll
@@ -1387,7 +1377,7 @@ void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Vn][Xg][Ybus gathered].
+// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Xg][Ybus gathered].
 void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
                   const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
                   int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
@@ -1413,7 +1403,6 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     const size_t o_va = off; off += align_up((size_t)ninst * n * 8);
     const size_t o_vm = off; off += align_up((size_t)ninst * n * 8);
     const size_t o_ib = off; off += align_up((size_t)ninst * n * 16);
-    const size_t o_vn = off; off += align_up((size_t)ninst * n * 16);
     const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
     // shared Ybus: its values gathered in the order of the mismatch and J-fill tables
     const bool gather = Yx_stride == 0;
@@ -1443,7 +1432,6 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.nye = (int)P.ye_pos.size();
     a.jc_slot = gr->jc_slot.p; a.jc_jidx = gr->jc_jidx.p; a.jc_val = gr->jc_val.p; a.njc = (int)P.jc_slot.size();
     a.dptr = deltas ? d_dptr : nullptr; a.dpos = d_dpos; a.dval = d_dval;
-    a.Vn = reinterpret_cast<double*>(ws + o_vn);
     a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
