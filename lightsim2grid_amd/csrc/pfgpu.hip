@@ -587,6 +587,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 #pragma unroll
                 for (int u = 0; u < kHB; ++u) q[u] = a.fs[(size_t)base + (size_t)min(u, K - 1) * U + s];
             };
+            auto ld_item = [&](int base, int K, int U, int k) { return a.fs[(size_t)base + (size_t)min(k, K - 1) * U + s]; };
             FT acc = (FT)0;
             // end of a chunk: the team's lanes reduce, the leader updates the task's slot
             auto chunk_end = [&](const int e, const int T) {
@@ -619,9 +620,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                                 kend += Lc;
                             }
                         }
-                    for (int k = kHB; k < K; k += 2) {
-                        const int4 r0 = a.fs[(size_t)base + (size_t)k * U + s];
-                        const int4 r1 = a.fs[(size_t)base + (size_t)min(k + 1, K - 1) * U + s];
+                    // the items past the head, two at a time; the loads of the next two are
+                    // issued before the current two are used (two register pairs in turn)
+                    auto step = [&](int k, const int4 r0, const int4 r1) {
                         const FT p0 = prod(r0), p1 = prod(r1);
                         acc += p0;
                         if (k == kend) {
@@ -632,6 +633,17 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                         if (k + 1 == kend) {
                             chunk_end(r1.w, T);
                             kend += Lc;
+                        }
+                    };
+                    if (K > kHB) {
+                        int4 A0 = ld_item(base, K, U, kHB), A1 = ld_item(base, K, U, kHB + 1);
+                        for (int k = kHB; k < K; k += 4) {
+                            const int4 B0 = ld_item(base, K, U, k + 2), B1 = ld_item(base, K, U, k + 3);
+                            step(k, A0, A1);
+                            if (k + 2 >= K) break;
+                            A0 = ld_item(base, K, U, k + 4);
+                            A1 = ld_item(base, K, U, k + 5);
+                            step(k + 2, B0, B1);
                         }
                     }
                 }
@@ -861,6 +873,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 #pragma unroll
                 for (int u = 0; u < kHB; ++u) q[u] = a.bs[(size_t)base + (size_t)min(u, K - 1) * U + s];
             };
+            auto ld_item = [&](int base, int K, int U, int k) { return a.bs[(size_t)base + (size_t)min(k, K - 1) * U + s]; };
             FT acc = (FT)0;
             auto chunk_end = [&](const int4 q, const int T) {
                 const int e = q.z;
@@ -888,9 +901,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                                 kend += Lc;
                             }
                         }
-                    for (int k = kHB; k < K; k += 2) {
-                        const int4 r0 = a.bs[(size_t)base + (size_t)k * U + s];
-                        const int4 r1 = a.bs[(size_t)base + (size_t)min(k + 1, K - 1) * U + s];
+                    // the items past the head, two at a time; the loads of the next two are
+                    // issued before the current two are used (two register pairs in turn)
+                    auto step = [&](int k, const int4 r0, const int4 r1) {
                         const FT p0 = prod(r0), p1 = prod(r1);
                         acc += p0;
                         if (k == kend) {
@@ -901,6 +914,17 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                         if (k + 1 == kend) {
                             chunk_end(r1, T);
                             kend += Lc;
+                        }
+                    };
+                    if (K > kHB) {
+                        int4 A0 = ld_item(base, K, U, kHB), A1 = ld_item(base, K, U, kHB + 1);
+                        for (int k = kHB; k < K; k += 4) {
+                            const int4 B0 = ld_item(base, K, U, k + 2), B1 = ld_item(base, K, U, k + 3);
+                            step(k, A0, A1);
+                            if (k + 2 >= K) break;
+                            A0 = ld_item(base, K, U, k + 4);
+                            A1 = ld_item(base, K, U, k + 5);
+                            step(k + 2, B0, B1);
                         }
                     }
                 }
