@@ -572,6 +572,36 @@ void build_jfill_table(DevicePlan& P) {
         P.ye_slot[4 * e + sel] = P.j_slot[q];
         P.ye_jidx[4 * e + sel] = q;
     }
+    // diagonal entries first (their J values need Ibus of their bus), in bus order
+    const int ne = (int)P.ye_pos.size();
+    std::vector<int> ord(ne);
+    for (int e = 0; e < ne; ++e) ord[e] = e;
+    auto diag = [&](int e) { return P.ye_rc[2 * e] == P.ye_rc[2 * e + 1]; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        if (diag(x) != diag(y)) return diag(x);
+        return diag(x) && P.ye_rc[2 * x] < P.ye_rc[2 * y];
+    });
+    std::vector<int> pos(ne), rc(2 * ne), sl(4 * ne), jx(4 * ne);
+    for (int i = 0; i < ne; ++i) {
+        const int e = ord[i];
+        pos[i] = P.ye_pos[e];
+        rc[2 * i] = P.ye_rc[2 * e];
+        rc[2 * i + 1] = P.ye_rc[2 * e + 1];
+        for (int k = 0; k < 4; ++k) {
+            sl[4 * i + k] = P.ye_slot[4 * e + k];
+            jx[4 * i + k] = P.ye_jidx[4 * e + k];
+        }
+    }
+    P.ye_pos.swap(pos);
+    P.ye_rc.swap(rc);
+    P.ye_slot.swap(sl);
+    P.ye_jidx.swap(jx);
+    P.nyd = 0;
+    P.ye_diag_of.assign(P.n, -1);
+    while (P.nyd < ne && P.ye_rc[2 * P.nyd] == P.ye_rc[2 * P.nyd + 1]) {
+        P.ye_diag_of[P.ye_rc[2 * P.nyd]] = P.nyd;
+        ++P.nyd;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -122,7 +122,11 @@ struct DevicePlan {
     // ye_slot holds their value slots (-1: absent; bit 30: store the reciprocal) and
     // ye_jidx their J entry indices. The constant J entries (slack-weight column) are
     // listed apart in jc_slot / jc_val.
+    // The diagonal entries come first (nyd of them, in bus order); ye_diag_of[bus] is the
+    // index of the bus's diagonal entry, -1 if it has none.
     std::vector<int> ye_pos, ye_rc, ye_slot, ye_jidx;
+    int nyd = 0;
+    std::vector<int> ye_diag_of;
     std::vector<int> jc_slot, jc_jidx;
     std::vector<double> jc_val;
     // dense-tile slots that hold no J entry (fill-in): zeroed by the J-fill phase

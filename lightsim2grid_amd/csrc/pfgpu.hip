@@ -134,7 +134,9 @@ struct KArgs {
     double* V;           // [ninst][n] complex
     double* Va;          // [ninst][n]
     double* Vm;          // [ninst][n]
-    double* Ib;          // [ninst][n] complex
+    double* Ib;          // [ninst][nyd] complex: Ibus of the bus of diagonal J-fill entry e < nyd
+    const int* ye_diag_of;  // per bus: its diagonal J-fill entry, -1 if none
+    int nyd;             // diagonal J-fill entries (they come first)
     void* Xg;            // spill [nblocks][nspill][G]
     int* iters;
     int* status;
@@ -351,7 +353,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     double* V = a.V + ii * 2 * (long long)n;
     double* Va = a.Va + ii * (long long)n;
     double* Vm = a.Vm + ii * (long long)n;
-    double* Ib = a.Ib + ii * 2 * (long long)n;
+    double* Ib = a.Ib + ii * 2 * (long long)a.nyd;
     // V of this instance: in LDS for lean launches (loaded here, stored back at the end), else global
     lds_dbl* const Vl = (lds_dbl*)(lds_raw + a.v_off) + 2 * g;
     auto ldV = [&](int b) -> c64 {
@@ -451,7 +453,11 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             edit_pos(dp);
             for (int r = s; r < n; r += nslot) {
                 c64 acc = {0.0, 0.0};
+                // the row's own data, loaded with its first entries
                 const int len = a.yr_ptr[r + 1] - a.yr_ptr[r];
+                const c64 sr = ld_c(S, r);
+                const double wr = a.single_slack ? 0.0 : a.w[r];
+                const int fp = a.fslot_p[r], fq = a.fslot_q[r], ed = a.ye_diag_of[r];
                 for (int k0 = 0; k0 == 0 || k0 < len; k0 += kEll) {
                     int2 pc[kEll];
 #pragma unroll
@@ -466,11 +472,12 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     for (int k = 0; k < kEll; ++k)
                         if (pc[k].x >= 0) acc = cadd(acc, cmul(yv[k], vv[k]));
                 }
-                Ib[2 * r] = acc.re;
-                Ib[2 * r + 1] = acc.im;
-                c64 mis = csub(cmul(ldV(r), cconj(acc)), ld_c(S, r));
-                if (!a.single_slack) mis.re += sa * a.w[r];
-                const int fp = a.fslot_p[r], fq = a.fslot_q[r];
+                if (ed >= 0) {
+                    Ib[2 * ed] = acc.re;
+                    Ib[2 * ed + 1] = acc.im;
+                }
+                c64 mis = csub(cmul(ldV(r), cconj(acc)), sr);
+                if (!a.single_slack) mis.re += sa * wr;
                 if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
                 if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
             }
@@ -521,28 +528,31 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 int pos[kJE];
                 int2 rc[kJE];
                 int4 sl[kJE];
+                c64 y[kJE], vr[kJE], vc[kJE], vnc[kJE], ib[kJE];
+                // loads that need nothing loaded first: the table entries, the gathered Ybus values
+                // and the Ibus of diagonal entries (they come first)
 #pragma unroll
                 for (int u = 0; u < kJE; ++u) {
                     const int e = min(e0 + u * nslot, a.nye - 1);
                     pos[u] = a.ye_pos[e];
                     rc[u] = a.ye_rc[e];
                     sl[u] = a.ye_slot[e];
+                    if (Yye) y[u] = ld_c(Yye, e);
+                    ib[u] = e0 + u * nslot < a.nyd ? ld_c(Ib, e) : c64{0.0, 0.0};
                 }
-                c64 y[kJE], vr[kJE], vc[kJE], vnc[kJE], ib[kJE];
 #pragma unroll
                 for (int u = 0; u < kJE; ++u) {
-                    y[u] = Yye ? edits(ld_c(Yye, min(e0 + u * nslot, a.nye - 1)), pos[u], dp) : Yat(pos[u], dp);
+                    y[u] = Yye ? edits(y[u], pos[u], dp) : Yat(pos[u], dp);
                     vr[u] = ldV(rc[u].x);
                     vc[u] = ldV(rc[u].y);
                     const double ic = 1.0 / sqrt(vc[u].re * vc[u].re + vc[u].im * vc[u].im);
                     vnc[u] = {vc[u].re * ic, vc[u].im * ic};
-                    ib[u] = rc[u].x == rc[u].y ? ld_c(Ib, rc[u].x) : c64{0.0, 0.0};
                 }
 #pragma unroll
                 for (int u = 0; u < kJE; ++u) {
                     const int e = e0 + u * nslot;
                     if (e >= a.nye) break;
-                    const bool diag = rc[u].x == rc[u].y;
+                    const bool diag = e < a.nyd;
                     c64 t = cmul(y[u], vc[u]);
                     c64 dvm = cmul(cconj(cmul(y[u], vnc[u])), vr[u]);
                     if (diag) {
@@ -1198,7 +1208,7 @@ struct pfgpu_grid {
     pfgpu::DevicePlan P;
     pfgpu_grid_info info{};
     DevBuf<int> yr_ptr, yr_col, yr_pos, fslot_p, fslot_q, ux_bus, ux_slot, j_ypos, j_rbus, j_cbus, j_slot, zero_slots;
-    DevBuf<int> yr_pc, yr_ell, ye_pos, ye_rc, ye_slot, ye_jidx, jc_slot, jc_jidx;
+    DevBuf<int> yr_pc, yr_ell, ye_pos, ye_rc, ye_slot, ye_jidx, ye_diag_of, jc_slot, jc_jidx;
     int ell_w = 0;  // columns of yr_ell
     DevBuf<double> jc_val;
     DevBuf<int8_t> ux_kind;
@@ -1426,7 +1436,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     size_t off = 0;
     const size_t o_va = off; off += align_up((size_t)ninst * n * 8);
     const size_t o_vm = off; off += align_up((size_t)ninst * n * 8);
-    const size_t o_ib = off; off += align_up((size_t)ninst * n * 16);
+    const size_t o_ib = off; off += align_up((size_t)ninst * std::max(P.nyd, 1) * 16);
     const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
     // shared Ybus: its values gathered in the order of the mismatch and J-fill tables
     const bool gather = Yx_stride == 0;
@@ -1468,6 +1478,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.S = d_S; a.s_stride = (long long)sbus_stride * 2;
     a.V = d_V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
+    a.ye_diag_of = gr->ye_diag_of.p; a.nyd = P.nyd;
     a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
     a.v_off = (int)lc.v_off;
     a.Yell = gather ? reinterpret_cast<const double*>(ws + o_yell) : nullptr;
@@ -1628,7 +1639,7 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
             d.upload(w);
         };
         up_or(g->ye_pos, P.ye_pos, 1); up_or(g->ye_rc, P.ye_rc, 2); up_or(g->ye_slot, P.ye_slot, 4);
-        up_or(g->ye_jidx, P.ye_jidx, 4); up_or(g->jc_slot, P.jc_slot, 1); up_or(g->jc_jidx, P.jc_jidx, 1);
+        up_or(g->ye_jidx, P.ye_jidx, 4); up_or(g->ye_diag_of, P.ye_diag_of, 1); up_or(g->jc_slot, P.jc_slot, 1); up_or(g->jc_jidx, P.jc_jidx, 1);
         up_or(g->jc_val, P.jc_val, 1);
         g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
         g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
