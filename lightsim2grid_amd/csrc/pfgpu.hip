@@ -113,6 +113,8 @@ struct KArgs {
     const int *dptr, *dpos;
     const double* dval;
     const int8_t* ux_kind;
+    const int2* bus_dx;  // per bus: {slot of its Va step, slot of its Vm step}, -1 if not an unknown
+    int slack_dx;        // slot of the slack_absorbed step, -1 for the single-slack form
     const int *j_ypos, *j_rbus, *j_cbus, *j_slot, *zero_slots;
     const uint8_t* j_sel;
     const double* j_cval;
@@ -231,6 +233,32 @@ __device__ __noinline__ double rect_of(VP V, int vs, const double* __restrict__ 
         vmin = fmin(vmin, m);
     }
     return vmin;
+}
+
+// One bus of the voltage update (BaseNRAlgo.tpp:138-151): Va and Vm (taken from V first in
+// the single-slack form, BaseNRSingleSlackAlgo.tpp:131-132) minus the Newton step, then
+// V = Vm exp(j Va); returns Vm.
+template <typename VP>
+__device__ __noinline__ double update_bus(VP V, int vs, double* __restrict__ Va, double* __restrict__ Vm, int b,
+                                          double dva, double dvm, bool polar_first) {
+    double va, vm;
+    if (polar_first) {
+        const double vr = V[2 * b * vs], vi = V[2 * b * vs + 1];
+        va = atan2(vi, vr);
+        vm = sqrt(vr * vr + vi * vi);
+    } else {
+        va = Va[b];
+        vm = Vm[b];
+    }
+    va -= dva;
+    vm -= dvm;
+    Va[b] = va;
+    Vm[b] = vm;
+    double sn, cs;
+    sincos(va, &sn, &cs);
+    V[2 * b * vs] = vm * cs;
+    V[2 * b * vs + 1] = vm * sn;
+    return vm;
 }
 
 // Lane streams. Level l uses lanes [0, U) with K items per lane, item k of lane s
@@ -1066,24 +1094,19 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         }
 
         // ---- voltage update ------------------------------------------------------------
-        if (act && a.single_slack) {  // BaseNRSingleSlackAlgo.tpp:131-132
-            if constexpr (LEAN) polar_of(Vl, G, Va, Vm, s, n, nslot);
-            else polar_of(V, 1, Va, Vm, s, n, nslot);
-        }
-        __syncthreads();
-        if (act)
-            for (int j = s; j < a.N; j += nslot) {
-                const double dx = (double)XL(a.ux_slot[j]);
-                const int kind = a.ux_kind[j], b = a.ux_bus[j];
-                if (kind == 0) s_slack[g] -= dx;
-                else if (kind == 1) Va[b] -= dx;
-                else Vm[b] -= dx;
-            }
-        __syncthreads();
+        // per bus, by the lane that owns it: its Va and Vm steps are read from their slots
         double vmin = INFINITY;
         if (act) {
-            if constexpr (LEAN) vmin = rect_of(Vl, G, Va, Vm, s, n, nslot);
-            else vmin = rect_of(V, 1, Va, Vm, s, n, nslot);
+            if (s == 0 && a.slack_dx >= 0) s_slack[g] -= (double)XL(a.slack_dx);
+            for (int b = s; b < n; b += nslot) {
+                const int2 us = a.bus_dx[b];
+                const double dva = us.x >= 0 ? (double)XL(us.x) : 0.0;
+                const double dvm = us.y >= 0 ? (double)XL(us.y) : 0.0;
+                double vm;
+                if constexpr (LEAN) vm = update_bus(Vl, G, Va, Vm, b, dva, dvm, a.single_slack != 0);
+                else vm = update_bus(V, 1, Va, Vm, b, dva, dvm, a.single_slack != 0);
+                vmin = fmin(vmin, vm);
+            }
         }
         vmin = inst_reduce<G, NT, false>(vmin, s_part);
         if (act && vmin < 0.0) {
@@ -1212,6 +1235,8 @@ struct pfgpu_grid {
     int ell_w = 0;  // columns of yr_ell
     DevBuf<double> jc_val;
     DevBuf<int8_t> ux_kind;
+    DevBuf<int> bus_dx;
+    int slack_dx = -1;
     DevBuf<uint8_t> j_sel;
     DevBuf<double> j_cval, w;
     // per-(G, block size) lane streams of the factor/forward and backward schedules
@@ -1468,6 +1493,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.dptr = deltas ? d_dptr : nullptr; a.dpos = d_dpos; a.dval = d_dval;
     a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
+    a.bus_dx = reinterpret_cast<const int2*>(gr->bus_dx.p); a.slack_dx = gr->slack_dx;
     a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
     a.j_slot = gr->j_slot.p; a.zero_slots = gr->zero_slots.p;
     const auto& ls = *gr->streams[shape_index(lc.G, lc.NT, fsize, lc.lean)];
@@ -1643,6 +1669,14 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         up_or(g->jc_val, P.jc_val, 1);
         g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
         g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
+        {
+            std::vector<int> bd(2 * (size_t)std::max(P.n, 1), -1);
+            for (size_t j = 0; j < P.ux_bus.size(); ++j) {
+                if (P.ux_kind[j] == 0) g->slack_dx = P.ux_slot[j];
+                else bd[2 * (size_t)P.ux_bus[j] + (P.ux_kind[j] == 1 ? 0 : 1)] = P.ux_slot[j];
+            }
+            g->bus_dx.upload(bd);
+        }
         g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
         g->j_cval.upload(P.j_cval); g->j_slot.upload(P.j_slot);
         g->zero_slots.upload(P.zero_slots.empty() ? std::vector<int>(1, 0) : P.zero_slots);
