@@ -903,7 +903,7 @@ ChunkPlacement place_chunked(const std::vector<int>& np, int G, int nslot, int l
         const int U = c == 1 ? std::min(nslot, (int)((total + lane_round - 1) / lane_round * lane_round)) : nslot;
         const double simd_load = std::max(1.0, std::ceil((double)U * G / 64.0) / 4.0);
         const int K = c * Lc;
-        const double cost = simd_load * (K + c * (7.0 + 1.5 * maxlt)) + 12.0 * std::max(0, (K - kHead + 1) / 2);
+        const double cost = simd_load * (K + c * (3.0 + 1.0 * maxlt)) + 12.0 * std::max(0, (K - kHead + 1) / 2);
         if (cost < best_cost - 1e-9) {
             best_cost = cost;
             best.Lc = Lc;
