@@ -266,7 +266,7 @@ __device__ __noinline__ double update_bus(VP V, int vs, double* __restrict__ Va,
 // slot, 25 assign (a fill-in entry: its slot held another value before), 26-28
 // log2(team), 29 reciprocal, 30 end of task, 31 the lane's last item of the level
 // (the lane loads nothing after it).
-constexpr int kSC = 2;  // items per chunk
+constexpr int kSC = 2;  // items per prefetch group of the general walk
 constexpr int kHB = 4;  // items of a level loaded ahead, during the level before it (lean walk)
 constexpr int kEnd = 1 << 30;
 __device__ __forceinline__ int4 no_item() { return int4{-1, -1, 0, 0}; }
@@ -600,10 +600,11 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         __syncthreads();
 
         // ---- factor + forward solve --------------------------------------------
-        // The chunks of all levels form one sequence, walked with three register
-        // buffers in turn: while one chunk is worked on, the next two (possibly of the
-        // next level, across the barrier) are loading. The operands of a chunk are read
-        // before its prefetch is issued, so waiting for them leaves the prefetches in flight.
+        // General walk (launches that spill): the kSC-item groups of all levels form one
+        // sequence, walked with three register buffers in turn: while one group is worked on,
+        // the next two (possibly of the next level, across the barrier) are loading. The
+        // operands of a group are read before its prefetch is issued, so waiting for them
+        // leaves the prefetches in flight. The lean walk is described at its own code.
         {
             bool bad = false;  // a zero or non-finite pivot met by this lane
             if constexpr (LEAN) {
