@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             } else {
             // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
             // log2 of its largest team), uniform
-            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? __builtin_amdgcn_readfirstlane(lvK[2 * l + w]) : 0; };
+            auto lvk = [&](int l, int w) -> int { return l < a.nlev ? lv_s[2 * l + w] : 0; };
             int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
             int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
             bool more = true;                 // the lane has items left in the load cursor's level
@@ -992,10 +992,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 w1 = n1;
             }
             } else {
-            const lds_int* bK = lvK + 2 * a.nlev;
             // level sizes of the load cursor (lK, lU) and of the process cursor (pK, pU, pT:
             // log2 of its largest team), uniform
-            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? __builtin_amdgcn_readfirstlane(bK[2 * l + w]) : 0; };
+            auto lvk = [&](int l, int w) -> int { return l < a.nblev ? lv_s[2 * (a.nlev + l) + w] : 0; };
             int ll = 0, lk = 0, lbase = 0;   // load cursor: level, item, level base
             int lK = lvk(0, 0), lU = lvk(0, 1) & 0xffffff;
             bool more = true;                 // the lane has items left in the load cursor's level
