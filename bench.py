@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -23,9 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_nr_case2000_n1_hbm_traffic.md")
+PUBLISHED_14BUS_MS = 0.0112  # NR (KLU) time in algo, l2rpn_case14_sandbox, 1 core (docs/benchmarks.rst:212)
 
 
 def measured_traffic(form: str):
@@ -42,17 +42,114 @@ def measured_traffic(form: str):
     return None
 
 
+def cpu_info() -> dict:
+    """The host CPU as this process sees it: logical CPUs, the CPUs it may run on (affinity and
+    the cgroup CPU quota), physical cores of the machine and the model name."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, p = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(p)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    usable = info["affinity"] or 1
+    if quota:
+        usable = min(usable, max(1, int(quota)))
+    info["usable"] = usable
+    cores, model = set(), None
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    continue
+                k, v = [x.strip() for x in line.split(":", 1)]
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    info["physical_cores"] = len(cores) or None
+    info["model"] = model
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def algorithmic_bytes(info, iters, w):
-    """Bytes per launch: sum over solves of it * [w (nnzJ + 3 nnzLU) + 16*4n + 8*2N]."""
+    """Bytes of solves with `iters` iterations: sum of it * [w (nnzJ + 3 nnzLU) + 16*4n + 8*2N]."""
     n, N = info["n_bus"], info["n_jac"]
     per_it = w * (info["nnz_jac"] + 3 * info["nnz_lu"]) + 64 * n + 16 * N
     return float(np.sum(iters)) * per_it
 
 
+def parity(V, iters, status, Vr, itr, convr):
+    """Converged flags, iteration deltas and max |V - V_ref| of GPU rows against the oracle."""
+    conv = status == 0
+    both = conv & convr
+    dit = (iters[both].astype(int) - itr[both].astype(int)) if both.any() else np.zeros(0, int)
+    hist = {str(k): int(v) for k, v in zip(*np.unique(dit, return_counts=True))}
+    return {"flags_equal": bool(np.array_equal(conv, convr)), "n_flag_diff": int(np.sum(conv != convr)),
+            "max_dV": float(np.max(np.abs(V[both] - Vr[both]))) if both.any() else None,
+            "iter_delta_hist": hist}
+
+
+def single_solve_latency(reps: int = 300) -> dict:
+    """Config 0: one 14-bus FP64 flat-start solve (tol 1e-8, max 10 iterations) through the solver
+    class (host arrays in, V out: the drop-in call) and through the device entry point."""
+    from lightsim2grid_amd import abi, gridgen, solver
+    g = gridgen.make("case14_synth")
+    d = g.solver_inputs()
+    s = solver.AMDGPUSolver()
+    args = (d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    assert s.compute_pf(*args)
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.compute_pf(*args)
+        t.append(time.perf_counter() - t0)
+    ctx = abi.Context(0)
+    gr = abi.Grid(ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    dY, dS, dV0 = (ctx.to_device(gr.ybus_values(d["Ybus"])), ctx.to_device(np.ascontiguousarray(d["Sbus"])),
+                   ctx.to_device(d["V0"]))
+    dV, dIt, dSt = ctx.empty((1, g.n), np.complex128), ctx.empty(1, np.int32), ctx.empty(1, np.int32)
+    td, tk = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st = gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, 1, dIt, dSt, 10, 1e-8)
+        td.append(time.perf_counter() - t0)
+        tk.append(st["kernel_s"])
+    out = {"workload": "case14_synth, FP64, flat start, tol 1e-8, max 10 iterations, one solve per call",
+           "iters": int(s.get_nb_iter()),
+           "solver_class_ms_p50": 1e3 * float(np.median(t)), "solver_class_ms_p99": 1e3 * float(np.percentile(t, 99)),
+           "device_call_ms_p50": 1e3 * float(np.median(td)), "kernel_ms_p50": 1e3 * float(np.median(tk)),
+           "published_cpu_ms": PUBLISHED_14BUS_MS,
+           "published_source": "NR (KLU) time in algo, l2rpn_case14_sandbox, i7-13700H 1 core (docs/benchmarks.rst:212)"}
+    gr.close()
+    ctx.close()
+    return out
+
+
 def serving_run(seconds: float, rate: float = 120.0, seed: int = 7) -> dict:
     """Serving mix: Poisson arrivals over resident 14/118/2000/10000-bus grids (weights
     0.4/0.3/0.2/0.1); single power flow 50 %, contingency batch of 1-256 outages 35 %,
-    time-series chunk of 1-96 steps 15 %; both precision forms."""
+    time-series chunk of 1-96 steps 15 %; both precision forms; p50 / p99 per grid size."""
     import threading
 
     from lightsim2grid_amd import gridgen
@@ -65,7 +162,6 @@ def serving_run(seconds: float, rate: float = 120.0, seed: int = 7) -> dict:
     nonbridge = [np.nonzero(~gridgen.bridges(g))[0] for g in grids]
     out = {"workload": "Poisson arrivals %.0f req/s for %.1f s; grids 14/118/2000/10000 buses (0.4/0.3/0.2/0.1); "
                        "single PF 50%%, N-1 batch 1-256 35%%, time-series chunk 1-96 15%%" % (rate, seconds)}
-    # warm-start voltages: each grid's base case, solved once (FP64) and kept resident
     base_v = []
     tmp = ServeLoop(0, 16, 0)
     for d in inputs:
@@ -108,34 +204,45 @@ def serving_run(seconds: float, rate: float = 120.0, seed: int = 7) -> dict:
                 if dt > 0:
                     time.sleep(dt)
                 if kind == "pf":
-                    ids.append(sl.submit_pf(gids[k], None))
+                    ids.append((k, kind, sl.submit_pf(gids[k], None)))
                 elif kind == "ca":
-                    ids.append(sl.submit_contingencies(gids[k], *payload))
+                    ids.append((k, kind, sl.submit_contingencies(gids[k], *payload)))
                 else:
-                    ids.append(sl.submit_time_series(gids[k], payload))
+                    ids.append((k, kind, sl.submit_time_series(gids[k], payload)))
 
         th = threading.Thread(target=submitter)
         t0 = time.perf_counter()
         th.start()
         th.join()
         lat, solves, conv = [], 0, 0
-        for i in ids:
+        per_grid = {nm: [] for nm in names}
+        per_kind = {"pf": [], "ca": [], "ts": []}
+        for (k, kind, i) in ids:
             r = sl.result(i, 300.0)
             lat.append(r["latency_s"])
+            per_grid[names[k]].append(r["latency_s"])
+            per_kind[kind].append(r["latency_s"])
             solves += len(r["status"])
             conv += int(np.sum(r["status"] == 0))
         wall = time.perf_counter() - t0
         launches = sl.launches()
         sl.close()
         lat = np.array(lat)
+
+        def pct(v):
+            return {"n": len(v), "p50_ms": 1e3 * float(np.percentile(v, 50)), "p99_ms": 1e3 * float(np.percentile(v, 99))} \
+                if v else {"n": 0}
         out[pname] = {"requests": len(ids), "solves": solves, "converged": conv, "wall_s": wall,
                       "solves_per_s": conv / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
-                      "p99_ms": 1e3 * float(np.percentile(lat, 99)), "launches": launches}
+                      "p99_ms": 1e3 * float(np.percentile(lat, 99)), "launches": launches,
+                      "per_grid": {nm: pct(v) for nm, v in per_grid.items()},
+                      "per_kind": {nm: pct(v) for nm, v in per_kind.items()}}
     return out
 
 
-def time_series_run(with_cpu: bool, T: int = 8760) -> dict:
-    """118-bus, 8760 hourly steps, warm-started chains (TimeSeriesCPP), both forms."""
+def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
+    """118-bus, 8760 hourly steps (TimeSeriesCPP), both forms: the reference's single warm-started
+    chain, and chains of 8 steps side by side; the CPU oracle on the same layouts."""
     import lightsim2grid_amd as L
     from lightsim2grid_amd import gridgen
     g = gridgen.make("case118_synth")
@@ -143,29 +250,35 @@ def time_series_run(with_cpu: bool, T: int = 8760) -> dict:
     gen_bus = np.r_[g.slack, g.pv]
     gen_p = Sb[:, gen_bus].real * (gen_bus != g.slack)
     load_p, load_q = -Sb[:, g.pq].real, -Sb[:, g.pq].imag
-    out = {"workload": "case118_synth, %d hourly steps, chains of 8 warm-started steps" % T}
-    for st, nm in ((L.SolverType.AMDGPUMixed, "fp32"), (L.SolverType.AMDGPU, "fp64")):
-        ts = L.TimeSeriesCPP(gridgen.to_gridmodel(g))
-        ts.change_solver(st)
-        ts.set_chain_length(8)
-        ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)  # warm-up
-        t0 = time.perf_counter()
-        status = ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)
-        wall = time.perf_counter() - t0
-        out[nm] = {"status": int(status), "steps": T, "wall_s": wall, "solves_per_s": T / wall,
-                   "solver_time_s": ts.solver_time()}
-    if with_cpu:
-        from oracle import oracle
-        d = g.solver_inputs()
-        nth = min(16, os.cpu_count() or 1)
-        _, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
-                                                         d["pq"], g.v0(), 10, 1e-8, False, nth)
-        out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nth, "kind": "port", "wall_s": wall}
+    out = {"workload": "case118_synth (1 slack, 53 PV), %d hourly steps, warm start from the previous step" % T}
+    for chain, lname in ((0, "single_chain"), (8, "chains_of_8")):
+        res = {}
+        for st, nm in ((L.SolverType.AMDGPUMixed, "fp32"), (L.SolverType.AMDGPU, "fp64")):
+            ts = L.TimeSeriesCPP(gridgen.to_gridmodel(g))
+            ts.change_solver(st)
+            ts.set_chain_length(chain)
+            ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)  # warm-up
+            t0 = time.perf_counter()
+            status = ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)
+            wall = time.perf_counter() - t0
+            res[nm] = {"status": int(status), "steps": T, "wall_s": wall, "solves_per_s": T / wall,
+                       "solver_time_s": ts.solver_time()}
+        if with_cpu:
+            from oracle import oracle
+            d = g.solver_inputs()
+            nch = 1 if chain == 0 else (T + chain - 1) // chain
+            _, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"],
+                                                             d["pv"], d["pq"], g.v0(), 10, 1e-8, False, nch,
+                                                             1 if nch == 1 else nthreads)
+            res["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": 1 if nch == 1 else nthreads,
+                                   "kind": "port", "wall_s": wall}
+        out[lname] = res
     return out
 
 
-def large_grid_run(with_cpu: bool, n_out: int = 5000, n_snap: int = 4) -> dict:
-    """10000-bus grid: n_out seeded non-bridge outages x n_snap load snapshots (load x U(0.8, 1.1))."""
+def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int = 4) -> dict:
+    """10000-bus grid: n_out seeded non-bridge outages x n_snap load snapshots (load x U(0.8, 1.1)),
+    plus the latency of one solve."""
     from lightsim2grid_amd import abi, gridgen
     g = gridgen.make("case10000_synth")
     d = g.solver_inputs()
@@ -189,7 +302,8 @@ def large_grid_run(with_cpu: bool, n_out: int = 5000, n_snap: int = 4) -> dict:
     dV = ctx.empty((ninst, g.n), np.complex128)
     dIt, dSt = ctx.empty(ninst, np.int32), ctx.empty(ninst, np.int32)
     dd = (ctx.to_device(dptr), ctx.to_device(dpos), ctx.to_device(dval))
-    out = {"workload": "case10000_synth: %d non-bridge outages x %d load snapshots = %d solves" % (len(outs), n_snap, ninst),
+    out = {"workload": "case10000_synth (distributed slack over the 2500 PV generators, loading 0.25): "
+                       "%d non-bridge outages x %d load snapshots = %d solves" % (len(outs), n_snap, ninst),
            "grid_load_s": t_load, "n_jac": gr.info["n_jac"], "nnz_lu": gr.info["nnz_lu"]}
     for prec, nm in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
         gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
@@ -199,15 +313,20 @@ def large_grid_run(with_cpu: bool, n_out: int = 5000, n_snap: int = 4) -> dict:
         status = dSt.to_host()
         conv = int(np.sum(status == 0))
         out[nm] = {"wall_s": wall, "kernel_ms": 1e3 * st["kernel_s"], "converged": conv, "solves_per_s": conv / wall,
-                   "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"]}
+                   "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"],
+                   "fallback_fp64": st["fallback_fp64"], "resident_workgroups": st["resident_workgroups"]}
+        lat = []
+        for _ in range(5):
+            r = gr.solve(Yx, d["Sbus"], Vb, 10, 1e-8, precision=prec, deltas=(dptr[:2], dpos[:dptr[1]], dval[:dptr[1]]))
+            lat.append(r["stats"]["kernel_s"])
+        out[nm]["single_solve_kernel_ms"] = 1e3 * float(np.median(lat))
     if with_cpu:
         from oracle import oracle
-        nth = min(16, os.cpu_count() or 1)
-        k = 16 * nth
+        k = 16 * nthreads
         sub = (dptr[:k + 1], dpos[:dptr[k]], dval[:dptr[k]])
         _, _, _, conv, _, wall = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"],
-                                                    d["pq"], Vb, sub, 10, 1e-8, False, False, nth)
-        out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nth, "kind": "port",
+                                                    d["pq"], Vb, sub, 10, 1e-8, False, False, nthreads)
+        out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nthreads, "kind": "port",
                                "sample": "%d outage solves (base injections), %.2f s" % (k, wall)}
     gr.close()
     ctx.close()
@@ -221,9 +340,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", default="case2000_synth")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0: min(16, nproc))")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads for the CPU baseline (0: every CPU this process may use)")
     ap.add_argument("--instances-per-wg", type=int, default=0)
-    ap.add_argument("--no-extra", action="store_true", help="skip the serving / time-series / 10000-bus lines")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -262,8 +382,7 @@ def main():
         return gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec,
                                deltas=(ddp, ddq, ddv), instances_per_wg=args.instances_per_wg)
 
-    results = {}
-    tuned = {}
+    results, tuned, gpu_rows = {}, {}, {}
     for prec, name in ((abi.FP64, "fp64"), (abi.FP32, "fp32")):
         if not args.instances_per_wg:
             # the project's launch autotuner (instances per workgroup x block size), untimed
@@ -272,49 +391,69 @@ def main():
         for _ in range(args.warmup):
             sweep(prec)
         ksum = 0.0
+        fb = 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
             st = sweep(prec)
             ksum += st["kernel_s"]
+            fb += st["fallback_fp64"]
         t1 = time.perf_counter()
         iters = dIt.to_host()
         status = dSt.to_host()
-        conv = int(np.sum(status == 0))
+        gpu_rows[name] = (dV.to_host(), iters, status)
+        conv = status == 0
         wall = t1 - t0
         w = 4 if prec == abi.FP32 else 8
-        bytes_launch = algorithmic_bytes(info, iters, w)
+        bytes_conv = algorithmic_bytes(info, iters[conv], w)  # converged solves only
         kavg = ksum / args.steps
         results[name] = dict(
-            converged=conv, ninst=ninst, wall_s=wall, ms_per_step=1e3 * wall / args.steps,
-            solves_per_s=conv * args.steps / wall, kernel_ms=1e3 * kavg, iters_mean=float(np.mean(iters)),
-            achieved_gbs=bytes_launch / kavg / 1e9, bytes_per_launch=bytes_launch, lds_slots=st["lds_slots"],
-            instances_per_wg=st["instances_per_wg"], status_hist=np.bincount(status, minlength=8).tolist(),
-            autotune=tuned.get(name))
+            converged=int(conv.sum()), ninst=ninst, wall_s=wall, ms_per_step=1e3 * wall / args.steps,
+            solves_per_s=int(conv.sum()) * args.steps / wall, kernel_ms=1e3 * kavg,
+            iters_mean=float(np.mean(iters[conv])), achieved_gbs=bytes_conv / kavg / 1e9, bytes_per_launch=bytes_conv,
+            lds_slots=st["lds_slots"], instances_per_wg=st["instances_per_wg"], block_size=st["block_size"],
+            resident_workgroups=st["resident_workgroups"], fallback_fp64_per_step=fb / args.steps,
+            status_hist=np.bincount(status, minlength=9).tolist(), autotune=tuned.get(name),
+            phase_ms_wg0={k: 1e3 * v for k, v in st["phase_s"].items()})
 
-    cpu = None
+    cinfo = cpu_info()
+    nthreads = args.cpu_threads or cinfo["usable"]
+    cpu, par = None, None
     if not args.no_cpu:
         from oracle import oracle
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        # bounded sample: the full sweep on nthreads threads (each thread solves the base case
-        # first, like an independent ContingencyAnalysis object), plus a 1-thread sample
+        # the full sweep on every CPU this process may use (each thread solves the base case first,
+        # like an independent ContingencyAnalysis object), plus a 1-thread sample
         deltas = (dptr, dpos, dval)
-        _, _, _, cconv, _, wall_mt = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"],
-                                                        d["pv"], d["pq"], Vb, deltas, 10, 1e-8, False, False, nthreads)
+        Vr, itr, _, cconv, _, wall_mt = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"],
+                                                           d["pv"], d["pq"], Vb, deltas, 10, 1e-8, False, False,
+                                                           nthreads)
         k1 = min(256, ninst)
         sub = (dptr[:k1 + 1], dpos[:dptr[k1]], dval[:dptr[k1]])
         _, _, _, cconv1, _, wall_1 = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"],
                                                         d["pv"], d["pq"], Vb, sub, 10, 1e-8, False, False, 1)
-        cpu = dict(value=float(np.sum(cconv)) / wall_mt, unit="converged power flows/s", cores=nthreads,
-                   kind="port",
+        one = float(np.sum(cconv1)) / wall_1
+        value = float(np.sum(cconv)) / wall_mt
+        phys = cinfo["physical_cores"]
+        cpu = dict(value=value, unit="converged power flows/s", cores=nthreads, kind="port",
                    sample=f"oracle (C++ FP64 restatement of BaseNRAlgo<KLU> + ContingencyAnalysis loop), full "
                           f"{ninst}-outage sweep on {nthreads} threads, {wall_mt:.2f} s",
-                   one_thread=dict(value=float(np.sum(cconv1)) / wall_1, sample=f"{k1} outages, {wall_1:.2f} s"))
+                   host=cinfo, one_thread=dict(value=one, sample=f"{k1} outages, {wall_1:.2f} s",
+                                               ms_per_contingency=1e3 / one),
+                   all_physical_cores_extrapolated=(one * phys) if phys else None,
+                   speedup={"fp32": results["fp32"]["solves_per_s"] / value,
+                            "fp64": results["fp64"]["solves_per_s"] / value,
+                            "fp32_vs_all_cores_extrapolated": (results["fp32"]["solves_per_s"] / (one * phys))
+                            if phys else None},
+                   note="threads = the CPUs this process may use on the GPU box (affinity and cgroup quota); "
+                        "the all-core figure scales the 1-thread rate by the machine's physical cores (an upper "
+                        "bound for the CPU)")
+        par = {name: parity(*gpu_rows[name], Vr, itr, cconv) for name in ("fp32", "fp64")}
 
     extras = {}
     if not args.no_extra:
-        for name, fn in (("serving", lambda: serving_run(args.serving_seconds)),
-                         ("time_series_118", lambda: time_series_run(not args.no_cpu)),
-                         ("contingency_10000", lambda: large_grid_run(not args.no_cpu))):
+        for name, fn in (("single_pf_14", single_solve_latency),
+                         ("serving", lambda: serving_run(args.serving_seconds)),
+                         ("time_series_118", lambda: time_series_run(not args.no_cpu, nthreads)),
+                         ("contingency_10000", lambda: large_grid_run(not args.no_cpu, nthreads))):
             try:
                 extras[name] = fn()
             except Exception as exc:  # an extra configuration never hides the headline line
@@ -335,16 +474,20 @@ def main():
         "dtype": "f32 factor / f64 mismatch",
         "data": "synthetic",
         "config": {"workload": f"{args.grid} N-1 sweep: {ninst} non-bridge single-branch outages, warm start "
-                               f"from the base-case V, tol 1e-8 pu, max 10 iterations",
+                               f"from the base-case V, tol 1e-8 pu, max 10 iterations; distributed slack over the "
+                               f"reference slack and the 400 PV generators, loads and dispatch at 0.35 of the "
+                               f"generated values (full loading does not converge on this synthetic grid); FP32 "
+                               f"factors with FP64 re-solve of stalled rows",
                    "n_bus": g.n, "n_branch": g.nbr, "n_jac": info["n_jac"], "nnz_jac": info["nnz_jac"],
                    "nnz_lu": info["nnz_lu"], "instances_per_step": ninst},
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic("fp32") if args.grid == "case2000_synth" else None,
                      "traffic_source": "profiles/k_nr_case2000_n1_hbm_traffic.md",
-                     "bytes_per_launch": r32["bytes_per_launch"],
+                     "bytes_per_launch": r32["bytes_per_launch"], "priced_on": "converged solves",
                      "kernel": "k_nr<float,G,block,lean>", "kernel_ms": r32["kernel_ms"]},
         "cpu_baseline": cpu,
+        "parity": par,
         "fp64_form": {"value": r64["solves_per_s"], "ms_per_step": r64["ms_per_step"], "kernel_ms": r64["kernel_ms"],
                       "achieved_gbs": r64["achieved_gbs"], "iters_mean": r64["iters_mean"],
                       "converged": r64["converged"]},

@@ -11,16 +11,20 @@
  * Ybus is CSC with sorted row indices, the layout the reference's solvers take
  * (Eigen::SparseMatrix<cplx_type>, src/powerflow_algorithm/BaseAlgo.hpp:126-136).
  *
- * Which reference interface each entry point replaces:
+ * Which reference interface each entry point replaces (paths under src/):
  *   pfgpu_grid_load      BaseNRAlgo's first-call work: index prep (BaseNRAlgo.tpp:59-75),
  *                        J pattern + value map (BaseNRAlgo.tpp:396-635) and
  *                        KLULinearSolver::initialize / klu_analyze (KLUSolver.cpp:24-45)
+ *   pfgpu_grid_reset     BaseNRAlgo::reset / KLULinearSolver::reset (BaseNRAlgo.tpp:188-200,
+ *                        KLUSolver.cpp:14-22): the next solve factors afresh
  *   pfgpu_solve_batch    BaseNRAlgo<KLULinearSolver>::compute_pf (BaseNRAlgo.tpp:15-186) or
  *                        BaseNRSingleSlackAlgo::compute_pf (BaseNRSingleSlackAlgo.tpp:12-183)
  *                        run over many instances at once; with Ybus deltas it replaces the
- *                        contingency loop ContingencyAnalysis::compute (ContingencyAnalysis.cpp:204-231),
- *                        with per-instance Sbus the step loop TimeSeries::compute_Vs (TimeSeries.cpp:80-103)
- *   pfgpu_grid_get_J     BaseNRAlgo::get_J (BaseNRAlgo.hpp:36-45)
+ *                        contingency loop ContingencyAnalysis::compute (ContingencyAnalysis.cpp:204-231)
+ *   pfgpu_solve_chains   the warm-started step loop of TimeSeries::compute_Vs (TimeSeries.cpp:80-103):
+ *                        one chain is the reference's loop; several chains run side by side
+ *   pfgpu_grid_get_J_pattern + the Jx_last output of pfgpu_solve_batch
+ *                        BaseNRAlgo::get_J (BaseNRAlgo.hpp:36-45)
  */
 #ifndef PFGPU_H
 #define PFGPU_H
@@ -32,7 +36,7 @@
 extern "C" {
 #endif
 
-#define PFGPU_ABI_VERSION 1
+#define PFGPU_ABI_VERSION 2
 
 /* API error codes (negative). Positive codes are the reference's ErrorType values:
  * 0 NoError, 1 SingularMatrix, 2 TooManyIterations, 3 InifiniteValue, 4 SolverAnalyze,
@@ -44,9 +48,21 @@ extern "C" {
 #define PFGPU_E_INTERNAL (-5)
 
 /* precision of the linear solve (factor + triangular solves); the mismatch,
- * the Jacobian entries, the voltage update and the convergence test are FP64 */
+ * the Jacobian entries, the voltage update and the convergence test are FP64.
+ * PFGPU_FP32: rows that do not converge with FP32 factors and stalled (their mismatch went
+ * below sqrt(tol) on the way), lost a pivot to cancellation, met a zero / non-finite pivot or a
+ * non-finite mismatch are re-solved with FP64 factors from their initial voltages (chains from
+ * their heads); pfgpu_stats.fallback_fp64 counts them. PFGPU_FP32_STRICT: no re-solve. */
 #define PFGPU_FP64 0
 #define PFGPU_FP32 1
+#define PFGPU_FP32_STRICT 2
+
+/* per-row flags (pfgpu_ctx_last_flags) */
+#define PFGPU_FLAG_STALL 1   /* not converged, mismatch went below sqrt(tol) on the way */
+#define PFGPU_FLAG_GROWTH 2  /* a pivot kept less than 2^-10 (FP32) / 1e-10 (FP64) of its value */
+
+/* status of a chain step after the chain's first failure (not solved): NotInitError */
+#define PFGPU_STATUS_NOT_RUN 8
 
 typedef struct pfgpu_ctx pfgpu_ctx;
 typedef struct pfgpu_grid pfgpu_grid;
@@ -63,18 +79,24 @@ typedef struct pfgpu_grid_info {
     int64_t n_pairs;    /* multiply pairs of one refactorisation + forward solve */
     double flops;       /* flops of one refactorisation */
     int single_slack;
+    int dc;             /* loaded with pfgpu_grid_load_dc */
 } pfgpu_grid_info;
 
 /* Per-call timings (seconds) and launch choices. */
 typedef struct pfgpu_stats {
     double total_s;       /* whole call, host wall clock */
-    double h2d_s;         /* host -> device copies (0 for the device entry point) */
-    double kernel_s;      /* NR kernel time (HIP events on the context stream) */
+    double h2d_s;         /* host -> device copies (0 for the device entry points) */
+    double kernel_s;      /* NR launches (HIP events on the context stream), FP64 re-solve included */
     double d2h_s;         /* device -> host copies */
     int instances_per_wg; /* G chosen for the launch */
     int lds_slots;        /* value slots kept in LDS per instance */
-    int workgroups;
-    int fallback_fp64;    /* FP32-form instances re-solved in FP64 */
+    int workgroups;       /* instance groups of G */
+    int fallback_fp64;    /* FP32-form rows (or chains) re-solved in FP64 */
+    /* wall clock of workgroup 0 per NR phase, summed over its iterations and instances:
+     * J fill, factor + forward solve, dense tail, backward solve, voltage update, mismatch */
+    double phase_s[6];
+    int block_size;           /* lanes per workgroup */
+    int resident_workgroups;  /* workgroups launched (each loops over instance groups) */
 } pfgpu_stats;
 
 /* Thread-local message of the last failing call. */
@@ -89,13 +111,25 @@ int pfgpu_ctx_destroy(pfgpu_ctx* ctx);
 int pfgpu_ctx_device_info(pfgpu_ctx* ctx, int* n_cu, int* lds_bytes_per_wg, size_t* free_bytes);
 
 /* Load a grid: builds the J pattern / scatter map, the ordering, the symbolic
- * static-pivot LU and the level schedule on the host, and uploads them.
- * Yx (base values) may be NULL; values are given per solve call anyway. */
+ * static-pivot LU and the level schedule on the host, and uploads them. Only the Ybus
+ * pattern (Yp, Yi) is taken here; its values are given to every solve call. */
 int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi,
                     const int* slack_ids, int nslack, const double* slack_weights,
                     const int* pv, int npv, const int* pq, int npq,
                     int single_slack, pfgpu_grid** out);
+/* DC power flow grid (BaseDCAlgo::compute_pf, src/powerflow_algorithm/BaseDCAlgo.tpp:15-179):
+ * the solve calls then take the DC Ybus values, factor B = Re(Ybus) without the slack rows and
+ * columns (slack buses: in neither slack_ids[1:] ++ pv nor pq) and solve B theta = Re(Sbus) once
+ * per instance on the NR kernel's factor and solve phases; V = |V0| exp(j (theta + arg V0[first
+ * slack])), 1 iteration, status SolverSolve (7) if theta is not finite or |theta| >= 1e6.
+ * max_iter and tol are ignored, as in the reference; PFGPU_FP64 only. */
+int pfgpu_grid_load_dc(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+                       const int* pv, int npv, const int* pq, int npq, pfgpu_grid** out);
 int pfgpu_grid_destroy(pfgpu_grid* grid);
+/* The next solve on the grid factors afresh: a zero / non-finite pivot in its first
+ * iteration is SolverFactor (5), later ones SolverReFactor (6), as with klu_factor then
+ * klu_refactor. A loaded grid starts fresh. */
+int pfgpu_grid_reset(pfgpu_grid* grid);
 int pfgpu_grid_get_info(const pfgpu_grid* grid, pfgpu_grid_info* info);
 /* J pattern in the reference layout: Jp[n_jac+1], Ji[nnz_jac]. */
 int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji);
@@ -111,8 +145,7 @@ int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji);
  *   V         [ninst x n complex] in: initial voltages, out: final voltages
  *   v0_stride n: V holds one initial vector per instance; 0: V[0:n] is the
  *             initial vector of every instance (V still receives ninst rows)
- *   precision PFGPU_FP64 or PFGPU_FP32 (FP32 factor/solve, FP64 elsewhere; an
- *             instance that does not converge in FP32 is re-solved in FP64)
+ *   precision PFGPU_FP64, PFGPU_FP32 or PFGPU_FP32_STRICT (see above)
  *   instances_per_wg  0 = automatic, else G in {1,2,4,8,16,32}
  *   iters/status      [ninst] outputs: NR iterations (linear solves) and the
  *             ErrorType (0 converged)
@@ -132,9 +165,8 @@ int pfgpu_solve_batch(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
  * context's device); the call runs on the context's stream and returns after
  * the stream is synchronised. d_V0 (read only, one row per instance if
  * v0_stride == n, one shared row if 0) gives the initial voltages and d_V
- * [ninst x n complex] receives the results. No host copies and no FP64
- * re-solve pass: the status of FP32 instances that did not converge is
- * reported as is.
+ * [ninst x n complex] receives the results (d_V0 == d_V solves in place).
+ * The FP64 re-solve of PFGPU_FP32 runs on the device too.
  */
 int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
                              const double* d_Yx, int Yx_stride,
@@ -145,11 +177,35 @@ int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
                              int* d_iters, int* d_status, pfgpu_stats* stats);
 
 /*
+ * Time series (TimeSeries::compute_Vs, TimeSeries.cpp:80-103): nrows steps, each with its own
+ * Sbus row, cut into chains of chain_len consecutive steps (chain_len >= nrows: the reference's
+ * single chain). The head of chain c starts from V0 (row c if v0_stride == n, the shared row if
+ * 0); every later step starts from the previous step's converged V. A chain stops at its first
+ * step that does not converge; its later steps get PFGPU_STATUS_NOT_RUN, 0 iterations and zero
+ * voltages. All chains run in one launch, the steps of a chain one after the other inside it.
+ *   Yx      shared Ybus values [nnz_ybus complex]
+ *   Sbus    [nrows x n complex]; V [nrows x n complex] out; iters / status [nrows] out
+ */
+int pfgpu_solve_chains(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int chain_len, const double* Yx,
+                       const double* Sbus, const double* V0, int v0_stride, double* V, int max_iter, double tol,
+                       int precision, int instances_per_wg, int* iters, int* status, pfgpu_stats* stats);
+/* The same on device-resident arrays (d_V0 must not alias d_V; Yx_stride 0 or nnz_ybus per chain). */
+int pfgpu_solve_chains_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int chain_len, const double* d_Yx,
+                              int Yx_stride, const double* d_Sbus, const double* d_V0, int v0_stride, double* d_V,
+                              int max_iter, double tol, int precision, int instances_per_wg, int* d_iters,
+                              int* d_status, pfgpu_stats* stats);
+
+/* Per-row PFGPU_FLAG_* of the last solve call on the context (of its first pass: the FP32
+ * launch for PFGPU_FP32); nrows must not exceed that call's rows. */
+int pfgpu_ctx_last_flags(pfgpu_ctx* ctx, int* flags, long long nrows);
+
+/*
  * Launch autotuner: runs the batch described by the arguments (same meaning as
  * pfgpu_solve_batch_device; outputs are overwritten) once per candidate launch
  * shape — instances per workgroup G in {1, 2, 4, 8, 16, 32} x block size in
  * {256, 1024} — keeps the fastest for this grid and precision, and returns it.
- * Later solves with instances_per_wg == 0 use the tuned shape.
+ * Later solves with instances_per_wg == 0 use the tuned shape. The FP32 forms are
+ * timed without the FP64 re-solve pass.
  */
 int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
                           const double* d_Yx, int Yx_stride,
@@ -172,6 +228,103 @@ int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
 int pfgpu_branch_flows_device(pfgpu_ctx* ctx, int ninst, int n, const double* d_V, int nbr,
                               const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
                               const double* d_vn_kv_from, double sn_mva, double* d_amps, double* d_p);
+
+/*
+ * Batched branch flows of solved rows with the contingency clean-up, on device arrays
+ * (BaseBatchSolverSynch::compute_amps_flows / compute_active_power_flows,
+ * BaseBatchSolverSynch.hpp:86-192, then ContingencyAnalysis::clean_flows,
+ * ContingencyAnalysis.cpp:237-250). Output row r (of nrows) uses row d_row_src[r] of d_V
+ * ([rows x n] complex; d_row_src NULL: row r; -1: a row of zero voltages, whose amps are
+ * NaN as in the reference). Branch b has solver buses d_f[b] / d_t[b] (d_f[b] < 0: out of
+ * service, flows 0), complex yff / yft and vn_kv of its origin bus.
+ *   amps = (|S| sn_mva) / (sqrt(3) (|Ef| vn_kv)), p = Re(S) sn_mva, S = Ef conj(yff Ef + yft Et).
+ * d_clean_ptr[nrows + 1] / d_clean_br (or both NULL) list per row the branches whose finite
+ * flows are set to 0 (the outaged branches of a contingency).
+ */
+int pfgpu_batch_flows_device(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src, int nbr,
+                             const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
+                             const double* d_vn_kv_from, double sn_mva, const int* d_clean_ptr, const int* d_clean_br,
+                             double* d_amps, double* d_p);
+
+/*
+ * Time-series Sbus rows on the device (TimeSeries::compute_Vs, TimeSeries.cpp:45-53):
+ *   Sbus[t, b] = sum gen_p[t, g] + sum sgen_p[t, s] - sum load_p[t, l] - j sum load_q[t, l]
+ * over the elements at solver bus b (CSR lists ptr[ns + 1] / element ids, in element order),
+ * divided by sn_mva unless it is 1 (within 1e-7). Injection matrices row-major [T x elements];
+ * d_Sbus [T x ns] complex.
+ */
+int pfgpu_sbus_rows_device(pfgpu_ctx* ctx, int T, int ns, const int* d_gen_ptr, const int* d_gen_el, int ngen,
+                           const double* d_gen_p, const int* d_sgen_ptr, const int* d_sgen_el, int nsgen,
+                           const double* d_sgen_p, const int* d_load_ptr, const int* d_load_el, int nload,
+                           const double* d_load_p, const double* d_load_q, double sn_mva, double* d_Sbus);
+
+/*
+ * Ybus values from branch parameters on a fixed CSC pattern (the Ybus part of
+ * GridModel::pre_process_solver, GridModel.cpp:450-571; line stamps of LineContainer.cpp:13-34):
+ * branch k: ys = (r - j x) / (r^2 + x^2), yff = ytt = ys + h/2, yft = ytf = -ys. Entry e sums
+ * the contributions d_contrib[d_contrib_ptr[e] .. d_contrib_ptr[e+1]) in list order,
+ * contribution c = 4 k + {0 ff, 1 tt, 2 ft, 3 tf}; branches with d_on[k] == 0 add nothing.
+ * d_h complex [nbr]; d_Yx [nnz] complex out.
+ */
+int pfgpu_ybus_values_device(pfgpu_ctx* ctx, int nnz, const int* d_contrib_ptr, const int* d_contrib, int nbr,
+                             const double* d_r, const double* d_x, const double* d_h, const int* d_on, double* d_Yx);
+
+/*
+ * Contingency Ybus edits on the device (ContingencyAnalysis::init_li_coeffs + remove_from_Ybus,
+ * ContingencyAnalysis.cpp:47-110): contingency c removes the branches
+ * d_cont_br[d_cont_ptr[c] .. d_cont_ptr[c+1]); branch k has CSC positions d_br_pos[4k .. 4k+3]
+ * (ff, tt, ft, tf; -1: out of service, no edit) and complex values d_br_y[4k .. 4k+3]. Writes the
+ * edits of c at [d_dptr[c], d_dptr[c+1]) of d_dpos / d_dval (d_dptr given: 4 per in-service
+ * branch), the delta format of the solve entry points.
+ */
+int pfgpu_contingency_edits_device(pfgpu_ctx* ctx, int ncont, const int* d_cont_ptr, const int* d_cont_br,
+                                   const int* d_br_pos, const double* d_br_y, const int* d_dptr, int* d_dpos,
+                                   double* d_dval);
+
+/*
+ * Connectivity of the grid after each contingency (ContingencyAnalysis::check_invertible,
+ * ContingencyAnalysis.cpp:14-45): breadth-first search from bus 0 over the Ybus entries whose
+ * edited value (Yx minus the contingency's deltas, NULL deltas: none) has |y| > 1e-8;
+ * d_connected[c] = 1 when every bus is reached. One workgroup per contingency; the grid must
+ * fit the LDS (n up to about 16000 buses).
+ */
+int pfgpu_connectivity_device(pfgpu_ctx* ctx, int ncont, int n, const int* d_Yp, const int* d_Yi, const double* d_Yx,
+                              const int* d_dptr, const int* d_dpos, const double* d_dval, int* d_connected);
+
+/*
+ * Serving loop (continuous batching over grids resident on the device). Requests on one grid
+ * that are queued while a launch runs join the next launch, up to max_instances_per_launch and
+ * what the HBM budget admits; each grid has its own worker, stream and context, so launches of
+ * different grids overlap on the device. No counterpart in the reference, whose batch entry
+ * points (ContingencyAnalysisCPP.compute / TimeSeriesCPP.compute_Vs, src/main.cpp:1186,1234)
+ * are synchronous calls.
+ *   hbm_budget 0: a quarter of the free device memory; precision as for the solve calls
+ */
+typedef struct pfgpu_server pfgpu_server;
+#define PFGPU_REQ_PF 0            /* one power flow (Sbus NULL: the grid's resident Sbus) */
+#define PFGPU_REQ_CONTINGENCIES 1 /* ninst outages as Ybus deltas (dptr[ninst + 1], dpos, dval) */
+#define PFGPU_REQ_TIME_SERIES 2   /* ninst Sbus rows, each warm-started from the resident V */
+typedef struct pfgpu_request_info {
+    int ninst;             /* rows of the result */
+    int launch_instances;  /* instances of the launch the request rode in */
+    double latency_s;      /* submit -> result */
+    double queue_s;        /* submit -> launch start */
+} pfgpu_request_info;
+int pfgpu_server_create(size_t hbm_budget, int max_instances_per_launch, int precision, pfgpu_server** out);
+/* Stops the workers; requests still queued complete with an error. */
+int pfgpu_server_destroy(pfgpu_server* srv);
+/* Loads a grid (pattern analysis, schedules) and keeps its Ybus values, base Sbus and warm-start
+ * V0 resident; grid_id receives its id. */
+int pfgpu_server_add_grid(pfgpu_server* srv, int n, const int* Yp, const int* Yi, const double* Yx, const double* Sbus,
+                          const double* V0, const int* slack_ids, int nslack, const double* slack_weights, const int* pv,
+                          int npv, const int* pq, int npq, int max_iter, double tol, int* grid_id);
+int pfgpu_submit(pfgpu_server* srv, int grid_id, int kind, int ninst, const double* Sbus, const int* dptr,
+                 const int* dpos, const double* dval, uint64_t* request_id);
+/* Waits up to timeout_s (0: no wait, < 0: until done). ready = 1: the result was copied into V
+ * [ninst x n complex], iters and status [ninst] (any may be NULL) and the request is retired;
+ * ready = 0: not done yet. A failed request returns an error code once ready. */
+int pfgpu_poll(pfgpu_server* srv, uint64_t request_id, double timeout_s, int* ready, double* V, int* iters, int* status,
+               pfgpu_request_info* info);
 
 /* Device memory helpers bound to the context's device and stream. */
 int pfgpu_dev_alloc(pfgpu_ctx* ctx, size_t bytes, void** dptr);

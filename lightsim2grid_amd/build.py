@@ -29,10 +29,26 @@ def _run(cmd, **kw):
     subprocess.check_call(cmd, **kw)
 
 
+# translation units of libpfgpu.so: the HIP kernels and their C ABI, the host analysis, the
+# host mirror of the solver surface (over the C ABI) and the serving loop with its C ABI
+LIB_SOURCES = ["pfgpu.hip", "batchprep.hip", "analysis.cpp", "gridmodel.cpp", "serve.cpp", "serve_capi.cpp"]
+
+
 def build_lib() -> str:
-    srcs = [os.path.join(CSRC, "pfgpu.hip"), os.path.join(CSRC, "analysis.cpp")]
+    """hipcc per translation unit (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        _run([HIPCC, *HIP_FLAGS, f"-I{ROOT}/include", "-c", "-o", obj, os.path.join(CSRC, src)])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(LIB_SOURCES)) as ex:
+        objs = list(ex.map(compile_one, LIB_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs, "-Wl,-soname,libpfgpu.so"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs, "-Wl,-soname,libpfgpu.so", "-pthread"])
     os.replace(tmp, LIB)
     return LIB
 
@@ -49,7 +65,7 @@ def build_module() -> str:
         return ""
     out = module_path()
     inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{ROOT}/include", f"-I{CSRC}"]
-    srcs = [src] + [os.path.join(CSRC, f) for f in ("gridmodel.cpp", "serve.cpp") if os.path.exists(os.path.join(CSRC, f))]
+    srcs = [src]
     tmp = out + ".tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", *inc, *srcs, "-o", tmp,
           f"-L{HERE}", "-lpfgpu", "-Wl,-rpath,$ORIGIN", "-pthread"])

@@ -7,7 +7,6 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
-#include <queue>
 #include <sstream>
 
 namespace ls2g {
@@ -23,14 +22,20 @@ std::string solver_type_name(SolverType t) {
         "SparseLU", "KLU", "GaussSeidel", "DC", "GaussSeidelSynch", "NICSLU", "SparseLUSingleSlack",
         "KLUSingleSlack", "NICSLUSingleSlack", "KLUDC", "NICSLUDC", "CKTSO", "CKTSOSingleSlack", "CKTSODC",
         "FDPF_XB_SparseLU", "FDPF_BX_SparseLU", "FDPF_XB_KLU", "FDPF_BX_KLU", "FDPF_XB_NICSLU", "FDPF_BX_NICSLU",
-        "FDPF_XB_CKTSO", "FDPF_BX_CKTSO", "AMDGPU", "AMDGPUSingleSlack", "AMDGPUMixed", "AMDGPUMixedSingleSlack"};
+        "FDPF_XB_CKTSO", "FDPF_BX_CKTSO", "AMDGPU", "AMDGPUSingleSlack", "AMDGPUMixed", "AMDGPUMixedSingleSlack",
+        "AMDGPUDC"};
     const int k = (int)t;
     return (k >= 0 && k < (int)(sizeof(names) / sizeof(names[0]))) ? names[k] : "Unknown";
 }
 
 bool is_gpu_type(SolverType t) {
     return t == SolverType::AMDGPU || t == SolverType::AMDGPUSingleSlack || t == SolverType::AMDGPUMixed ||
-           t == SolverType::AMDGPUMixedSingleSlack;
+           t == SolverType::AMDGPUMixedSingleSlack || t == SolverType::AMDGPUDC;
+}
+// ChooseSolver::is_dc (ChooseSolver.hpp:88-97)
+bool is_dc_type(SolverType t) {
+    return t == SolverType::DC || t == SolverType::KLUDC || t == SolverType::NICSLUDC || t == SolverType::CKTSODC ||
+           t == SolverType::AMDGPUDC;
 }
 
 void check_rc(int rc, const char* what) {
@@ -53,6 +58,22 @@ pfgpu_ctx* engine_ctx() {
     });
     if (!ctx) throw std::runtime_error("AMDGPU solver: no usable GPU (" + err + ")");
     return ctx;
+}
+
+void* DevMem::get(size_t bytes) {
+    if (bytes > n_ || !p_) {
+        release();
+        void* q = nullptr;
+        check_rc(pfgpu_dev_alloc(engine_ctx(), std::max<size_t>(bytes, 1), &q), "pfgpu_dev_alloc");
+        p_ = q;
+        n_ = std::max<size_t>(bytes, 1);
+    }
+    return p_;
+}
+void DevMem::release() {
+    if (p_) pfgpu_dev_free(engine_ctx(), p_);
+    p_ = nullptr;
+    n_ = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -92,9 +113,14 @@ void GpuNRSolver::ensure_grid(const CscC& Y, const IntVect& slack_ids, const Rea
     if (grid_) pfgpu_grid_destroy(grid_);
     grid_ = nullptr;
     pfgpu_grid* g = nullptr;
-    check_rc(pfgpu_grid_load(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(), (int)slack_ids.size(),
-                             w.data(), pv.data(), (int)pv.size(), pq.data(), (int)pq.size(), single_ ? 1 : 0, &g),
-             "pfgpu_grid_load");
+    if (dc_)
+        check_rc(pfgpu_grid_load_dc(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(),
+                                    (int)slack_ids.size(), pv.data(), (int)pv.size(), pq.data(), (int)pq.size(), &g),
+                 "pfgpu_grid_load_dc");
+    else
+        check_rc(pfgpu_grid_load(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(), (int)slack_ids.size(),
+                                 w.data(), pv.data(), (int)pv.size(), pq.data(), (int)pq.size(), single_ ? 1 : 0, &g),
+                 "pfgpu_grid_load");
     grid_ = g;
     check_rc(pfgpu_grid_get_info(grid_, &info_), "pfgpu_grid_get_info");
     key_p_ = Y.p; key_i_ = Y.i; key_slack_ = slack_ids; key_pv_ = pv; key_pq_ = pq; key_w_ = w;
@@ -122,6 +148,7 @@ bool GpuNRSolver::compute_pf(const CscC& Ybus, CplxVect& V, const CplxVect& Sbus
     check_sizes(Ybus, V.size(), Sbus.size(), single_ ? "BaseNRSingleSlackAlgo::compute_pf" : "BaseNRAlgo::compute_pf");
     auto t0 = clk::now();
     timer_Fx_ = timer_solve_ = timer_check_ = timer_total_ = timer_initialize_ = timer_pre_proc_ = 0;
+    timer_refactor_ = timer_fillJ_ = timer_VaVm_ = 0;
     ensure_grid(Ybus, slack_ids, slack_weights, pv, pq);
     const int n = Ybus.ncol;
     V_ = V;
@@ -135,7 +162,11 @@ bool GpuNRSolver::compute_pf(const CscC& Ybus, CplxVect& V, const CplxVect& Sbus
              "pfgpu_solve_batch");
     nr_iter_ = it;
     err_ = (ErrorType)st;
-    timer_solve_ = stats.kernel_s;
+    timer_fillJ_ = stats.phase_s[0];
+    timer_refactor_ = stats.phase_s[1] + stats.phase_s[2];
+    timer_solve_ = stats.phase_s[3];
+    timer_VaVm_ = stats.phase_s[4];
+    timer_Fx_ = stats.phase_s[5];
     timer_pre_proc_ = stats.h2d_s;
     timer_total_ = since(t0);
     return err_ == ErrorType::NoError;
@@ -163,6 +194,31 @@ void GpuNRSolver::solve_batch(const CscC& Ybus, const IntVect& slack_ids, const 
              "pfgpu_solve_batch");
 }
 
+void GpuNRSolver::solve_batch_device(const CscC& Ybus, const IntVect& slack_ids, const RealVect& w, const IntVect& pv,
+                                     const IntVect& pq, int ninst, const double* d_Yx, const int* d_dptr,
+                                     const int* d_dpos, const double* d_dval, const double* d_S,
+                                     bool sbus_per_instance, const double* d_V0, bool v0_per_instance, double* d_V,
+                                     int max_iter, double tol, int* d_iters, int* d_status, pfgpu_stats* st) {
+    ensure_grid(Ybus, slack_ids, w, pv, pq);
+    if (ninst == 0) return;
+    const int n = Ybus.ncol;
+    check_rc(pfgpu_solve_batch_device(engine_ctx(), grid_, ninst, d_Yx, 0, d_dptr, d_dpos, d_dval, d_S,
+                                      sbus_per_instance ? n : 0, d_V0, v0_per_instance ? n : 0, d_V, max_iter, tol,
+                                      precision_, 0, d_iters, d_status, st),
+             "pfgpu_solve_batch_device");
+}
+
+void GpuNRSolver::solve_chains_device(const CscC& Ybus, const IntVect& slack_ids, const RealVect& w, const IntVect& pv,
+                                      const IntVect& pq, int nrows, int chain_len, const double* d_Yx,
+                                      const double* d_S, const double* d_V0, double* d_V, int max_iter, double tol,
+                                      int* d_iters, int* d_status, pfgpu_stats* st) {
+    ensure_grid(Ybus, slack_ids, w, pv, pq);
+    if (nrows == 0) return;
+    check_rc(pfgpu_solve_chains_device(engine_ctx(), grid_, nrows, chain_len, d_Yx, 0, d_S, d_V0, 0, d_V, max_iter, tol,
+                                       precision_, 0, d_iters, d_status, st),
+             "pfgpu_solve_chains_device");
+}
+
 CscR GpuNRSolver::get_J() const {
     CscR J;
     if (!grid_) return J;
@@ -183,11 +239,12 @@ ChooseSolver::ChooseSolver() : type_(SolverType::AMDGPU) {
     s_[1] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64);
     s_[2] = std::make_unique<GpuNRSolver>(false, PFGPU_FP32);
     s_[3] = std::make_unique<GpuNRSolver>(true, PFGPU_FP32);
+    s_[4] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64, true);
 }
 
 std::vector<SolverType> ChooseSolver::available_solvers() const {
     return {SolverType::AMDGPU, SolverType::AMDGPUSingleSlack, SolverType::AMDGPUMixed,
-            SolverType::AMDGPUMixedSingleSlack};
+            SolverType::AMDGPUMixedSingleSlack, SolverType::AMDGPUDC};
 }
 
 void ChooseSolver::change_solver(SolverType t) {
@@ -197,7 +254,7 @@ void ChooseSolver::change_solver(SolverType t) {
         std::ostringstream os;
         os << "Impossible to use the solver " << solver_type_name(t)
            << ": it is not available in this build (available: AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, "
-              "AMDGPUMixedSingleSlack).";
+              "AMDGPUMixedSingleSlack, AMDGPUDC).";
         throw std::runtime_error(os.str());
     }
     current().reset();
@@ -270,17 +327,29 @@ void GridModel::remove_gen_slackbus(int gen_id) {
 void GridModel::deactivate_powerline(int id) { line_status_.at(id) = false; }
 void GridModel::reactivate_powerline(int id) { line_status_.at(id) = true; }
 
+void GridModel::branch_admittance_dc(int k, cplx& yff, cplx& yft, cplx& ytf, cplx& ytt) const {
+    // TwoSidesContainer_rxh_A::_update_model_coeffs_one_el, DC part: 1 / x, no shunt
+    const double y = 1.0 / line_x_[k];
+    yff = ytt = cplx(y, 0.0);
+    yft = ytf = cplx(-y, 0.0);
+}
+
 void GridModel::branch_admittance(int k, cplx& yff, cplx& yft, cplx& ytf, cplx& ytt) const {
-    // TwoSidesContainer_rxh_A::_update_model_coeffs_one_el with h split 0.5 / 0.5 (LineContainer.cpp:13-34)
-    const cplx ys = 1.0 / cplx(line_r_[k], line_x_[k]);
-    yff = ys + 0.5 * line_h_[k];
-    ytt = ys + 0.5 * line_h_[k];
+    // TwoSidesContainer_rxh_A::_update_model_coeffs_one_el with h split 0.5 / 0.5 (LineContainer.cpp:13-34);
+    // ys = 1 / (r + j x) evaluated as (r - j x) / (r^2 + x^2), the operations (and their order) of the
+    // device Ybus assembly (pfgpu_ybus_values_device)
+    const double r = line_r_[k], x = line_x_[k];
+    const double d = r * r + x * x;
+    const cplx ys(r / d, -x / d);
+    const cplx hh(0.5 * line_h_[k].real(), 0.5 * line_h_[k].imag());
+    yff = cplx(ys.real() + hh.real(), ys.imag() + hh.imag());
+    ytt = yff;
     yft = -ys;
     ytf = -ys;
 }
 
 // GridModel::pre_process_solver (GridModel.cpp:450-571) for this subset.
-SolverInput GridModel::pre_process_solver(const CplxVect& Vinit) const {
+SolverInput GridModel::pre_process_solver(const CplxVect& Vinit, bool ac) const {
     const int nb = total_bus();
     if ((int)Vinit.size() != nb) {
         std::ostringstream os;
@@ -306,7 +375,8 @@ SolverInput GridModel::pre_process_solver(const CplxVect& Vinit) const {
     for (size_t k = 0; k < line_r_.size(); ++k) {
         if (!line_status_[k]) continue;
         cplx yff, yft, ytf, ytt;
-        branch_admittance((int)k, yff, yft, ytf, ytt);
+        if (ac) branch_admittance((int)k, yff, yft, ytf, ytt);
+        else branch_admittance_dc((int)k, yff, yft, ytf, ytt);
         const int f = in.id_me_to_solver[line_from_[k]], t = in.id_me_to_solver[line_to_[k]];
         add(f, f, yff); add(t, t, ytt); add(f, t, yft); add(t, f, ytf);
     }
@@ -317,7 +387,8 @@ SolverInput GridModel::pre_process_solver(const CplxVect& Vinit) const {
         bool diag = false;
         for (auto& e : v) diag |= (e.first == c);
         if (!diag) v.push_back({c, cplx(0, 0)});  // keep every diagonal entry structurally present
-        std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        // stable: duplicates (parallel branches, the diagonal) sum in branch order
+        std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
         for (size_t q = 0; q < v.size(); ++q) {
             if (!in.Ybus.i.empty() && (int)in.Ybus.i.size() > in.Ybus.p[c] && in.Ybus.i.back() == v[q].first) {
                 in.Ybus.x.back() += v[q].second;  // parallel elements sum up (setFromTriplets)
@@ -369,6 +440,19 @@ SolverInput GridModel::pre_process_solver(const CplxVect& Vinit) const {
     return in;
 }
 
+CplxVect GridModel::dc_pf(const CplxVect& Vinit, int max_iter, double tol) {
+    last_ = pre_process_solver(Vinit, false);
+    CplxVect V = last_.V;
+    const bool conv = dc_solver_->compute_pf(last_.Ybus, V, last_.Sbus, last_.slack_ids, last_.slack_weights, last_.pv,
+                                             last_.pq, max_iter, tol / sn_mva_);
+    CplxVect res;
+    if (!conv) return res;
+    V_solver_ = dc_solver_->get_V();
+    res.assign(total_bus(), cplx(0, 0));
+    for (size_t b = 0; b < last_.id_solver_to_me.size(); ++b) res[last_.id_solver_to_me[b]] = V_solver_[b];
+    return res;
+}
+
 CplxVect GridModel::ac_pf(const CplxVect& Vinit, int max_iter, double tol) {
     last_ = pre_process_solver(Vinit);
     CplxVect V = last_.V;
@@ -385,30 +469,50 @@ CplxVect GridModel::ac_pf(const CplxVect& Vinit, int max_iter, double tol) {
 // ---------------------------------------------------------------------------
 // Batch drivers
 // ---------------------------------------------------------------------------
-// amps / active power at the origin side of every line for every row of voltages
-// (BaseBatchSolverSynch.hpp:86-192): I = yff Ef + yft Et, S = Ef conj(I);
-// A = |S| sn_mva / (sqrt(3) |Ef| vn_kv(from)), P = Re(S) sn_mva.
+void BaseBatchSolverSynch::keep_device_rows(int ns, const IntVect& row_src) {
+    ns_ = ns;
+    d_row_src_.upload(row_src);
+}
+
+// Amps / active power at the origin side of every line for every result row
+// (BaseBatchSolverSynch.hpp:86-192), on the device from the voltages the last compute left there:
+// I = yff Ef + yft Et, S = Ef conj(I); A = |S| sn_mva / (sqrt(3) |Ef| vn_kv(from)), P = Re(S) sn_mva.
+// Rows that did not converge have zero voltages (NaN amps); lines out of service give 0.
 void BaseBatchSolverSynch::compute_flows_from_Vs(bool amps) {
-    if (voltages_.empty())
+    if (voltages_.empty() || ns_ == 0)
         throw std::runtime_error("BaseMultiplePowerflow::compute_flows_from_Vs: cannot compute the flows as the "
                                  "voltages are not set. Have you called compute(...) ? ");
     auto t0 = clk::now();
-    const int nl = grid_model_.nb_powerline(), nb = grid_model_.total_bus();
-    const double sn = grid_model_.get_sn_mva(), s3 = std::sqrt(3.0);
-    RealVect& out = amps ? amps_ : pflows_;
-    out.assign((size_t)nrows_ * nl, 0.0);
+    const int nl = grid_model_.nb_powerline();
+    IntVect f(nl, -1), t(nl, -1);
+    CplxVect yff(nl), yft(nl);
+    RealVect vn(nl, 1.0);
     for (int k = 0; k < nl; ++k) {
         if (!grid_model_.line_status(k)) continue;
-        cplx yff, yft, ytf, ytt;
-        grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
-        const int f = grid_model_.line_from(k), t = grid_model_.line_to(k);
-        const double vn = grid_model_.bus_vn_kv(f);
-        for (int r = 0; r < nrows_; ++r) {
-            const cplx Ef = voltages_[(size_t)r * nb + f], Et = voltages_[(size_t)r * nb + t];
-            const cplx S = Ef * std::conj(yff * Ef + yft * Et);
-            out[(size_t)r * nl + k] = amps ? std::abs(S) * sn / (s3 * std::abs(Ef) * vn) : S.real() * sn;
-        }
+        cplx ytf, ytt;
+        grid_model_.branch_admittance(k, yff[k], yft[k], ytf, ytt);
+        f[k] = id_me_to_solver_[grid_model_.line_from(k)];
+        t[k] = id_me_to_solver_[grid_model_.line_to(k)];
+        vn[k] = grid_model_.bus_vn_kv(grid_model_.line_from(k));
     }
+    IntVect cptr, cbr;
+    clean_lists(cptr, cbr);
+    const bool clean = !cbr.empty();
+    int* df = d_tmp_[0].upload(f);
+    int* dt = d_tmp_[1].upload(t);
+    double* dyff = reinterpret_cast<double*>(d_tmp_[2].upload(yff));
+    double* dyft = reinterpret_cast<double*>(d_tmp_[3].upload(yft));
+    double* dvn = d_tmp_[4].upload(vn);
+    int* dcp = clean ? d_tmp_[5].upload(cptr) : nullptr;
+    int* dcb = clean ? d_tmp_[6].upload(cbr) : nullptr;
+    RealVect& out = amps ? amps_ : pflows_;
+    out.assign((size_t)nrows_ * nl, 0.0);
+    double* dout = static_cast<double*>(d_tmp_[7].get(std::max<size_t>(out.size(), 1) * 8));
+    check_rc(pfgpu_batch_flows_device(engine_ctx(), nrows_, ns_, d_V_.as<double>(), d_row_src_.as<int>(), nl, df, dt,
+                                      dyff, dyft, dvn, grid_model_.get_sn_mva(), dcp, dcb, amps ? dout : nullptr,
+                                      amps ? nullptr : dout),
+             "pfgpu_batch_flows_device");
+    if (!out.empty()) check_rc(pfgpu_memcpy_d2h(engine_ctx(), out.data(), dout, out.size() * 8), "d2h");
     if (amps) timer_amps_ = since(t0);
 }
 
@@ -443,6 +547,7 @@ void ContingencyAnalysis::clear() {
 void ContingencyAnalysis::clear_results_only() {
     voltages_.clear(); amps_.clear(); pflows_.clear();
     nrows_ = 0;
+    ns_ = 0;
     nb_solved_ = 0;
     timer_total_ = timer_solver_ = timer_pre_proc_ = timer_amps_ = timer_modif_ybus_ = 0;
 }
@@ -453,62 +558,96 @@ std::vector<IntVect> ContingencyAnalysis::my_defaults() const {
 }
 
 namespace {
-// per-contingency Ybus coefficients (ContingencyAnalysis::init_li_coeffs), as CSC value positions
-struct ContDeltas {
-    IntVect dptr, dpos;
-    CplxVect dval;
-};
 int csc_pos(const CscC& Y, int r, int c) {
     auto b = Y.i.begin() + Y.p[c], e = Y.i.begin() + Y.p[c + 1];
     auto it = std::lower_bound(b, e, r);
     if (it == e || *it != r) throw std::runtime_error("ContingencyAnalysis: coefficient not in the Ybus pattern");
     return (int)(it - Y.i.begin());
 }
-// BFS with |y| > 1e-8 (ContingencyAnalysis::check_invertible)
-bool connected(const CscC& Y, const CplxVect& x) {
-    const int n = Y.ncol;
-    std::vector<char> visited(n, 0), added(n, 0);
-    std::queue<int> q;
-    int col = 0;
-    while (true) {
-        visited[col] = 1;
-        for (int k = Y.p[col]; k < Y.p[col + 1]; ++k) {
-            const int r = Y.i[k];
-            if (!visited[r] && !added[r] && std::abs(x[k]) > 1e-8) { q.push(r); added[r] = 1; }
-        }
-        if (q.empty()) break;
-        col = q.front();
-        q.pop();
+}  // namespace
+
+// ContingencyAnalysis::init_li_coeffs (ContingencyAnalysis.cpp:47-110): the Ybus positions and values
+// of each line's four coefficients, -1 for a line out of service
+void ContingencyAnalysis::branch_tables(const SolverInput& in, IntVect& pos, CplxVect& y) const {
+    const int nl = grid_model_.nb_powerline();
+    pos.assign(4 * (size_t)nl, -1);
+    y.assign(4 * (size_t)nl, cplx(0, 0));
+    for (int k = 0; k < nl; ++k) {
+        if (!grid_model_.line_status(k)) continue;
+        cplx yff, yft, ytf, ytt;
+        grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
+        const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
+        pos[4 * k] = csc_pos(in.Ybus, f, f);
+        pos[4 * k + 1] = csc_pos(in.Ybus, t, t);
+        pos[4 * k + 2] = csc_pos(in.Ybus, f, t);
+        pos[4 * k + 3] = csc_pos(in.Ybus, t, f);
+        y[4 * k] = yff;
+        y[4 * k + 1] = ytt;
+        y[4 * k + 2] = yft;
+        y[4 * k + 3] = ytf;
     }
-    for (char v : visited) if (!v) return false;
-    return true;
+}
+
+void ContingencyAnalysis::clean_lists(IntVect& ptr, IntVect& br) const {
+    ptr.assign(1, 0);
+    br.clear();
+    for (const auto& cont : defaults_) {
+        for (int k : cont) br.push_back(k);
+        ptr.push_back((int)br.size());
+    }
+}
+
+namespace {
+// contingency CSR (branch lists) and edit offsets (4 per in-service branch) of a list of contingencies
+struct ContLists {
+    IntVect ptr{0}, br, dptr{0};
+};
+void add_cont(ContLists& L, const std::set<int>& cont, const IntVect& pos) {
+    int ne = 0;
+    for (int k : cont) {
+        L.br.push_back(k);
+        if (pos[4 * k] >= 0) ne += 4;
+    }
+    L.ptr.push_back((int)L.br.size());
+    L.dptr.push_back(L.dptr.back() + ne);
 }
 }  // namespace
 
 IntVect ContingencyAnalysis::is_grid_connected_after_contingency() {
     SolverInput in = grid_model_.pre_process_solver(CplxVect(grid_model_.total_bus(), cplx(1, 0)));
-    IntVect res;
-    CplxVect x = in.Ybus.x;
-    for (const auto& cont : defaults_) {
-        std::vector<std::pair<int, cplx>> edits;
-        for (int k : cont) {
-            if (!grid_model_.line_status(k)) continue;
-            cplx yff, yft, ytf, ytt;
-            grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
-            const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
-            edits.push_back({csc_pos(in.Ybus, f, f), yff});
-            edits.push_back({csc_pos(in.Ybus, t, t), ytt});
-            edits.push_back({csc_pos(in.Ybus, f, t), yft});
-            edits.push_back({csc_pos(in.Ybus, t, f), ytf});
-        }
-        for (auto& e : edits) x[e.first] -= e.second;
-        res.push_back(connected(in.Ybus, x) ? 1 : 0);
-        for (auto& e : edits) x[e.first] += e.second;
-    }
+    IntVect pos;
+    CplxVect y;
+    branch_tables(in, pos, y);
+    ContLists L;
+    for (const auto& cont : defaults_) add_cont(L, cont, pos);
+    const int ncont = (int)defaults_.size();
+    IntVect res(ncont, 0);
+    if (ncont == 0) return res;
+    DevMem m[9];
+    const int n = in.Ybus.ncol;
+    int* dcp = m[0].upload(L.ptr);
+    int* dcb = m[1].upload(L.br);
+    int* dpos_br = m[2].upload(pos);
+    double* dy = reinterpret_cast<double*>(m[3].upload(y));
+    int* ddp = m[4].upload(L.dptr);
+    int* ddq = static_cast<int*>(m[5].get(std::max(L.dptr.back(), 1) * sizeof(int)));
+    double* ddv = static_cast<double*>(m[6].get(std::max(L.dptr.back(), 1) * 16));
+    check_rc(pfgpu_contingency_edits_device(engine_ctx(), ncont, dcp, dcb, dpos_br, dy, ddp, ddq, ddv),
+             "pfgpu_contingency_edits_device");
+    int* dYp = m[7].upload(in.Ybus.p);
+    int* dYi = m[8].upload(in.Ybus.i);
+    DevMem mx, mc;
+    double* dYx = reinterpret_cast<double*>(mx.upload(in.Ybus.x));
+    int* dconn = static_cast<int*>(mc.get(ncont * sizeof(int)));
+    check_rc(pfgpu_connectivity_device(engine_ctx(), ncont, n, dYp, dYi, dYx, ddp, ddq, ddv, dconn),
+             "pfgpu_connectivity_device");
+    check_rc(pfgpu_memcpy_d2h(engine_ctx(), res.data(), dconn, ncont * sizeof(int)), "d2h");
     return res;
 }
 
-// ContingencyAnalysis::compute (ContingencyAnalysis.cpp:166-233) as one batched GPU call.
+// ContingencyAnalysis::compute (ContingencyAnalysis.cpp:166-233), on the device: the contingency Ybus
+// edits and the connectivity check of every contingency, then one batched solve of the connected
+// ones; the voltages stay on the device for the flows.
 void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double tol) {
     auto t_all = clk::now();
     auto t_pre = clk::now();
@@ -519,99 +658,117 @@ void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double to
     const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
     const int ncont = (int)defaults_.size();
     nrows_ = ncont;
+    ns_ = 0;
+    id_me_to_solver_ = in.id_me_to_solver;
     voltages_.assign((size_t)ncont * nb, cplx(0, 0));
     amps_.clear();
     pflows_.clear();
     GpuNRSolver& s = solver_.current();
+    if (s.is_dc()) throw std::runtime_error("ContingencyAnalysis: the DC solver is not available in the batch classes of this build");
     s.reset();
     // base case ("powerflow in n", BaseBatchSolverSynch.hpp:296-313): tol as given
     CplxVect Vb = in.V;
     const bool base_conv = s.compute_pf(in.Ybus, Vb, in.Sbus, in.slack_ids, in.slack_weights, in.pv, in.pq, max_iter, tol);
     timer_solver_ += s.get_computation_time();
-    CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
+    const CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
     timer_pre_proc_ = since(t_pre);
     if (!base_conv) { timer_total_ = since(t_all); return; }
-    // Ybus edits and connectivity
+    // Ybus edits and connectivity of every contingency (device)
     auto t_y = clk::now();
-    ContDeltas D;
-    D.dptr.push_back(0);
+    IntVect pos;
+    CplxVect y;
+    branch_tables(in, pos, y);
+    ContLists all;
+    for (const auto& cont : defaults_) add_cont(all, cont, pos);
+    const int n = in.Ybus.ncol;
+    pfgpu_ctx* ctx = engine_ctx();
+    IntVect conn(ncont, 0);
+    DevMem m[9];
+    double* dYx = reinterpret_cast<double*>(d_Y_.upload(in.Ybus.x));
+    int* dpos_br = m[2].upload(pos);
+    double* dy = reinterpret_cast<double*>(m[3].upload(y));
+    if (ncont > 0) {
+        int* dcp = m[0].upload(all.ptr);
+        int* dcb = m[1].upload(all.br);
+        int* ddp = m[4].upload(all.dptr);
+        int* ddq = static_cast<int*>(m[5].get(std::max(all.dptr.back(), 1) * sizeof(int)));
+        double* ddv = static_cast<double*>(m[6].get(std::max(all.dptr.back(), 1) * 16));
+        check_rc(pfgpu_contingency_edits_device(ctx, ncont, dcp, dcb, dpos_br, dy, ddp, ddq, ddv),
+                 "pfgpu_contingency_edits_device");
+        int* dYp = m[7].upload(in.Ybus.p);
+        int* dYi = m[8].upload(in.Ybus.i);
+        DevMem mc;
+        int* dconn = static_cast<int*>(mc.get(ncont * sizeof(int)));
+        check_rc(pfgpu_connectivity_device(ctx, ncont, n, dYp, dYi, dYx, ddp, ddq, ddv, dconn),
+                 "pfgpu_connectivity_device");
+        check_rc(pfgpu_memcpy_d2h(ctx, conn.data(), dconn, ncont * sizeof(int)), "d2h");
+    }
+    // the connected contingencies, in order, and their edits (device)
     IntVect which;
-    CplxVect x = in.Ybus.x;
-    int c = 0;
-    for (const auto& cont : defaults_) {
-        std::vector<std::pair<int, cplx>> edits;
-        for (int k : cont) {
-            if (!grid_model_.line_status(k)) continue;
-            cplx yff, yft, ytf, ytt;
-            grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
-            const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
-            edits.push_back({csc_pos(in.Ybus, f, f), yff});
-            edits.push_back({csc_pos(in.Ybus, t, t), ytt});
-            edits.push_back({csc_pos(in.Ybus, f, t), yft});
-            edits.push_back({csc_pos(in.Ybus, t, f), ytf});
+    ContLists sel;
+    {
+        int c = 0;
+        for (const auto& cont : defaults_) {
+            if (conn[c]) {
+                which.push_back(c);
+                add_cont(sel, cont, pos);
+            }
+            ++c;
         }
-        for (auto& e : edits) x[e.first] -= e.second;
-        const bool inv = connected(in.Ybus, x);
-        for (auto& e : edits) x[e.first] += e.second;
-        if (inv) {
-            which.push_back(c);
-            for (auto& e : edits) { D.dpos.push_back(e.first); D.dval.push_back(e.second); }
-            D.dptr.push_back((int)D.dpos.size());
-        }
-        ++c;
+    }
+    const int m_solve = (int)which.size();
+    int* sdp = d_tmp_[0].upload(sel.dptr);
+    int* sdq = static_cast<int*>(d_tmp_[1].get(std::max(sel.dptr.back(), 1) * sizeof(int)));
+    double* sdv = static_cast<double*>(d_tmp_[2].get(std::max(sel.dptr.back(), 1) * 16));
+    if (m_solve > 0) {
+        int* scp = m[0].upload(sel.ptr);
+        int* scb = m[1].upload(sel.br);
+        check_rc(pfgpu_contingency_edits_device(ctx, m_solve, scp, scb, dpos_br, dy, sdp, sdq, sdv),
+                 "pfgpu_contingency_edits_device");
     }
     timer_modif_ybus_ = since(t_y);
     // batched solve, tol / sn_mva (ContingencyAnalysis.cpp:223)
-    CplxVect Vout;
-    IntVect iters, status;
-    pfgpu_stats st{};
     auto t_s = clk::now();
-    s.solve_batch(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, (int)which.size(), &D.dptr, &D.dpos, &D.dval,
-                  in.Sbus, false, Vstart, false, max_iter, tol / sn, Vout, iters, status, &st);
+    double* dS = reinterpret_cast<double*>(d_S_.upload(in.Sbus));
+    double* dV0 = reinterpret_cast<double*>(d_V0_.upload(Vstart));
+    double* dV = static_cast<double*>(d_V_.get((size_t)std::max(m_solve, 1) * ns * 16));
+    int* dIt = static_cast<int*>(d_iters_.get((size_t)std::max(m_solve, 1) * sizeof(int)));
+    int* dSt = static_cast<int*>(d_status_.get((size_t)std::max(m_solve, 1) * sizeof(int)));
+    pfgpu_stats st{};
+    s.solve_batch_device(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, m_solve, dYx, sdp, sdq, sdv, dS, false,
+                         dV0, false, dV, max_iter, tol / sn, dIt, dSt, &st);
+    CplxVect Vout((size_t)m_solve * ns);
+    IntVect status(m_solve);
+    if (m_solve > 0) {
+        check_rc(pfgpu_memcpy_d2h(ctx, Vout.data(), dV, Vout.size() * 16), "d2h");
+        check_rc(pfgpu_memcpy_d2h(ctx, status.data(), dSt, status.size() * sizeof(int)), "d2h");
+    }
     timer_solver_ += since(t_s);
-    nb_solved_ = (int)which.size();
-    for (size_t w = 0; w < which.size(); ++w) {
+    nb_solved_ = m_solve;
+    IntVect row_src(ncont, -1);
+    for (int w = 0; w < m_solve; ++w) {
         if (status[w] != 0) continue;
         const int row = which[w];
-        for (int b = 0; b < ns; ++b) voltages_[(size_t)row * nb + in.id_solver_to_me[b]] = Vout[w * ns + b];
+        row_src[row] = w;
+        for (int b = 0; b < ns; ++b) voltages_[(size_t)row * nb + in.id_solver_to_me[b]] = Vout[(size_t)w * ns + b];
     }
+    keep_device_rows(ns, row_src);
     timer_total_ = since(t_all);
-}
-
-// ContingencyAnalysis::clean_flows (ContingencyAnalysis.cpp:237-250)
-void ContingencyAnalysis::clean_flows(bool amps) {
-    RealVect& out = amps ? amps_ : pflows_;
-    const int nl = grid_model_.nb_powerline();
-    int c = 0;
-    for (const auto& cont : defaults_) {
-        for (int k : cont) {
-            double& el = out[(size_t)c * nl + k];
-            if (std::isfinite(el)) el = 0.0;
-        }
-        ++c;
-    }
-}
-void ContingencyAnalysis::compute_flows() {
-    compute_flows_from_Vs(true);
-    clean_flows(true);
-}
-void ContingencyAnalysis::compute_power_flows() {
-    compute_flows_from_Vs(false);
-    clean_flows(false);
 }
 
 void TimeSeries::clear() {
     voltages_.clear(); amps_.clear(); pflows_.clear(); sbuses_.clear();
     nrows_ = 0;
+    ns_ = 0;
     nb_solved_ = 0;
     status_ = 1;
     timer_total_ = timer_solver_ = timer_pre_proc_ = timer_amps_ = 0;
 }
 
-// TimeSeries::compute_Vs (TimeSeries.cpp:14-109). The steps are cut in chains of
-// chain_length consecutive steps; chain heads start from the initial V, every
-// other step from the previous step's V of its chain; all chains advance
-// together, one batched GPU call per position in the chain.
+// TimeSeries::compute_Vs (TimeSeries.cpp:14-109) on the device: the Sbus rows are built there from
+// the injections, then the warm-started step loop runs inside one launch (pfgpu_solve_chains):
+// by default as the reference's single chain; with set_chain_length(L), chains of L steps side by
+// side, each chain head starting from the initial V.
 int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p, const RealVect& load_p,
                            const RealVect& load_q, const CplxVect& Vinit, int max_iter, double tol) {
     auto t_all = clk::now();
@@ -627,21 +784,38 @@ int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p,
     SolverInput in = grid_model_.pre_process_solver(Vinit);
     const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
     nbus_solver_ = ns;
+    id_me_to_solver_ = in.id_me_to_solver;
+    ns_ = 0;
+    // element lists per solver bus (element order), for the device Sbus rows
+    auto lists = [&](int count, auto bus_of, IntVect& ptr, IntVect& el) {
+        std::vector<IntVect> per(ns);
+        for (int e = 0; e < count; ++e) per[in.id_me_to_solver[bus_of(e)]].push_back(e);
+        ptr.assign(1, 0);
+        el.clear();
+        for (int b = 0; b < ns; ++b) {
+            el.insert(el.end(), per[b].begin(), per[b].end());
+            ptr.push_back((int)el.size());
+        }
+    };
+    IntVect gp, ge, sp, se, lp, le;
+    lists(ng, [&](int e) { return grid_model_.gen_bus(e); }, gp, ge);
+    lists(nsg, [&](int e) { return grid_model_.sgen_bus(e); }, sp, se);
+    lists(nl, [&](int e) { return grid_model_.load_bus(e); }, lp, le);
+    pfgpu_ctx* ctx = engine_ctx();
+    DevMem m[10];
+    double* dS = static_cast<double*>(d_S_.get((size_t)std::max(T, 1) * ns * 16));
+    check_rc(pfgpu_sbus_rows_device(ctx, T, ns, m[0].upload(gp), m[1].upload(ge), ng, m[2].upload(gen_p),
+                                    m[3].upload(sp), m[4].upload(se), nsg, m[5].upload(sgen_p), m[6].upload(lp),
+                                    m[7].upload(le), nl, m[8].upload(load_p), m[9].upload(load_q), sn, dS),
+             "pfgpu_sbus_rows_device");
     sbuses_.assign((size_t)T * ns, cplx(0, 0));
-    for (int t = 0; t < T; ++t) {
-        cplx* S = &sbuses_[(size_t)t * ns];
-        for (int g = 0; g < ng; ++g) S[in.id_me_to_solver[grid_model_.gen_bus(g)]] += gen_p[(size_t)t * ng + g];
-        for (int g = 0; g < nsg; ++g) S[in.id_me_to_solver[grid_model_.sgen_bus(g)]] += sgen_p[(size_t)t * nsg + g];
-        for (int l = 0; l < nl; ++l)
-            S[in.id_me_to_solver[grid_model_.load_bus(l)]] -= cplx(load_p[(size_t)t * nl + l], load_q[(size_t)t * nl + l]);
-        if (std::fabs(sn - 1.0) > kTolEqualFloat)
-            for (int b = 0; b < ns; ++b) S[b] /= sn;
-    }
+    if (T > 0) check_rc(pfgpu_memcpy_d2h(ctx, sbuses_.data(), dS, sbuses_.size() * 16), "d2h");
     nrows_ = T;
     voltages_.assign((size_t)T * nb, cplx(0, 0));
     amps_.clear();
     pflows_.clear();
     GpuNRSolver& s = solver_.current();
+    if (s.is_dc()) throw std::runtime_error("TimeSeries: the DC solver is not available in the batch classes of this build");
     s.reset();
     CplxVect Vb = in.V;
     const bool base_conv = s.compute_pf(in.Ybus, Vb, in.Sbus, in.slack_ids, in.slack_weights, in.pv, in.pq, max_iter, tol);
@@ -649,50 +823,33 @@ int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p,
     timer_pre_proc_ = since(t_pre);
     if (!base_conv) { status_ = 0; timer_total_ = since(t_all); return -1; }
     const CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
-    const int L = std::max(1, std::min(chain_len_ > 0 ? chain_len_ : T, T));
-    const int nch = (T + L - 1) / L;
-    CplxVect Vcur((size_t)nch * ns);
-    for (int c = 0; c < nch; ++c) std::copy(Vstart.begin(), Vstart.end(), Vcur.begin() + (size_t)c * ns);
-    std::vector<int> conv_step(T, 0), iters_step(T, 0);
-    for (int pos = 0; pos < L; ++pos) {
-        // chains that still have a step at this position
-        IntVect steps;
-        for (int c = 0; c < nch; ++c) {
-            const int t = c * L + pos;
-            if (t < T) steps.push_back(t);
-        }
-        if (steps.empty()) break;
-        const int m = (int)steps.size();
-        CplxVect Sb((size_t)m * ns), V0((size_t)m * ns), Vout;
-        for (int k = 0; k < m; ++k) {
-            std::copy(sbuses_.begin() + (size_t)steps[k] * ns, sbuses_.begin() + (size_t)(steps[k] + 1) * ns,
-                      Sb.begin() + (size_t)k * ns);
-            std::copy(Vcur.begin() + (size_t)(steps[k] / L) * ns, Vcur.begin() + (size_t)(steps[k] / L + 1) * ns,
-                      V0.begin() + (size_t)k * ns);
-        }
-        IntVect it, stt;
-        pfgpu_stats st{};
-        auto t_s = clk::now();
-        s.solve_batch(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, m, nullptr, nullptr, nullptr, Sb, true, V0,
-                      true, max_iter, tol / sn, Vout, it, stt, &st);
-        timer_solver_ += since(t_s);
-        for (int k = 0; k < m; ++k) {
-            const int t = steps[k], c = t / L;
-            conv_step[t] = stt[k] == 0;
-            iters_step[t] = it[k];
-            // compute_one_powerflow only takes V from the solver on convergence (BaseBatchSolverSynch.cpp:37-39)
-            if (conv_step[t]) std::copy(Vout.begin() + (size_t)k * ns, Vout.begin() + (size_t)(k + 1) * ns,
-                                        Vcur.begin() + (size_t)c * ns);
-            for (int b = 0; b < ns && conv_step[t]; ++b)
-                voltages_[(size_t)t * nb + in.id_solver_to_me[b]] = Vout[(size_t)k * ns + b];
-        }
+    const int L = std::max(1, std::min(chain_len_ > 0 ? chain_len_ : T, std::max(T, 1)));
+    auto t_s = clk::now();
+    double* dYx = reinterpret_cast<double*>(d_Y_.upload(in.Ybus.x));
+    double* dV0 = reinterpret_cast<double*>(d_V0_.upload(Vstart));
+    double* dV = static_cast<double*>(d_V_.get((size_t)std::max(T, 1) * ns * 16));
+    int* dIt = static_cast<int*>(d_iters_.get((size_t)std::max(T, 1) * sizeof(int)));
+    int* dSt = static_cast<int*>(d_status_.get((size_t)std::max(T, 1) * sizeof(int)));
+    pfgpu_stats st{};
+    s.solve_chains_device(in.Ybus, in.slack_ids, in.slack_weights, in.pv, in.pq, T, L, dYx, dS, dV0, dV, max_iter,
+                          tol / sn, dIt, dSt, &st);
+    CplxVect Vout((size_t)T * ns);
+    IntVect stt(T);
+    if (T > 0) {
+        check_rc(pfgpu_memcpy_d2h(ctx, Vout.data(), dV, Vout.size() * 16), "d2h");
+        check_rc(pfgpu_memcpy_d2h(ctx, stt.data(), dSt, stt.size() * sizeof(int)), "d2h");
     }
+    timer_solver_ += since(t_s);
     // the reference stops at the first divergence (TimeSeries.cpp:94-99)
     int first_fail = T;
     for (int t = 0; t < T; ++t)
-        if (!conv_step[t]) { first_fail = t; break; }
-    for (int t = first_fail; t < T; ++t)
-        std::fill(voltages_.begin() + (size_t)t * nb, voltages_.begin() + (size_t)(t + 1) * nb, cplx(0, 0));
+        if (stt[t] != 0) { first_fail = t; break; }
+    IntVect row_src(T, -1);
+    for (int t = 0; t < first_fail; ++t) {
+        row_src[t] = t;
+        for (int b = 0; b < ns; ++b) voltages_[(size_t)t * nb + in.id_solver_to_me[b]] = Vout[(size_t)t * ns + b];
+    }
+    keep_device_rows(ns, row_src);
     nb_solved_ = std::min(T, first_fail + 1);
     status_ = first_fail == T ? 1 : 0;
     timer_total_ = since(t_all);

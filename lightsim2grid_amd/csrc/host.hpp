@@ -12,6 +12,7 @@
 //   BaseBatchSolverSynch            src/batch_algorithm/BaseBatchSolverSynch.{hpp,cpp}
 #pragma once
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <memory>
@@ -38,8 +39,9 @@ enum class SolverType : int {
     CKTSO, CKTSOSingleSlack, CKTSODC,
     FDPF_XB_SparseLU, FDPF_BX_SparseLU, FDPF_XB_KLU, FDPF_BX_KLU,
     FDPF_XB_NICSLU, FDPF_BX_NICSLU, FDPF_XB_CKTSO, FDPF_BX_CKTSO,
-    // MI355X batched Newton-Raphson (distributed / single slack; FP64 or FP32 factors)
-    AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, AMDGPUMixedSingleSlack
+    // MI355X batched Newton-Raphson (distributed / single slack; FP64 or FP32 factors), and the DC
+    // power flow on the same factor / solve kernels
+    AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, AMDGPUMixedSingleSlack, AMDGPUDC
 };
 
 enum class ErrorType : int {
@@ -49,6 +51,7 @@ enum class ErrorType : int {
 
 std::string solver_type_name(SolverType t);
 bool is_gpu_type(SolverType t);
+bool is_dc_type(SolverType t);
 
 // CSC matrix as the solvers take it (column pointers, sorted row indices, values)
 struct CscC {
@@ -66,6 +69,30 @@ struct CscR {
 pfgpu_ctx* engine_ctx();
 void check_rc(int rc, const char* what);
 
+// Grow-only device buffer on the engine context.
+class DevMem {
+public:
+    DevMem() = default;
+    ~DevMem() { release(); }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    void* get(size_t bytes);
+    template <class T>
+    T* as() const { return static_cast<T*>(p_); }
+    // copies v into the buffer (grown as needed) and returns its device address
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = static_cast<T*>(get(std::max<size_t>(v.size(), 1) * sizeof(T)));
+        if (!v.empty()) check_rc(pfgpu_memcpy_h2d(engine_ctx(), d, v.data(), v.size() * sizeof(T)), "h2d");
+        return d;
+    }
+    void release();
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
 // ---------------------------------------------------------------------------
 // One GPU NR solver (the reference's BaseNRAlgo / BaseNRSingleSlackAlgo
 // counterpart). It caches the loaded grid (J pattern, ordering, schedule)
@@ -74,7 +101,9 @@ void check_rc(int rc, const char* what);
 // ---------------------------------------------------------------------------
 class GpuNRSolver {
 public:
-    GpuNRSolver(bool single_slack, int precision) : single_(single_slack), precision_(precision) {}
+    // dc: the DC power flow (BaseDCAlgo) on the same kernels; max_iter and tol are ignored
+    GpuNRSolver(bool single_slack, int precision, bool dc = false)
+        : single_(single_slack), precision_(precision), dc_(dc) {}
     ~GpuNRSolver();
     GpuNRSolver(const GpuNRSolver&) = delete;
     GpuNRSolver& operator=(const GpuNRSolver&) = delete;
@@ -87,6 +116,18 @@ public:
                      const IntVect& pq, int ninst, const IntVect* dptr, const IntVect* dpos, const CplxVect* dval,
                      const CplxVect& Sbus, bool sbus_per_instance, const CplxVect& V0, bool v0_per_instance,
                      int max_iter, double tol, CplxVect& Vout, IntVect& iters, IntVect& status, pfgpu_stats* st);
+    // The same on device arrays of the engine context (pfgpu_solve_batch_device); the grid is
+    // (re)loaded from the pattern and the slack / pv / pq sets as for compute_pf.
+    void solve_batch_device(const CscC& Ybus, const IntVect& slack_ids, const RealVect& slack_weights,
+                            const IntVect& pv, const IntVect& pq, int ninst, const double* d_Yx, const int* d_dptr,
+                            const int* d_dpos, const double* d_dval, const double* d_S, bool sbus_per_instance,
+                            const double* d_V0, bool v0_per_instance, double* d_V, int max_iter, double tol,
+                            int* d_iters, int* d_status, pfgpu_stats* st);
+    // Time-series chains on device arrays (pfgpu_solve_chains_device).
+    void solve_chains_device(const CscC& Ybus, const IntVect& slack_ids, const RealVect& slack_weights,
+                             const IntVect& pv, const IntVect& pq, int nrows, int chain_len, const double* d_Yx,
+                             const double* d_S, const double* d_V0, double* d_V, int max_iter, double tol,
+                             int* d_iters, int* d_status, pfgpu_stats* st);
 
     void reset();
     const CplxVect& get_V() const { return V_; }
@@ -99,13 +140,18 @@ public:
     std::tuple<double, double, double, double> get_timers() const {
         return {timer_Fx_, timer_solve_, timer_check_, timer_total_};
     }
-    // (Fx, solve, refactor, initialize, check, dSbus, fillJ, VaVm, pre_proc, total), BaseNRAlgo.hpp:47-62.
-    // The kernel fuses refactor, solve, dSbus, J fill and the update: their time is reported under "solve".
+    // (Fx, solve, refactor, initialize, check, dSbus, fillJ, VaVm, pre_proc, total), BaseNRAlgo.hpp:47-62,
+    // from the kernel's phase clocks: Fx = mismatch phase (the convergence test is part of it, check = 0),
+    // solve = backward solve (the forward solve is fused into the factor phase), refactor = factor +
+    // forward solve + dense tail, fillJ = the J fill (it computes dS/dV in the same pass: dSbus = 0),
+    // VaVm = voltage update; initialize = grid load (analysis), pre_proc = host-to-device copies.
     std::tuple<double, double, double, double, double, double, double, double, double, double> get_timers_jacobian() const {
-        return {timer_Fx_, timer_solve_, 0.0, timer_initialize_, timer_check_, 0.0, 0.0, 0.0, timer_pre_proc_, timer_total_};
+        return {timer_Fx_, timer_solve_, timer_refactor_, timer_initialize_, timer_check_, 0.0, timer_fillJ_,
+                timer_VaVm_, timer_pre_proc_, timer_total_};
     }
     CscR get_J() const;
     bool single_slack() const { return single_; }
+    bool is_dc() const { return dc_; }
     int precision() const { return precision_; }
     const pfgpu_grid_info* grid_info() const { return grid_ ? &info_ : nullptr; }
 
@@ -114,6 +160,7 @@ private:
 
     bool single_;
     int precision_;
+    bool dc_ = false;
     pfgpu_grid* grid_ = nullptr;
     pfgpu_grid_info info_{};
     IntVect key_p_, key_i_, key_slack_, key_pv_, key_pq_;
@@ -123,6 +170,7 @@ private:
     ErrorType err_ = ErrorType::NotInitError;
     RealVect Jx_;
     double timer_Fx_ = 0, timer_solve_ = 0, timer_check_ = 0, timer_total_ = 0, timer_initialize_ = 0, timer_pre_proc_ = 0;
+    double timer_refactor_ = 0, timer_fillJ_ = 0, timer_VaVm_ = 0;
 };
 
 // ChooseSolver: holds one solver per available type and forwards to the current one.
@@ -132,7 +180,7 @@ public:
     std::vector<SolverType> available_solvers() const;
     void change_solver(SolverType t);
     SolverType get_type() const { return type_; }
-    bool ac_solver_used() const { return true; }
+    bool ac_solver_used() const { return !is_dc_type(type_); }
     GpuNRSolver& current();
     const GpuNRSolver& current() const;
     bool compute_pf(const CscC& Ybus, CplxVect& V, const CplxVect& Sbus, const IntVect& slack_ids,
@@ -143,7 +191,7 @@ public:
 
 private:
     SolverType type_;
-    std::unique_ptr<GpuNRSolver> s_[4];
+    std::unique_ptr<GpuNRSolver> s_[5];
 };
 
 // ---------------------------------------------------------------------------
@@ -182,15 +230,21 @@ public:
     SolverType get_solver_type() const { return solver_->get_type(); }
     ChooseSolver& get_solver() { return *solver_; }
     CplxVect ac_pf(const CplxVect& Vinit, int max_iter, double tol);
+    // DC power flow (GridModel::dc_pf, GridModel.cpp:676-759): DC Ybus from the lines' 1 / x
+    // stamps, solved by the GPU DC solver; empty when it fails
+    CplxVect dc_pf(const CplxVect& Vinit, int max_iter, double tol);
+    GpuNRSolver& get_dc_solver() { return *dc_solver_; }
 
     int total_bus() const { return (int)bus_vn_kv_.size(); }
     int nb_powerline() const { return (int)line_r_.size(); }
-    SolverInput pre_process_solver(const CplxVect& Vinit) const;
+    // ac false: the DC Ybus (line stamps 1 / x, no shunts; TwoSidesContainer_rxh_A.hpp:934-940)
+    SolverInput pre_process_solver(const CplxVect& Vinit, bool ac = true) const;
     const SolverInput& last_solver_input() const { return last_; }
     const CplxVect& get_V_solver() const { return V_solver_; }
 
     // branch data used by the batch classes (lines only; this subset has no trafos)
     void branch_admittance(int k, cplx& yff, cplx& yft, cplx& ytf, cplx& ytt) const;
+    void branch_admittance_dc(int k, cplx& yff, cplx& yft, cplx& ytf, cplx& ytt) const;
     int line_from(int k) const { return line_from_[k]; }
     int line_to(int k) const { return line_to_[k]; }
     bool line_status(int k) const { return line_status_[k]; }
@@ -218,6 +272,7 @@ private:
     RealVect load_p_, load_q_;
     IntVect load_bus_;
     std::shared_ptr<ChooseSolver> solver_ = std::make_shared<ChooseSolver>();
+    std::shared_ptr<GpuNRSolver> dc_solver_ = std::make_shared<GpuNRSolver>(true, PFGPU_FP64, true);
     SolverInput last_;
     CplxVect V_solver_;
 };
@@ -247,9 +302,15 @@ public:
     int nb_rows() const { return nrows_; }
     int nb_bus_total() const { return grid_model_.total_bus(); }
     int nb_branch() const { return grid_model_.nb_powerline(); }
+    // Flows of every result row on the device, from the solved voltages kept there
+    // (pfgpu_batch_flows_device); clean lists the branches zeroed per row (contingencies).
     void compute_flows_from_Vs(bool amps);
 
 protected:
+    // device results of the last compute: solved voltages (solver bus order), and per result
+    // row the solved row it comes from (-1: zero voltages)
+    void keep_device_rows(int ns, const IntVect& row_src);
+    virtual void clean_lists(IntVect& ptr, IntVect& br) const { ptr.assign(nrows_ + 1, 0); br.clear(); }
     GridModel grid_model_;
     ChooseSolver solver_;
     bool init_from_n_ = false;
@@ -258,6 +319,9 @@ protected:
     int nrows_ = 0;
     CplxVect voltages_;
     RealVect amps_, pflows_;
+    IntVect id_me_to_solver_;  // of the last compute
+    int ns_ = 0;               // solver buses of the last compute
+    DevMem d_V_, d_row_src_, d_iters_, d_status_, d_S_, d_Y_, d_V0_, d_tmp_[8];
 };
 
 class ContingencyAnalysis : public BaseBatchSolverSynch {
@@ -275,12 +339,15 @@ public:
     std::vector<IntVect> my_defaults() const;
     IntVect is_grid_connected_after_contingency();
     void compute(const CplxVect& Vinit, int max_iter, double tol);
-    void compute_flows();
-    void compute_power_flows();
+    void compute_flows() { compute_flows_from_Vs(true); }
+    void compute_power_flows() { compute_flows_from_Vs(false); }
     double modif_Ybus_time() const { return timer_modif_ybus_; }
 
 private:
-    void clean_flows(bool amps);
+    // ContingencyAnalysis::clean_flows: each row zeroes the finite flows of its own branches
+    void clean_lists(IntVect& ptr, IntVect& br) const override;
+    // per-branch Ybus positions (solver CSC: ff, tt, ft, tf; -1 out of service) and stamps
+    void branch_tables(const SolverInput& in, IntVect& pos, CplxVect& y) const;
     std::set<std::set<int>> defaults_;
     double timer_modif_ybus_ = 0;
 };
@@ -297,8 +364,8 @@ public:
     const CplxVect& get_sbuses() const { return sbuses_; }
     int nb_bus_solver() const { return nbus_solver_; }
     void clear();
-    // GPU batching: steps are solved in `chain_length`-long warm-started chains run side by side
-    // (chain_length >= T is the reference's single sequential chain).
+    // Steps run as warm-started chains of `chain_length` steps side by side; 0 (the default) or
+    // chain_length >= T is the reference's single sequential chain.
     void set_chain_length(int L) { chain_len_ = L; }
     int get_chain_length() const { return chain_len_; }
 
@@ -306,7 +373,7 @@ private:
     int status_ = 1;
     CplxVect sbuses_;
     int nbus_solver_ = 0;
-    int chain_len_ = 8;
+    int chain_len_ = 0;
 };
 
 }  // namespace ls2g

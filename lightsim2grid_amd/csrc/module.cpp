@@ -72,9 +72,9 @@ py::object cscc_to_py(const CscC& Y) {
                                   py::arg("shape") = py::make_tuple(Y.nrow, Y.ncol));
 }
 
-template <bool SINGLE, int PREC>
+template <bool SINGLE, int PREC, bool DC = false>
 struct SolverT : GpuNRSolver {
-    SolverT() : GpuNRSolver(SINGLE, PREC) {}
+    SolverT() : GpuNRSolver(SINGLE, PREC, DC) {}
 };
 
 template <class S>
@@ -118,7 +118,7 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
     m.attr("version") = py::str("0.1.0");
 
     py::enum_<SolverType> st(m, "SolverType", "This enum controls the solver you want to use.");
-    for (int k = 0; k <= (int)SolverType::AMDGPUMixedSingleSlack; ++k)
+    for (int k = 0; k <= (int)SolverType::AMDGPUDC; ++k)
         st.value(solver_type_name((SolverType)k).c_str(), (SolverType)k);
     st.export_values();
 
@@ -143,6 +143,8 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
                                             "GPU Newton-Raphson, distributed slack, FP32 factors with FP64 mismatch/update.");
     bind_solver<SolverT<true, PFGPU_FP32>>(m, "AMDGPUMixedSolverSingleSlack",
                                            "GPU Newton-Raphson, single slack, FP32 factors with FP64 mismatch/update.");
+    bind_solver<SolverT<true, PFGPU_FP64, true>>(m, "AMDGPUDCSolver",
+                                                 "GPU DC power flow on the same factor / solve kernels (drop-in for KLUDCSolver).");
 
     py::class_<ChooseSolver>(m, "AnySolver", "Dispatch over the available solver types (ChooseSolver).")
         .def(py::init<>())
@@ -196,6 +198,15 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
             {
                 py::gil_scoped_release nogil;
                 r = g.ac_pf(v, max_iter, tol);
+            }
+            return to_np(r);
+        })
+        .def("dc_pf", [](GridModel& g, const carr& Vinit, int max_iter, double tol) {
+            CplxVect v = cvec(Vinit);
+            CplxVect r;
+            {
+                py::gil_scoped_release nogil;
+                r = g.dc_pf(v, max_iter, tol);
             }
             return to_np(r);
         })

@@ -37,8 +37,17 @@
 
 #include "../../include/pfgpu.h"
 #include "analysis.hpp"
+#include "engine.hpp"
+
+std::string& pfgpu_engine::last_error() {
+    thread_local std::string msg;
+    return msg;
+}
 
 namespace {
+
+using pfgpu_engine::ApiError;
+using pfgpu_engine::guarded;
 
 constexpr int kMaxBlock = 1024;
 constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
@@ -51,44 +60,16 @@ typedef __attribute__((address_space(3))) unsigned char lds_u8;
 
 // Per-workgroup state of the NR kernel, in dynamic LDS after the value slots and the level
 // table. The kernel declares no static LDS, so its value slots start at LDS address 0.
-// Layout: part[kWaves][G], slack[G], dval[G][kMaxEdits][2] (doubles), then done, it, status,
-// piv [G], dpos[G][kMaxEdits], nd, d0 [G], ndone (ints).
+// Layout: part[kWaves][G], slack[G], fbest[G], dval[G][kMaxEdits][2] (doubles), then done,
+// it, status, piv, grow [G], dpos[G][kMaxEdits], nd, d0 [G], ndone, grp (ints).
 __host__ __device__ constexpr int wg_state_bytes(int G, int NT) {
-    return 8 * G * (NT / 64 + 1 + 2 * kMaxEdits) + 4 * (G * (6 + kMaxEdits) + 1);
+    return 8 * G * (NT / 64 + 2 + 2 * kMaxEdits) + 4 * (G * (7 + kMaxEdits) + 2);
 }
 
-thread_local std::string g_last_error;
-
-struct ApiError : std::runtime_error {
-    int code;
-    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_OK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw ApiError(PFGPU_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        return f();
-    } catch (const ApiError& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "out of host memory";
-        return PFGPU_E_NOMEM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return PFGPU_E_INVALID;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return PFGPU_E_INTERNAL;
-    }
-}
+// Per-row diagnostic flags written by the NR kernel (pfgpu_stats / the fallback passes):
+constexpr int kFlagStall = 1;   // not converged, but the mismatch fell below the stall level on the way
+constexpr int kFlagGrowth = 2;  // a pivot lost more than the growth threshold to cancellation
+constexpr int kStatusNotRun = 8;  // chain step after the chain's first failure (NotInitError)
 
 // ---------------------------------------------------------------------------
 // Kernel arguments
@@ -142,7 +123,30 @@ struct KArgs {
     void* Xg;            // spill [nblocks][nspill][G]
     int* iters;
     int* status;
-    double* Jx_last;     // optional (instance 0)
+    double* Jx_last;     // optional (row 0)
+    // launch instance i solves the caller's instance imap[i] (null: i): the index of its Ybus
+    // values / edits, its V0 and V rows, its Sbus rows and its outputs
+    const int* imap;
+    // time-series chains: instance c runs the steps c * chain + k, k < chain, one after the
+    // other, each warm-started from the previous step's V (TimeSeries.cpp:80-103); the rows
+    // of V, Sbus, iters, status and flags are steps; a chain stops at its first failure and
+    // its later steps get kStatusNotRun. chain == 1: one solve per instance.
+    int chain;
+    long long nrows;     // rows of V / iters / status (ninst * chain, the last chain may be short)
+    int fresh;           // 1: the first iteration factors a fresh solver (pivot failure -> SolverFactor)
+    double stall_tol;    // kFlagStall: the best mismatch of a failed solve went below this
+    double growth_tau;   // kFlagGrowth: |pivot| < growth_tau * |its value before elimination|
+    int* flags;          // per row (null: not written)
+    unsigned long long* phase_t;  // workgroup 0's wall clock per phase (null: not measured)
+    int* queue;          // work-queue counter (zeroed before the launch)
+    long long ngroups;   // instance groups of G: ceil(ninst / G)
+    // DC power flow (BaseDCAlgo::compute_pf, BaseDCAlgo.tpp:15-179) on the same factor and
+    // solve phases: the grid is loaded in the single-slack layout with every non-slack bus
+    // carrying only its angle, J = Re(Ybus) without the slack rows / columns, F = J theta -
+    // Re(Sbus) with theta = Va - Va(slack0); one Newton step solves the linear system, then
+    // the status is SolverSolve (7) if theta is not finite or |theta| >= 1e6
+    int dc;
+    int slack0;
 };
 
 struct c64 {
@@ -327,15 +331,18 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     lds_u8* const shb = (lds_u8*)lds_raw + a.sh_off;
     lds_dbl* const s_part = (lds_dbl*)shb;  // [kWaves][G]
     lds_dbl* const s_slack = s_part + kWaves * G;
-    lds_dbl* const s_dval = s_slack + G;  // [G][kMaxEdits][2]: this instance's Ybus edits
+    lds_dbl* const s_fbest = s_slack + G;  // best mismatch of the current solve
+    lds_dbl* const s_dval = s_fbest + G;  // [G][kMaxEdits][2]: this instance's Ybus edits
     lds_int* const s_done = (lds_int*)(s_dval + G * kMaxEdits * 2);
     lds_int* const s_it = s_done + G;
     lds_int* const s_status = s_it + G;
     lds_int* const s_piv = s_status + G;
-    lds_int* const s_dpos = s_piv + G;  // [G][kMaxEdits]
+    lds_int* const s_grow = s_piv + G;
+    lds_int* const s_dpos = s_grow + G;  // [G][kMaxEdits]
     lds_int* const s_nd = s_dpos + G * kMaxEdits;
     lds_int* const s_d0 = s_nd + G;
     lds_int* const s_ndone = s_d0 + G;
+    lds_int* const s_grp = s_ndone + 1;
     // {items per lane, used lanes} of every factor level, then of every backward level
     // (in LDS after the value slots)
     lds_int* lvK = (lds_int*)(lds_raw + a.lvk_off);
@@ -347,10 +354,30 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     const int tid = threadIdx.x;
     const int g = tid % G, s = tid / G;
     constexpr int nslot = NT / G;
-    const long long inst = (long long)blockIdx.x * G + g;
-    const bool valid = inst < a.ninst;
-    const long long ii = valid ? inst : 0;
     const int n = a.n;
+    // phase clocks of workgroup 0 (J fill, factor + forward, dense tail, backward, update, mismatch)
+    const bool timing = a.phase_t != nullptr && blockIdx.x == 0 && tid == 0;
+    unsigned long long t_ph = timing ? wall_clock64() : 0ull;
+    auto tmark = [&](int ph) {
+        if (timing) {
+            const unsigned long long t = wall_clock64();
+            a.phase_t[ph] += t - t_ph;
+            t_ph = t;
+        }
+    };
+    // Persistent workgroups: a launch has as many workgroups as can be resident; each takes
+    // groups of G instances from a work queue (a counter zeroed by the launch's first kernel)
+    // until none is left, so the per-workgroup workspace does not grow with the batch.
+    for (;;) {
+    if (tid == 0) *s_grp = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const long long grp = *s_grp;
+    __syncthreads();
+    if (grp >= a.ngroups) break;
+    const long long inst = grp * G + g;
+    const bool valid = inst < a.ninst;
+    const long long ii = (long long)blockIdx.x * G + g;  // workspace row of this lane's instance
+    const long long oi = valid ? (a.imap ? (long long)a.imap[inst] : inst) : 0;  // the caller's instance
     FT* Xgb = reinterpret_cast<FT*>(a.Xg) + (size_t)blockIdx.x * (size_t)a.nspill * G;
     const int nlds = a.nlds;
     auto XL = [&](int slot) -> FT {
@@ -376,9 +403,12 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (slot < nlds) Xl[slot * G + g] = v;
         else Xgb[(size_t)(slot - nlds) * G + g] = v;
     };
-    const double* Y = a.Yx + ii * a.y_stride;
-    const double* S = a.S + ii * a.s_stride;
-    double* V = a.V + ii * 2 * (long long)n;
+    const double* Y = a.Yx + oi * a.y_stride;
+    // the current step's row (chain mode) or the instance's row
+    long long row = oi * a.chain;
+    bool run = valid;  // this instance solves the current step (uniform over its lanes)
+    const double* S = a.S + row * a.s_stride;
+    double* V = a.V + row * 2 * (long long)n;
     double* Va = a.Va + ii * (long long)n;
     double* Vm = a.Vm + ii * (long long)n;
     double* Ib = a.Ib + ii * 2 * (long long)a.nyd;
@@ -399,8 +429,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         s_nd[g] = 0;
         s_d0[g] = 0;
         if (a.dptr != nullptr && valid) {
-            s_d0[g] = a.dptr[ii];
-            s_nd[g] = a.dptr[ii + 1] - a.dptr[ii];
+            s_d0[g] = a.dptr[oi];
+            s_nd[g] = a.dptr[oi + 1] - a.dptr[oi];
         }
     }
     __syncthreads();
@@ -443,9 +473,20 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     const double* const Yell = a.Yell;
     const double* const Yye = a.Yye;
 
+    bool alive = valid;  // chain mode: the chain has not failed yet
+    for (int step = 0; step < a.chain; ++step) {
+    row = oi * a.chain + step;
+    run = alive && row < a.nrows;
+    S = a.S + (run ? row : 0) * a.s_stride;
+    V = a.V + (run ? row : 0) * 2 * (long long)n;
+    if (!LEAN && run && step > 0)  // warm start from the previous step's V (own buses: no barrier)
+        for (int b = s; b < n; b += nslot) {
+            V[2 * b] = V[2 * b - 2 * (long long)n];
+            V[2 * b + 1] = V[2 * b + 1 - 2 * (long long)n];
+        }
     // ---- init: Va/Vm from V, slack_absorbed = Re(sum Sbus) (BaseNRAlgo.tpp:60,77-79)
     double ssum = 0.0;
-    if (valid) {
+    if (run) {
         if constexpr (LEAN) polar_of(Vl, G, Va, Vm, s, n, nslot);
         else polar_of(V, 1, Va, Vm, s, n, nslot);
         for (int b = s; b < n; b += nslot) ssum += S[2 * b];
@@ -464,7 +505,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             s_it[g] = 0;
             s_status[g] = 0;
             s_piv[g] = 0;
-            s_done[g] = valid ? 0 : 1;
+            s_grow[g] = 0;
+            s_fbest[g] = INFINITY;
+            s_done[g] = run ? 0 : 1;
         }
         __syncthreads();
     }
@@ -475,7 +518,30 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     // load before them, coalesced across lanes, and summed in row order.
     auto mismatch = [&]() -> double {
         double fm = 0.0;
-        if (valid && !s_done[g]) {
+        if (run && !s_done[g] && a.dc) {
+            // DC: F = Re(Y) theta - Re(S) over the non-slack columns; fm = max |theta| of the rows
+            EditPos dp;
+            edit_pos(dp);
+            const double vas = Va[a.slack0];
+            for (int r = s; r < n; r += nslot) {
+                const int len = a.yr_ptr[r + 1] - a.yr_ptr[r];
+                const int fp = a.fslot_p[r];
+                double acc = 0.0;
+                for (int k0 = 0; k0 == 0 || k0 < len; k0 += kEll) {
+#pragma unroll
+                    for (int k = 0; k < kEll; ++k) {
+                        const int2 pc = a.yr_ell[(size_t)(k0 + k) * n + r];
+                        if (pc.x < 0 || a.bus_dx[pc.y].x < 0) continue;
+                        const c64 yv = Yell ? edits(ld_c(Yell, (k0 + k) * n + r), pc.x, dp) : Yat(pc.x, dp);
+                        acc += yv.re * (Va[pc.y] - vas);
+                    }
+                }
+                if (fp >= 0) {
+                    XS(fp, (FT)(acc - S[2 * r]));
+                    fm = fmax(fm, inf_if_nonfinite(fabs(Va[r] - vas)));
+                }
+            }
+        } else if (run && !s_done[g]) {
             const double sa = s_slack[g];
             EditPos dp;
             edit_pos(dp);
@@ -514,11 +580,13 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     };
 
     double fmx = mismatch();
-    if (s == 0 && valid && !s_done[g]) {
+    if (s == 0 && run && !s_done[g] && !a.dc) {  // DC: always one linear solve
+        s_fbest[g] = fmx;
         if (fmx < a.tol) s_done[g] = 1;  // converged at iteration 0
         else if (a.max_iter <= 0) { s_status[g] = 2; s_done[g] = 1; }
     }
     __syncthreads();
+    tmark(5);
 
     for (;;) {
         if (tid == 0) {
@@ -528,7 +596,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         }
         __syncthreads();
         if (*s_ndone == G) break;
-        bool act = valid && !s_done[g];
+        bool act = run && !s_done[g];
         if (s == 0 && act) s_it[g] += 1;
 
         // ---- J fill: zero the tile fill-in slots, write every J entry ------------------
@@ -537,7 +605,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (act) {
             auto put = [&](int slot, int jidx, double jv) {
                 if (slot < 0) return;
-                if (a.Jx_last != nullptr && inst == 0) a.Jx_last[jidx] = jv;
+                if (a.Jx_last != nullptr && row == 0) a.Jx_last[jidx] = jv;
                 FT v = (FT)jv;
                 if (slot & (1 << 30)) {  // pivot with nothing to eliminate
                     if (bad_pivot(v)) s_piv[g] = 1;
@@ -580,6 +648,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 for (int u = 0; u < kJE; ++u) {
                     const int e = e0 + u * nslot;
                     if (e >= a.nye) break;
+                    if (a.dc) {  // DC: the J entry of a P row and an angle column is Re(Y)
+                        put(sl[u].x, (a.Jx_last != nullptr && row == 0) ? a.ye_jidx[e].x : 0, y[u].re);
+                        continue;
+                    }
                     const bool diag = e < a.nyd;
                     c64 t = cmul(y[u], vc[u]);
                     c64 dvm = cmul(cconj(cmul(y[u], vnc[u])), vr[u]);
@@ -589,7 +661,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     }
                     const c64 dva = cmul(cconj({-t.re, -t.im}), {-vr[u].im, vr[u].re});  // * (j V_r)
                     int4 jx = {0, 0, 0, 0};
-                    if (a.Jx_last != nullptr && inst == 0) jx = a.ye_jidx[e];
+                    if (a.Jx_last != nullptr && row == 0) jx = a.ye_jidx[e];
                     put(sl[u].x, jx.x, dva.re);
                     put(sl[u].y, jx.y, dva.im);
                     put(sl[u].z, jx.z, dvm.re);
@@ -598,6 +670,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         }
         __syncthreads();
+        tmark(0);
 
         // ---- factor + forward solve --------------------------------------------
         // General walk (launches that spill): the kSC-item groups of all levels form one
@@ -606,7 +679,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // operands of a group are read before its prefetch is issued, so waiting for them
         // leaves the prefetches in flight. The lean walk is described at its own code.
         {
-            bool bad = false;  // a zero or non-finite pivot met by this lane
+            bool bad = false;   // a zero or non-finite pivot met by this lane
+            bool grow = false;  // a pivot that lost more than growth_tau of its value to cancellation
+            const FT gtau = (FT)a.growth_tau;
             if constexpr (LEAN) {
             // Level table word 0: K | Lc << 16. A lane's K items of a level form K / Lc
             // chunks, and every task ends at the last item of a chunk, at item indices that are
@@ -634,9 +709,11 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 const FT sum = team_sum<G>(acc, lt, T);
                 if ((e & kEnd) && (s & ((1 << lt) - 1)) == 0) {
                     const int off = e & ((1 << 25) - 1);
-                    FT v = (((e >> 25) & 1) ? (FT)0 : XB(off)) - sum;  // fill-in: assign
+                    const FT a0 = ((e >> 25) & 1) ? (FT)0 : XB(off);  // fill-in: assign
+                    FT v = a0 - sum;
                     if ((e >> 29) & 1) {
                         bad |= bad_pivot(v);
+                        grow |= fabs(v) < gtau * fabs(a0);
                         v = pivot_recip(v);
                     }
                     XBset(off, v);
@@ -780,12 +857,14 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                                 // pointer for this slot trips the gfx950 backend)
                                 const bool as = (e >> 25) & 1;  // fill-in: the slot's old content is not ours
                                 if (slot < nlds) {
-                                    FT v = (as ? (FT)0 : Xl[slot * G + g]) - sum;
-                                    if (rc) { bad |= bad_pivot(v); v = pivot_recip(v); }
+                                    const FT a0 = as ? (FT)0 : Xl[slot * G + g];
+                                    FT v = a0 - sum;
+                                    if (rc) { bad |= bad_pivot(v); grow |= fabs(v) < gtau * fabs(a0); v = pivot_recip(v); }
                                     Xl[slot * G + g] = v;
                                 } else {
-                                    FT v = (as ? (FT)0 : Xgb[(size_t)(slot - nlds) * G + g]) - sum;
-                                    if (rc) { bad |= bad_pivot(v); v = pivot_recip(v); }
+                                    const FT a0 = as ? (FT)0 : Xgb[(size_t)(slot - nlds) * G + g];
+                                    FT v = a0 - sum;
+                                    if (rc) { bad |= bad_pivot(v); grow |= fabs(v) < gtau * fabs(a0); v = pivot_recip(v); }
                                     Xgb[(size_t)(slot - nlds) * G + g] = v;
                                 }
                             }
@@ -817,7 +896,9 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
             }
             if (bad) s_piv[g] = 1;
+            if (grow) s_grow[g] = 1;
         }
+        tmark(1);
         // ---- dense tail: the m x m tile of every instance ---------------------------------
         // Tile at slots [0, m*(m+1)): row-major with row stride m + 1, the right-hand side
         // held as column m. Right-looking LU with diagonal pivots, two columns per pass (one
@@ -895,12 +976,15 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
             __syncthreads();
         }
-        if (s == 0 && act && s_piv[g]) {  // zero / non-finite pivot: klu_refactor failure
-            s_status[g] = 6;
+        tmark(2);
+        // zero / non-finite pivot: klu_factor failure on the first factorisation of a fresh
+        // solver (KLUSolver.cpp:38-44), klu_refactor failure afterwards (:50-57)
+        if (s == 0 && act && s_piv[g]) {
+            s_status[g] = (a.fresh && s_it[g] == 1) ? 5 : 6;
             s_done[g] = 1;
         }
         __syncthreads();
-        act = valid && !s_done[g];
+        act = run && !s_done[g];
 
         // ---- backward solve (same pipelined walk as the factor phase) -----------------
         {
@@ -1093,6 +1177,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         }
 
+        tmark(3);
         // ---- voltage update ------------------------------------------------------------
         // per bus, by the lane that owns it: its Va and Vm steps are read from their slots
         double vmin = INFINITY;
@@ -1114,26 +1199,52 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             else polar_of(V, 1, Va, Vm, s, n, nslot);
         }
         __syncthreads();
+        tmark(4);
 
         // ---- mismatch + convergence ----------------------------------------------------
         fmx = mismatch();
-        if (s == 0 && act) {
+        if (s == 0 && act && a.dc) {
+            s_status[g] = fmx < 1e6 ? 0 : 7;  // max |theta| (inf if not finite)
+            s_done[g] = 1;
+        } else if (s == 0 && act) {
+            s_fbest[g] = fmin(s_fbest[g], fmx);
             if (!(fmx <= 1.7976931348623157e308)) { s_status[g] = 3; s_done[g] = 1; }
             else if (fmx < a.tol) { s_status[g] = 0; s_done[g] = 1; }
             else if (s_it[g] >= a.max_iter) { s_status[g] = 2; s_done[g] = 1; }
         }
         __syncthreads();
+        tmark(5);
     }
-    if constexpr (LEAN)
-        if (valid)
+    // ---- end of the step: results of this row; a failed chain stops ---------------------
+    if (run) {
+        if constexpr (LEAN)
             for (int b = s; b < n; b += nslot) {
                 V[2 * b] = Vl[2 * b * G];
                 V[2 * b + 1] = Vl[2 * b * G + 1];
             }
-    if (s == 0 && valid) {
-        a.iters[inst] = s_it[g];
-        a.status[inst] = s_status[g];
+        if (s == 0) {
+            const int stt = s_status[g];
+            a.iters[row] = s_it[g];
+            a.status[row] = stt;
+            if (a.flags)
+                a.flags[row] = (stt != 0 && s_fbest[g] < a.stall_tol ? kFlagStall : 0) | (s_grow[g] ? kFlagGrowth : 0);
+        }
+        alive = s_status[g] == 0;
+    } else if (valid && row < a.nrows) {  // a step after its chain's failure
+        for (int b = s; b < n; b += nslot) {
+            double* Vr = a.V + row * 2 * (long long)n;
+            Vr[2 * b] = 0.0;
+            Vr[2 * b + 1] = 0.0;
+        }
+        if (s == 0) {
+            a.iters[row] = 0;
+            a.status[row] = kStatusNotRun;
+            if (a.flags) a.flags[row] = 0;
+        }
     }
+    __syncthreads();  // the next step rewrites the workgroup state
+    }  // step
+    }  // instance groups
 }
 
 // Per-instance initial V (broadcast or per instance).
@@ -1152,15 +1263,67 @@ __global__ void k_gather_y(const double* __restrict__ Y, const int2* __restrict_
     }
 }
 
-__global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n, int ninst) {
+// The first kernel of a solve launch: zeroes the NR kernel's work-queue counter and, unless
+// the voltages are solved in place, writes the initial V of every launch instance (its first
+// row: the chain head in chain mode) from V0 (one row per instance, or one shared row).
+__global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n, int ninst, const int* imap,
+                         int chain, int* queue, unsigned long long* phase, int copy) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 8 && phase) phase[threadIdx.x] = 0;
+    if (!copy) return;
     const long long total = (long long)ninst * n;
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
          k += (long long)gridDim.x * blockDim.x) {
         const long long i = k / n;
         const int b = (int)(k - i * n);
-        V[2 * k] = V0[i * v0_stride + 2 * b];
-        V[2 * k + 1] = V0[i * v0_stride + 2 * b + 1];
+        const long long oi = imap ? (long long)imap[i] : i;
+        const long long r = oi * chain;
+        V[2 * (r * n + b)] = V0[oi * v0_stride + 2 * b];
+        V[2 * (r * n + b) + 1] = V0[oi * v0_stride + 2 * b + 1];
     }
+}
+
+// Compaction of the rows to re-solve in FP64 after an FP32-factor launch, in row order, by one
+// workgroup of 1024 lanes: rows whose status is not 0 (and not a chain's not-run step) and that
+// stalled, lost a pivot to cancellation, met a zero / non-finite pivot or a non-finite mismatch.
+// Chain mode takes whole chains (a chain is re-run from its head). Writes the list and its
+// length.
+__global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, const int* flags, long long nrows,
+                                                            int chain, int* list, int* count) {
+    __shared__ int wsum[16];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    const long long nunits = (nrows + chain - 1) / chain;
+    for (long long u0 = 0; u0 < nunits; u0 += 1024) {
+        const long long u = u0 + tid;
+        bool take = false;
+        if (u < nunits)
+            for (int k = 0; k < chain; ++k) {
+                const long long r = u * chain + k;
+                if (r >= nrows) break;
+                const int st = status[r];
+                if (st == 0 || st == kStatusNotRun) continue;
+                take = st == 3 || st == 5 || st == 6 || (flags[r] & (kFlagStall | kFlagGrowth)) != 0;
+                break;  // a chain's first failure decides
+            }
+        const unsigned long long m = __ballot(take);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (take) list[off + before] = (int)u;
+        __syncthreads();
+        if (tid == 0) {
+            int t = 0;
+            for (int w = 0; w < 16; ++w) t += wsum[w];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *count = base;
 }
 
 // Branch flows at the origin side, one (instance, branch) per thread.
@@ -1211,18 +1374,6 @@ struct DevBuf {
 
 }  // namespace
 
-struct pfgpu_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    size_t budget = 0;
-    int n_cu = 0;
-    int lds_max = 0;
-    // grow-only workspace
-    void* ws = nullptr;
-    size_t ws_bytes = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::mutex mu;
-};
 
 struct pfgpu_grid {
     pfgpu_ctx* ctx = nullptr;
@@ -1246,6 +1397,10 @@ struct pfgpu_grid {
     std::vector<std::unique_ptr<Streams>> streams;  // by (G, block size, value size, lean)
     // autotuned launch shape per precision (G, block), 0 = not tuned
     int tuned_G[2] = {0, 0}, tuned_NT[2] = {0, 0};
+    // the next solve's first iteration is the solver's first factorisation (KLULinearSolver::
+    // initialize, KLUSolver.cpp:24-45); cleared by a solve, set by load and pfgpu_grid_reset
+    bool fresh = true;
+    bool dc = false;  // a DC grid (pfgpu_grid_load_dc): one linear solve per instance
 };
 
 namespace {
@@ -1422,67 +1577,117 @@ void ensure_streams(pfgpu_grid* gr, int G, int NT, int fsize, bool lean) {
     gr->streams[ix] = std::move(t);
 }
 
+void* grow_buffer(void*& p, size_t& cap, size_t bytes) {
+    if (bytes > cap) {
+        if (p) HIP_OK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        HIP_OK(hipMalloc(&p, bytes));
+        cap = bytes;
+    }
+    return p;
+}
+
 void* ws_get(pfgpu_ctx* ctx, size_t bytes) {
     if (bytes > ctx->budget) throw ApiError(PFGPU_E_BUDGET, "batch workspace exceeds the context's HBM budget");
-    if (bytes > ctx->ws_bytes) {
-        if (ctx->ws) HIP_OK(hipFree(ctx->ws));
-        ctx->ws = nullptr;
-        ctx->ws_bytes = 0;
-        HIP_OK(hipMalloc(&ctx->ws, bytes));
-        ctx->ws_bytes = bytes;
-    }
-    return ctx->ws;
+    return grow_buffer(ctx->ws, ctx->ws_bytes, bytes);
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Core solve on device buffers. Workspace layout: [Va][Vm][Ib][Xg][Ybus gathered].
-void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx, int Yx_stride, const int* d_dptr,
-                  const int* d_dpos, const double* d_dval, const double* d_S, int sbus_stride, const double* d_V0,
-                  int v0_stride, double* d_V, int max_iter, double tol, int precision, int G_req, int* d_iters, int* d_status,
-                  double* d_Jx_last, pfgpu_stats* st, int NT_req = 0) {
+// One solve call as the NR launches see it (device pointers).
+struct SolveArgs {
+    int ninst = 0;          // launch units: instances, or chains in chain mode
+    int chain = 1;          // steps per chain (1: one solve per instance)
+    long long nrows = 0;    // rows of V / Sbus / iters / status (ninst * chain, the last chain may be short)
+    const double* Yx = nullptr;
+    int Yx_stride = 0;
+    const int *dptr = nullptr, *dpos = nullptr;
+    const double* dval = nullptr;
+    const double* S = nullptr;
+    int sbus_stride = 0;
+    const double* V0 = nullptr;
+    int v0_stride = 0;
+    double* V = nullptr;
+    int max_iter = 10;
+    double tol = 1e-8;
+    int precision = PFGPU_FP64;  // PFGPU_FP64, PFGPU_FP32 or PFGPU_FP32_STRICT
+    int G_req = 0, NT_req = 0;
+    int* iters = nullptr;
+    int* status = nullptr;
+    double* Jx_last = nullptr;
+};
+
+// Resident workgroups of a launch shape (occupancy of the kernel variant at its LDS size).
+int resident_blocks(const pfgpu_ctx* ctx, int fsize, int G, int NT, bool lean, size_t lds) {
+    const void* fn;
+    if (fsize == 4)
+        fn = NT == 1024 ? (lean ? nr_kernel_g<float, 1024, true>(G) : nr_kernel_g<float, 1024, false>(G))
+                        : (lean ? nr_kernel_g<float, 256, true>(G) : nr_kernel_g<float, 256, false>(G));
+    else
+        fn = NT == 1024 ? (lean ? nr_kernel_g<double, 1024, true>(G) : nr_kernel_g<double, 1024, false>(G))
+                        : (lean ? nr_kernel_g<double, 256, true>(G) : nr_kernel_g<double, 256, false>(G));
+    if (lds > 65536) HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds));
+    if (per_cu < 1) throw ApiError(PFGPU_E_INTERNAL, "the NR launch shape cannot be resident on a CU");
+    return per_cu * std::max(1, ctx->n_cu);
+}
+
+// Per-call rows in ctx->aux: [queue counter | fallback count | pad] [phase clocks: 8 x u64]
+// [flags: nrows ints] [fallback list: ninst ints]
+struct AuxLayout {
+    size_t o_queue = 0, o_count = 4, o_phase = 64, o_flags = 128, o_list = 0, bytes = 0;
+};
+AuxLayout aux_layout(long long nrows, long long ninst) {
+    AuxLayout L;
+    L.o_list = align_up(L.o_flags + (size_t)std::max(nrows, 1LL) * 4);
+    L.bytes = align_up(L.o_list + (size_t)std::max(ninst, 1LL) * 4);
+    return L;
+}
+
+// One NR launch (k_init_v, k_gather_y for a shared Ybus, k_nr) over `nunits` launch units;
+// imap (device, or null) maps them to the caller's instances / chains.
+void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precision, int G_req, int NT_req,
+                 int nunits, const int* imap, const AuxLayout& AL, bool fresh, pfgpu_stats* st) {
     const pfgpu::DevicePlan& P = gr->P;
     const int n = P.n;
-    if (ninst <= 0) return;
-    if (precision != PFGPU_FP64 && precision != PFGPU_FP32) throw ApiError(PFGPU_E_INVALID, "bad precision");
-    if (Yx_stride != 0 && Yx_stride != P.nnzY) throw ApiError(PFGPU_E_INVALID, "Yx_stride must be 0 or nnz_ybus");
-    if (sbus_stride != 0 && sbus_stride != n) throw ApiError(PFGPU_E_INVALID, "sbus_stride must be 0 or n");
-    if (v0_stride != 0 && v0_stride != n) throw ApiError(PFGPU_E_INVALID, "v0_stride must be 0 or n");
-    if (max_iter < 0) throw ApiError(PFGPU_E_INVALID, "max_iter must be >= 0");
-    const int fsize = precision == PFGPU_FP32 ? 4 : 8;
-    if (G_req == 0 && NT_req == 0 && gr->tuned_G[precision]) {
-        G_req = gr->tuned_G[precision];
-        NT_req = gr->tuned_NT[precision];
+    const int fsize = precision == PFGPU_FP64 ? 8 : 4;
+    const int pslot = precision == PFGPU_FP64 ? 0 : 1;
+    if (G_req == 0 && NT_req == 0 && gr->tuned_G[pslot]) {
+        G_req = gr->tuned_G[pslot];
+        NT_req = gr->tuned_NT[pslot];
     }
-    LaunchChoice lc = choose_launch(ctx, P, ninst, fsize, G_req, NT_req);
+    LaunchChoice lc = choose_launch(ctx, P, nunits, fsize, G_req, NT_req);
     ensure_streams(gr, lc.G, lc.NT, fsize, lc.lean);
-    const bool deltas = d_dptr != nullptr;
-    if (deltas && Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
+    const long long ngroups = ((long long)nunits + lc.G - 1) / lc.G;
+    const int nres = (int)std::min<long long>(ngroups, resident_blocks(ctx, fsize, lc.G, lc.NT, lc.lean, lc.lds));
+    const long long slots = (long long)nres * lc.G;  // workspace rows: one per resident instance
     size_t off = 0;
-    const size_t o_va = off; off += align_up((size_t)ninst * n * 8);
-    const size_t o_vm = off; off += align_up((size_t)ninst * n * 8);
-    const size_t o_ib = off; off += align_up((size_t)ninst * std::max(P.nyd, 1) * 16);
-    const size_t o_x = off; off += align_up((size_t)lc.nblocks * lc.nspill * lc.G * fsize);
-    // shared Ybus: its values gathered in the order of the mismatch and J-fill tables
-    const bool gather = Yx_stride == 0;
+    const size_t o_va = off; off += align_up((size_t)slots * n * 8);
+    const size_t o_vm = off; off += align_up((size_t)slots * n * 8);
+    const size_t o_ib = off; off += align_up((size_t)slots * std::max(P.nyd, 1) * 16);
+    const size_t o_x = off; off += align_up((size_t)nres * lc.nspill * lc.G * fsize);
+    const bool gather = q.Yx_stride == 0;
     const size_t o_yell = off; off += gather ? align_up((size_t)gr->ell_w * n * 16) : 0;
     const size_t o_yye = off; off += gather ? align_up(P.ye_pos.size() * 16) : 0;
     char* ws = static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
+    char* aux = static_cast<char*>(ctx->aux);
     hipStream_t stream = ctx->stream;
-    // the kernel applies the per-instance Ybus edits (deltas) to the shared base itself
-    const double* Ypt = d_Yx;
-    const long long ystride = (long long)Yx_stride * 2;
-    if (d_V0 != d_V || v0_stride == 0) {
-        // d_V0 == d_V with a shared row is allowed: row 0 is only read by the threads writing row 0 itself
-        hipLaunchKernelGGL(k_init_v, dim3(1024), dim3(256), 0, stream, d_V0, (long long)v0_stride * 2, d_V, n, ninst);
-        HIP_OK(hipGetLastError());
-    }
+    int* queue = reinterpret_cast<int*>(aux + AL.o_queue);
+    unsigned long long* phase = st ? reinterpret_cast<unsigned long long*>(aux + AL.o_phase) : nullptr;
+    // in place (V0 == V, one row per instance): nothing to copy, rows are read where they are
+    const bool copy = !(q.V0 == q.V && q.v0_stride != 0 && q.chain == 1);
+    HIP_OK(hipEventRecord(ctx->ev0, stream));
+    hipLaunchKernelGGL(k_init_v, dim3(copy ? 1024 : 1), dim3(256), 0, stream, q.V0, (long long)q.v0_stride * 2, q.V, n,
+                       nunits, imap, q.chain, queue, phase, copy ? 1 : 0);
+    HIP_OK(hipGetLastError());
     KArgs a{};
     a.n = n; a.N = P.N; a.nJ = P.nJ; a.nzero = (int)P.zero_slots.size(); a.nlev = P.nlev; a.nblev = P.nblev;
     a.m = P.m;
     if (lc.nlds < P.m * P.m + P.m) throw ApiError(PFGPU_E_INTERNAL, "dense tail does not fit in LDS");
     a.nlds = lc.nlds; a.nspill = lc.nspill;
-    a.single_slack = gr->info.single_slack; a.ninst = ninst; a.max_iter = max_iter; a.tol = tol;
+    a.single_slack = gr->info.single_slack; a.ninst = nunits; a.max_iter = q.max_iter; a.tol = q.tol;
     a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
     a.yr_pc = reinterpret_cast<const int2*>(gr->yr_pc.p);
     a.yr_ell = reinterpret_cast<const int2*>(gr->yr_ell.p);
@@ -1490,7 +1695,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.ye_slot = reinterpret_cast<const int4*>(gr->ye_slot.p); a.ye_jidx = reinterpret_cast<const int4*>(gr->ye_jidx.p);
     a.nye = (int)P.ye_pos.size();
     a.jc_slot = gr->jc_slot.p; a.jc_jidx = gr->jc_jidx.p; a.jc_val = gr->jc_val.p; a.njc = (int)P.jc_slot.size();
-    a.dptr = deltas ? d_dptr : nullptr; a.dpos = d_dpos; a.dval = d_dval;
+    a.dptr = q.dptr; a.dpos = q.dpos; a.dval = q.dval;
     a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
     a.ux_bus = gr->ux_bus.p; a.ux_slot = gr->ux_slot.p; a.ux_kind = gr->ux_kind.p;
     a.bus_dx = reinterpret_cast<const int2*>(gr->bus_dx.p); a.slack_dx = gr->slack_dx;
@@ -1500,24 +1705,36 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
     a.lv_K = ls.lv_K.p; a.lvk_off = (int)lc.lvk_off; a.sh_off = (int)lc.sh_off;
     a.fs = reinterpret_cast<const int4*>(ls.fs.p); a.bs = reinterpret_cast<const int4*>(ls.bs.p);
     a.w = gr->w.p;
-    a.Yx = Ypt; a.y_stride = ystride;
-    a.S = d_S; a.s_stride = (long long)sbus_stride * 2;
-    a.V = d_V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
+    a.Yx = q.Yx; a.y_stride = (long long)q.Yx_stride * 2;
+    a.S = q.S; a.s_stride = (long long)q.sbus_stride * 2;
+    a.V = q.V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
     a.ye_diag_of = gr->ye_diag_of.p; a.nyd = P.nyd;
-    a.iters = d_iters; a.status = d_status; a.Jx_last = d_Jx_last;
+    a.iters = q.iters; a.status = q.status; a.Jx_last = q.Jx_last;
     a.v_off = (int)lc.v_off;
     a.Yell = gather ? reinterpret_cast<const double*>(ws + o_yell) : nullptr;
     a.Yye = gather ? reinterpret_cast<const double*>(ws + o_yye) : nullptr;
-    HIP_OK(hipEventRecord(ctx->ev0, stream));
+    a.imap = imap;
+    a.chain = q.chain;
+    a.nrows = q.nrows;
+    a.fresh = fresh ? 1 : 0;
+    a.stall_tol = std::sqrt(q.tol);
+    a.growth_tau = fsize == 4 ? 1.0 / 1024.0 : 1e-10;
+    a.flags = reinterpret_cast<int*>(aux + AL.o_flags);
+    a.phase_t = phase;
+    a.queue = queue;
+    a.ngroups = ngroups;
+    a.dc = gr->dc ? 1 : 0;
+    a.slack0 = P.slack0;
+    if (gr->dc) a.max_iter = 1;
     if (gather) {
         const int nell = gr->ell_w * n, nye = (int)P.ye_pos.size();
-        hipLaunchKernelGGL(k_gather_y, dim3((nell + nye + 255) / 256), dim3(256), 0, stream, d_Yx, a.yr_ell, nell,
+        hipLaunchKernelGGL(k_gather_y, dim3((nell + nye + 255) / 256), dim3(256), 0, stream, q.Yx, a.yr_ell, nell,
                            a.ye_pos, nye, reinterpret_cast<double*>(ws + o_yell), reinterpret_cast<double*>(ws + o_yye));
         HIP_OK(hipGetLastError());
     }
-    if (precision == PFGPU_FP32) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
-    else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, lc.nblocks, lc.lds, stream);
+    if (fsize == 4) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
+    else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
     HIP_OK(hipEventRecord(ctx->ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     if (st) {
@@ -1525,16 +1742,83 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, int ninst, const double* d_Yx,
         HIP_OK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         st->kernel_s += ms * 1e-3;
         st->instances_per_wg = lc.G;
+        st->block_size = lc.NT;
         st->lds_slots = lc.nlds;
-        st->workgroups = lc.nblocks;
+        st->workgroups = (int)std::min<long long>(ngroups, INT_MAX);
+        st->resident_workgroups = nres;
+        unsigned long long ph[8];
+        HIP_OK(hipMemcpy(ph, phase, sizeof(ph), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 6; ++k) st->phase_s[k] += (double)ph[k] / ctx->wall_hz;
+    }
+}
+
+// Core solve on device buffers: the launch in the requested form; with FP32 factors (unless
+// strict), the rows that stalled or broke down are re-solved from their initial voltages with
+// FP64 factors (chains from their heads).
+void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_stats* st) {
+    const pfgpu::DevicePlan& P = gr->P;
+    const int n = P.n;
+    if (q0.ninst <= 0 || q0.nrows <= 0) return;
+    SolveArgs q = q0;
+    if (q.precision != PFGPU_FP64 && q.precision != PFGPU_FP32 && q.precision != PFGPU_FP32_STRICT)
+        throw ApiError(PFGPU_E_INVALID, "bad precision");
+    if (q.Yx_stride != 0 && q.Yx_stride != P.nnzY) throw ApiError(PFGPU_E_INVALID, "Yx_stride must be 0 or nnz_ybus");
+    if (q.sbus_stride != 0 && q.sbus_stride != n) throw ApiError(PFGPU_E_INVALID, "sbus_stride must be 0 or n");
+    if (q.v0_stride != 0 && q.v0_stride != n) throw ApiError(PFGPU_E_INVALID, "v0_stride must be 0 or n");
+    if (q.max_iter < 0) throw ApiError(PFGPU_E_INVALID, "max_iter must be >= 0");
+    if (q.chain < 1) throw ApiError(PFGPU_E_INVALID, "chain length must be >= 1");
+    if (q.chain > 1 && q.V0 == q.V) throw ApiError(PFGPU_E_INVALID, "chains need V0 apart from V");
+    if (q.dptr && q.Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
+    if (gr->dc && q.precision != PFGPU_FP64) throw ApiError(PFGPU_E_INVALID, "a DC grid is solved with FP64 factors");
+    const AuxLayout AL = aux_layout(q.nrows, q.ninst);
+    char* aux = static_cast<char*>(grow_buffer(ctx->aux, ctx->aux_bytes, AL.bytes));
+    const bool fp32 = q.precision != PFGPU_FP64;
+    const bool fallback = q.precision == PFGPU_FP32;
+    // in place with an FP64 re-solve possible: keep the initial voltages
+    const double* V0_keep = q.V0;
+    if (fallback && q.V0 == q.V && q.v0_stride != 0) {
+        const size_t bytes = (size_t)q.ninst * n * 16;
+        double* keep = static_cast<double*>(grow_buffer(ctx->io, ctx->io_bytes, bytes));
+        HIP_OK(hipMemcpyAsync(keep, q.V0, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        V0_keep = keep;
+    }
+    const bool fresh = gr->fresh;
+    launch_pass(ctx, gr, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st);
+    gr->fresh = false;
+    ctx->last_rows = q.nrows;
+    if (!fallback) return;
+    int* list = reinterpret_cast<int*>(aux + AL.o_list);
+    int* count = reinterpret_cast<int*>(aux + AL.o_count);
+    hipLaunchKernelGGL(k_collect_fallback, dim3(1), dim3(1024), 0, ctx->stream, q.status,
+                       reinterpret_cast<const int*>(aux + AL.o_flags), q.nrows, q.chain, list, count);
+    HIP_OK(hipGetLastError());
+    int nfb = 0;
+    HIP_OK(hipMemcpyAsync(&nfb, count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (nfb <= 0) return;
+    SolveArgs r = q;
+    r.V0 = V0_keep;
+    r.Jx_last = nullptr;
+    pfgpu_stats s2{};
+    launch_pass(ctx, gr, r, PFGPU_FP64, 0, 0, nfb, list, AL, fresh, st ? &s2 : nullptr);
+    if (st) {
+        st->fallback_fp64 = nfb;
+        st->kernel_s += s2.kernel_s;
+        for (int k = 0; k < 6; ++k) st->phase_s[k] += s2.phase_s[k];
     }
 }
 
 }  // namespace
 
+namespace {
+int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+              const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack, bool dc,
+              pfgpu_grid** out);
+}  // namespace
+
 extern "C" {
 
-const char* pfgpu_last_error(void) { return g_last_error.c_str(); }
+const char* pfgpu_last_error(void) { return pfgpu_engine::last_error().c_str(); }
 int pfgpu_abi_version(void) { return PFGPU_ABI_VERSION; }
 
 int pfgpu_ctx_create(int device, size_t hbm_budget_bytes, pfgpu_ctx** out) {
@@ -1550,6 +1834,9 @@ int pfgpu_ctx_create(int device, size_t hbm_budget_bytes, pfgpu_ctx** out) {
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
         HIP_OK(hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        int wall_khz = 0;
+        if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
+            c->wall_hz = 1e3 * wall_khz;
         size_t fr = 0, tot = 0;
         HIP_OK(hipMemGetInfo(&fr, &tot));
         c->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * (double)fr);
@@ -1565,7 +1852,8 @@ int pfgpu_ctx_destroy(pfgpu_ctx* ctx) {
         if (!ctx) return 0;
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-        if (ctx->ws) (void)hipFree(ctx->ws);
+        for (void* p : {ctx->ws, ctx->aux, ctx->io, ctx->stage})
+            if (p) (void)hipFree(p);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1592,6 +1880,39 @@ int pfgpu_ctx_device_info(pfgpu_ctx* ctx, int* n_cu, int* lds_bytes, size_t* fre
 int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
                     const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack,
                     pfgpu_grid** out) {
+    return load_grid(ctx, n, Yp, Yi, slack_ids, nslack, slack_weights, pv, npv, pq, npq, single_slack, false, out);
+}
+
+int pfgpu_grid_load_dc(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+                       const int* pv, int npv, const int* pq, int npq, pfgpu_grid** out) {
+    return guarded([&]() -> int {
+        if (!ctx || !out || !slack_ids || n <= 0 || nslack < 1 || npv < 0 || npq < 0 || (npv > 0 && !pv) ||
+            (npq > 0 && !pq))
+            throw ApiError(PFGPU_E_INVALID, "invalid grid arguments");
+        // BaseDCAlgo::compute_pf (BaseDCAlgo.tpp:61-72): my_pv = slack_ids[1:] ++ pv; the buses in
+        // neither my_pv nor pq are the slack buses (their angle is fixed), every other bus carries
+        // its angle: the single-slack layout with all of them as PV and no PQ bus
+        std::vector<char> unk(n, 0);
+        for (int k = 1; k < nslack; ++k)
+            if (slack_ids[k] >= 0 && slack_ids[k] < n) unk[slack_ids[k]] = 1;
+        for (int k = 0; k < npv; ++k)
+            if (pv[k] >= 0 && pv[k] < n) unk[pv[k]] = 1;
+        for (int k = 0; k < npq; ++k)
+            if (pq[k] >= 0 && pq[k] < n) unk[pq[k]] = 1;
+        std::vector<int> ang, slack;
+        for (int b = 0; b < n; ++b) (unk[b] ? ang : slack).push_back(b);
+        if (slack.empty()) throw ApiError(PFGPU_E_INVALID, "All buses are tagged as PV or PQ, there can be no slack.");
+        std::vector<double> w(n, 0.0);
+        w[slack[0]] = 1.0;
+        return load_grid(ctx, n, Yp, Yi, slack.data(), (int)slack.size(), w.data(), ang.data(), (int)ang.size(),
+                         nullptr, 0, 1, true, out);
+    });
+}
+
+namespace {
+int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+              const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack, bool dc,
+              pfgpu_grid** out) {
     return guarded([&]() -> int {
         if (!ctx || !out || !Yp || !Yi || !slack_ids || !slack_weights || n <= 0 || nslack < 1 || npv < 0 || npq < 0 ||
             (npv > 0 && !pv) || (npq > 0 && !pq))
@@ -1638,6 +1959,8 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         g->info.n_pairs = (int64_t)P.pairs.size() / 4;
         g->info.flops = P.flops;
         g->info.single_slack = single_slack ? 1 : 0;
+        g->info.dc = dc ? 1 : 0;
+        g->dc = dc;
         g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
         {
             std::vector<int> pc(2 * P.yr_pos.size());
@@ -1685,6 +2008,7 @@ int pfgpu_grid_load(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const i
         return 0;
     });
 }
+}  // namespace
 
 int pfgpu_grid_destroy(pfgpu_grid* grid) {
     return guarded([&]() -> int {
@@ -1712,6 +2036,14 @@ int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji) {
     });
 }
 
+int pfgpu_grid_reset(pfgpu_grid* grid) {
+    return guarded([&]() -> int {
+        if (!grid) throw ApiError(PFGPU_E_INVALID, "grid is NULL");
+        grid->fresh = true;
+        return 0;
+    });
+}
+
 int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
                           const int* d_dptr, const int* d_dpos, const double* d_dval, const double* d_Sbus,
                           int sbus_stride, const double* d_V0, int v0_stride, double* d_V, int max_iter, double tol,
@@ -1720,28 +2052,37 @@ int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const dou
     return guarded([&]() -> int {
         if (!ctx || !grid || !d_Yx || !d_Sbus || !d_V0 || !d_V || !d_iters || !d_status || ninst <= 0)
             throw ApiError(PFGPU_E_INVALID, "invalid autotune arguments");
-        if (precision != PFGPU_FP64 && precision != PFGPU_FP32) throw ApiError(PFGPU_E_INVALID, "bad precision");
+        if (precision != PFGPU_FP64 && precision != PFGPU_FP32 && precision != PFGPU_FP32_STRICT)
+            throw ApiError(PFGPU_E_INVALID, "bad precision");
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
+        const int fsize = precision == PFGPU_FP64 ? 8 : 4;
+        const int pslot = precision == PFGPU_FP64 ? 0 : 1;
         double best = 1e300;
         int bG = 1, bNT = 256;
+        SolveArgs q;
+        q.ninst = ninst; q.nrows = ninst; q.Yx = d_Yx; q.Yx_stride = Yx_stride; q.dptr = d_dptr; q.dpos = d_dpos;
+        q.dval = d_dval; q.S = d_Sbus; q.sbus_stride = sbus_stride; q.V0 = d_V0; q.v0_stride = v0_stride; q.V = d_V;
+        q.max_iter = max_iter; q.tol = tol; q.iters = d_iters; q.status = d_status;
+        // the kernel alone is timed: no FP64 re-solve pass
+        q.precision = precision == PFGPU_FP64 ? PFGPU_FP64 : PFGPU_FP32_STRICT;
         for (int G = 1; G <= kMaxG; G <<= 1) {
             for (int NT : {256, 1024}) {
-                if (64 / G < 1) continue;
-                LaunchChoice lc = choose_launch(ctx, grid->P, ninst, precision == PFGPU_FP32 ? 4 : 8, G, NT);
+                LaunchChoice lc = choose_launch(ctx, grid->P, ninst, fsize, G, NT);
                 if (lc.G != G) continue;  // the dense tail did not fit at this G
+                q.G_req = G;
+                q.NT_req = NT;
                 double tmin = 1e300;
                 for (int rep = 0; rep < 2; ++rep) {
                     pfgpu_stats st{};
-                    solve_device(ctx, grid, ninst, d_Yx, Yx_stride, d_dptr, d_dpos, d_dval, d_Sbus, sbus_stride, d_V0,
-                                 v0_stride, d_V, max_iter, tol, precision, G, d_iters, d_status, nullptr, &st, NT);
+                    solve_device(ctx, grid, q, &st);
                     tmin = std::min(tmin, st.kernel_s);
                 }
                 if (tmin < best) { best = tmin; bG = G; bNT = NT; }
             }
         }
-        grid->tuned_G[precision] = bG;
-        grid->tuned_NT[precision] = bNT;
+        grid->tuned_G[pslot] = bG;
+        grid->tuned_NT[pslot] = bNT;
         if (best_G) *best_G = bG;
         if (best_block) *best_block = bNT;
         if (best_seconds) *best_seconds = best;
@@ -1814,6 +2155,29 @@ int pfgpu_memcpy_d2d(pfgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
     });
 }
 
+int pfgpu_ctx_last_flags(pfgpu_ctx* ctx, int* flags, long long nrows) {
+    return guarded([&]() -> int {
+        if (!ctx || (!flags && nrows > 0) || nrows < 0) throw ApiError(PFGPU_E_INVALID, "invalid arguments");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        if (nrows > ctx->last_rows) throw ApiError(PFGPU_E_INVALID, "more rows than the last solve call had");
+        if (nrows == 0) return 0;
+        const AuxLayout AL = aux_layout(1, 1);
+        HIP_OK(hipMemcpy(flags, static_cast<char*>(ctx->aux) + AL.o_flags, (size_t)nrows * 4, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+namespace {
+void check_deltas_host(const int* dptr, const int* dpos, int ninst, int nnzY) {
+    if (dptr[0] != 0) throw ApiError(PFGPU_E_INVALID, "dptr[0] must be 0");
+    for (int i = 0; i < ninst; ++i)
+        if (dptr[i + 1] < dptr[i]) throw ApiError(PFGPU_E_INVALID, "dptr must be non-decreasing");
+    for (int k = 0; k < dptr[ninst]; ++k)
+        if (dpos[k] < 0 || dpos[k] >= nnzY) throw ApiError(PFGPU_E_INVALID, "delta position out of range");
+}
+}  // namespace
+
 int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
                              const int* d_dptr, const int* d_dpos, const double* d_dval, const double* d_Sbus,
                              int sbus_stride, const double* d_V0, int v0_stride, double* d_V, int max_iter, double tol,
@@ -1827,12 +2191,113 @@ int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const 
         HIP_OK(hipSetDevice(ctx->device));
         auto t0 = std::chrono::steady_clock::now();
         if (stats) *stats = pfgpu_stats{};
-        solve_device(ctx, grid, ninst, d_Yx, Yx_stride, d_dptr, d_dpos, d_dval, d_Sbus, sbus_stride, d_V0, v0_stride,
-                     d_V, max_iter, tol, precision, instances_per_wg, d_iters, d_status, nullptr, stats);
+        SolveArgs q;
+        q.ninst = ninst; q.nrows = ninst; q.Yx = d_Yx; q.Yx_stride = Yx_stride; q.dptr = d_dptr; q.dpos = d_dpos;
+        q.dval = d_dval; q.S = d_Sbus; q.sbus_stride = sbus_stride; q.V0 = d_V0; q.v0_stride = v0_stride; q.V = d_V;
+        q.max_iter = max_iter; q.tol = tol; q.precision = precision; q.G_req = instances_per_wg;
+        q.iters = d_iters; q.status = d_status;
+        solve_device(ctx, grid, q, stats);
         if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return 0;
     });
 }
+
+int pfgpu_solve_chains_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int chain_len, const double* d_Yx,
+                              int Yx_stride, const double* d_Sbus, const double* d_V0, int v0_stride, double* d_V,
+                              int max_iter, double tol, int precision, int instances_per_wg, int* d_iters,
+                              int* d_status, pfgpu_stats* stats) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !d_Yx || !d_Sbus || !d_V0 || !d_V || !d_iters || !d_status || nrows < 0 || chain_len < 1)
+            throw ApiError(PFGPU_E_INVALID, "invalid chain arguments");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        if (stats) *stats = pfgpu_stats{};
+        SolveArgs q;
+        q.chain = std::min(chain_len, std::max(nrows, 1));
+        q.ninst = (nrows + q.chain - 1) / q.chain;
+        q.nrows = nrows;
+        q.Yx = d_Yx; q.Yx_stride = Yx_stride; q.S = d_Sbus; q.sbus_stride = grid->P.n; q.V0 = d_V0;
+        q.v0_stride = v0_stride; q.V = d_V; q.max_iter = max_iter; q.tol = tol; q.precision = precision;
+        q.G_req = instances_per_wg; q.iters = d_iters; q.status = d_status;
+        solve_device(ctx, grid, q, stats);
+        if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    });
+}
+
+namespace {
+// Host-buffer solve: the caller's arrays are copied into the context's staging buffer, solved
+// there, and the results copied back.
+int solve_host(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, int chain, long long nrows, const double* Yx,
+               int Yx_stride, const int* dptr, const int* dpos, const double* dval, const double* Sbus, int sbus_stride,
+               const double* V0, int v0_stride, double* V, int max_iter, double tol, int precision,
+               int instances_per_wg, int* iters, int* status, double* Jx_last, pfgpu_stats* stats) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    pfgpu_stats st{};
+    const pfgpu::DevicePlan& P = grid->P;
+    const int n = P.n;
+    const int nd = dptr ? dptr[ninst] : 0;
+    const size_t ny = (size_t)(Yx_stride ? ninst : 1) * P.nnzY * 16;
+    const size_t ns = (size_t)(sbus_stride ? nrows : 1) * n * 16;
+    const size_t nv0 = (size_t)(v0_stride ? ninst : 1) * n * 16;
+    const size_t nv = (size_t)nrows * n * 16;
+    size_t off = 0;
+    const size_t o_y = off; off += align_up(ny);
+    const size_t o_s = off; off += align_up(ns);
+    const size_t o_v0 = off; off += align_up(nv0);
+    const size_t o_v = off; off += align_up(nv);
+    const size_t o_it = off; off += align_up((size_t)nrows * 4);
+    const size_t o_st = off; off += align_up((size_t)nrows * 4);
+    const size_t o_j = off; off += Jx_last ? align_up((size_t)P.nJ * 8) : 0;
+    const size_t o_dp = off; off += dptr ? align_up((size_t)(ninst + 1) * 4) : 0;
+    const size_t o_dq = off; off += dptr ? align_up((size_t)std::max(nd, 1) * 4) : 0;
+    const size_t o_dv = off; off += dptr ? align_up((size_t)std::max(nd, 1) * 16) : 0;
+    char* b = static_cast<char*>(grow_buffer(ctx->stage, ctx->stage_bytes, std::max<size_t>(off, 256)));
+    hipStream_t s = ctx->stream;
+    HIP_OK(hipMemcpyAsync(b + o_y, Yx, ny, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(b + o_s, Sbus, ns, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(b + o_v0, V0, nv0, hipMemcpyHostToDevice, s));
+    if (dptr) {
+        HIP_OK(hipMemcpyAsync(b + o_dp, dptr, (size_t)(ninst + 1) * 4, hipMemcpyHostToDevice, s));
+        if (nd) {
+            HIP_OK(hipMemcpyAsync(b + o_dq, dpos, (size_t)nd * 4, hipMemcpyHostToDevice, s));
+            HIP_OK(hipMemcpyAsync(b + o_dv, dval, (size_t)nd * 16, hipMemcpyHostToDevice, s));
+        }
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    auto t1 = clk::now();
+    st.h2d_s = std::chrono::duration<double>(t1 - t0).count();
+    SolveArgs q;
+    q.ninst = ninst; q.chain = chain; q.nrows = nrows;
+    q.Yx = reinterpret_cast<const double*>(b + o_y); q.Yx_stride = Yx_stride;
+    if (dptr) {
+        q.dptr = reinterpret_cast<const int*>(b + o_dp);
+        q.dpos = reinterpret_cast<const int*>(b + o_dq);
+        q.dval = reinterpret_cast<const double*>(b + o_dv);
+    }
+    q.S = reinterpret_cast<const double*>(b + o_s); q.sbus_stride = sbus_stride;
+    q.V0 = reinterpret_cast<const double*>(b + o_v0); q.v0_stride = v0_stride;
+    q.V = reinterpret_cast<double*>(b + o_v);
+    q.max_iter = max_iter; q.tol = tol; q.precision = precision; q.G_req = instances_per_wg;
+    q.iters = reinterpret_cast<int*>(b + o_it); q.status = reinterpret_cast<int*>(b + o_st);
+    q.Jx_last = Jx_last ? reinterpret_cast<double*>(b + o_j) : nullptr;
+    solve_device(ctx, grid, q, &st);
+    auto t2 = clk::now();
+    HIP_OK(hipMemcpyAsync(V, b + o_v, nv, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(iters, b + o_it, (size_t)nrows * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(status, b + o_st, (size_t)nrows * 4, hipMemcpyDeviceToHost, s));
+    if (Jx_last) HIP_OK(hipMemcpyAsync(Jx_last, b + o_j, (size_t)P.nJ * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    st.d2h_s = std::chrono::duration<double>(clk::now() - t2).count();
+    st.total_s = std::chrono::duration<double>(clk::now() - t0).count();
+    if (stats) *stats = st;
+    return 0;
+}
+}  // namespace
 
 int pfgpu_solve_batch(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* Yx, int Yx_stride, const int* dptr,
                       const int* dpos, const double* dval, const double* Sbus, int sbus_stride, double* V, int v0_stride,
@@ -1843,123 +2308,27 @@ int pfgpu_solve_batch(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double*
             throw ApiError(PFGPU_E_INVALID, "invalid solve arguments");
         if ((dptr == nullptr) != (dpos == nullptr) || (dptr == nullptr) != (dval == nullptr))
             throw ApiError(PFGPU_E_INVALID, "dptr/dpos/dval must be all set or all NULL");
+        if (stats) *stats = pfgpu_stats{};
         if (ninst == 0) return 0;
-        std::unique_lock<std::mutex> lk(ctx->mu);
-        HIP_OK(hipSetDevice(ctx->device));
-        using clk = std::chrono::steady_clock;
-        auto t0 = clk::now();
-        pfgpu_stats st{};
-        const pfgpu::DevicePlan& P = grid->P;
-        const int n = P.n;
-        if (dptr) {
-            if (dptr[0] != 0) throw ApiError(PFGPU_E_INVALID, "dptr[0] must be 0");
-            for (int i = 0; i < ninst; ++i)
-                if (dptr[i + 1] < dptr[i]) throw ApiError(PFGPU_E_INVALID, "dptr must be non-decreasing");
-            for (int k = 0; k < dptr[ninst]; ++k)
-                if (dpos[k] < 0 || dpos[k] >= P.nnzY) throw ApiError(PFGPU_E_INVALID, "delta position out of range");
-        }
-        const size_t ny = (size_t)(Yx_stride ? ninst : 1) * P.nnzY * 16;
-        const size_t ns = (size_t)(sbus_stride ? ninst : 1) * n * 16;
-        const size_t nv = (size_t)ninst * n * 16;
-        const int nd = dptr ? dptr[ninst] : 0;
-        std::vector<double> v0_copy;
-        if (precision == PFGPU_FP32) v0_copy.assign(V, V + (v0_stride ? (size_t)ninst : 1) * n * 2);
-        struct Tmp {
-            std::vector<void*> ptrs;
-            ~Tmp() { for (void* p : ptrs) (void)hipFree(p); }
-            void* get(size_t b) { void* p = nullptr; HIP_OK(hipMalloc(&p, std::max<size_t>(b, 1))); ptrs.push_back(p); return p; }
-        } tmp;
-        double* dY = static_cast<double*>(tmp.get(ny));
-        double* dS = static_cast<double*>(tmp.get(ns));
-        double* dV = static_cast<double*>(tmp.get(nv));
-        int* dIt = static_cast<int*>(tmp.get((size_t)ninst * 4));
-        int* dSt = static_cast<int*>(tmp.get((size_t)ninst * 4));
-        double* dJ = Jx_last ? static_cast<double*>(tmp.get((size_t)P.nJ * 8)) : nullptr;
-        int *ddp = nullptr, *ddq = nullptr;
-        double* ddv = nullptr;
-        if (dptr) {
-            ddp = static_cast<int*>(tmp.get((size_t)(ninst + 1) * 4));
-            ddq = static_cast<int*>(tmp.get((size_t)std::max(nd, 1) * 4));
-            ddv = static_cast<double*>(tmp.get((size_t)std::max(nd, 1) * 16));
-        }
-        hipStream_t s = ctx->stream;
-        HIP_OK(hipMemcpyAsync(dY, Yx, ny, hipMemcpyHostToDevice, s));
-        HIP_OK(hipMemcpyAsync(dS, Sbus, ns, hipMemcpyHostToDevice, s));
-        HIP_OK(hipMemcpyAsync(dV, V, v0_stride ? nv : (size_t)n * 16, hipMemcpyHostToDevice, s));
-        if (dptr) {
-            HIP_OK(hipMemcpyAsync(ddp, dptr, (size_t)(ninst + 1) * 4, hipMemcpyHostToDevice, s));
-            if (nd) {
-                HIP_OK(hipMemcpyAsync(ddq, dpos, (size_t)nd * 4, hipMemcpyHostToDevice, s));
-                HIP_OK(hipMemcpyAsync(ddv, dval, (size_t)nd * 16, hipMemcpyHostToDevice, s));
-            }
-        }
-        HIP_OK(hipStreamSynchronize(s));
-        auto t1 = clk::now();
-        st.h2d_s = std::chrono::duration<double>(t1 - t0).count();
-        solve_device(ctx, grid, ninst, dY, Yx_stride, ddp, ddq, ddv, dS, sbus_stride, dV, v0_stride, dV, max_iter, tol,
-                     precision, instances_per_wg, dIt, dSt, dJ, &st);
-        auto t2 = clk::now();
-        HIP_OK(hipMemcpyAsync(V, dV, nv, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(iters, dIt, (size_t)ninst * 4, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(status, dSt, (size_t)ninst * 4, hipMemcpyDeviceToHost, s));
-        if (Jx_last) HIP_OK(hipMemcpyAsync(Jx_last, dJ, (size_t)P.nJ * 8, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        st.d2h_s = std::chrono::duration<double>(clk::now() - t2).count();
-        if (precision == PFGPU_FP32) {
-            // Instances that did not converge with FP32 factors are re-solved with
-            // FP64 factors from their original initial voltages.
-            std::vector<int> bad;
-            for (int i = 0; i < ninst; ++i)
-                if (status[i] != 0) bad.push_back(i);
-            if (!bad.empty()) {
-                const int nb = (int)bad.size();
-                std::vector<double> subY, subS, subV((size_t)nb * n * 2), subdv;
-                std::vector<int> subdp, subdq, subIt(nb), subSt(nb);
-                if (Yx_stride) {
-                    subY.resize((size_t)nb * P.nnzY * 2);
-                    for (int k = 0; k < nb; ++k)
-                        std::memcpy(&subY[(size_t)k * P.nnzY * 2], Yx + (size_t)bad[k] * P.nnzY * 2, (size_t)P.nnzY * 16);
-                }
-                if (sbus_stride) {
-                    subS.resize((size_t)nb * n * 2);
-                    for (int k = 0; k < nb; ++k)
-                        std::memcpy(&subS[(size_t)k * n * 2], Sbus + (size_t)bad[k] * n * 2, (size_t)n * 16);
-                }
-                for (int k = 0; k < nb; ++k)
-                    std::memcpy(&subV[(size_t)k * n * 2], v0_copy.data() + (v0_stride ? (size_t)bad[k] * n * 2 : 0),
-                                (size_t)n * 16);
-                if (dptr) {
-                    subdp.push_back(0);
-                    for (int k = 0; k < nb; ++k) {
-                        for (int q = dptr[bad[k]]; q < dptr[bad[k] + 1]; ++q) {
-                            subdq.push_back(dpos[q]);
-                            subdv.push_back(dval[2 * q]);
-                            subdv.push_back(dval[2 * q + 1]);
-                        }
-                        subdp.push_back((int)subdq.size());
-                    }
-                    if (subdq.empty()) { subdq.push_back(0); subdv.push_back(0); subdv.push_back(0); }
-                }
-                pfgpu_stats st2{};
-                lk.unlock();
-                const int rc = pfgpu_solve_batch(ctx, grid, nb, Yx_stride ? subY.data() : Yx, Yx_stride,
-                                                 dptr ? subdp.data() : nullptr, dptr ? subdq.data() : nullptr,
-                                                 dptr ? subdv.data() : nullptr, sbus_stride ? subS.data() : Sbus,
-                                                 sbus_stride, subV.data(), n, max_iter, tol, PFGPU_FP64,
-                                                 instances_per_wg, subIt.data(), subSt.data(), nullptr, &st2);
-                if (rc != 0) return rc;
-                for (int k = 0; k < nb; ++k) {
-                    std::memcpy(V + (size_t)bad[k] * n * 2, &subV[(size_t)k * n * 2], (size_t)n * 16);
-                    iters[bad[k]] = subIt[k];
-                    status[bad[k]] = subSt[k];
-                }
-                st.fallback_fp64 = nb;
-                st.kernel_s += st2.kernel_s;
-            }
-        }
-        st.total_s = std::chrono::duration<double>(clk::now() - t0).count();
-        if (stats) *stats = st;
-        return 0;
+        if (dptr) check_deltas_host(dptr, dpos, ninst, grid->P.nnzY);
+        return solve_host(ctx, grid, ninst, 1, ninst, Yx, Yx_stride, dptr, dpos, dval, Sbus, sbus_stride, V, v0_stride,
+                          V, max_iter, tol, precision, instances_per_wg, iters, status, Jx_last, stats);
+    });
+}
+
+int pfgpu_solve_chains(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int chain_len, const double* Yx, const double* Sbus,
+                       const double* V0, int v0_stride, double* V, int max_iter, double tol, int precision,
+                       int instances_per_wg, int* iters, int* status, pfgpu_stats* stats) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !Yx || !Sbus || !V0 || !V || !iters || !status || nrows < 0 || chain_len < 1)
+            throw ApiError(PFGPU_E_INVALID, "invalid chain arguments");
+        if (v0_stride != 0 && v0_stride != grid->P.n) throw ApiError(PFGPU_E_INVALID, "v0_stride must be 0 or n");
+        if (stats) *stats = pfgpu_stats{};
+        if (nrows == 0) return 0;
+        const int L = std::min(chain_len, nrows);
+        const int nch = (nrows + L - 1) / L;
+        return solve_host(ctx, grid, nch, L, nrows, Yx, 0, nullptr, nullptr, nullptr, Sbus, grid->P.n, V0, v0_stride, V,
+                          max_iter, tol, precision, instances_per_wg, iters, status, nullptr, stats);
     });
 }
 

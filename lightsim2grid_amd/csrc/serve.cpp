@@ -33,11 +33,25 @@ void ServeLoop::close() {
         stop_ = true;
     }
     cv_work_.notify_all();
-    cv_done_.notify_all();
     for (auto& g : grids_)
         if (g->worker.joinable()) g->worker.join();
+    {
+        // requests still queued fail, so that no caller waits for them forever
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& g : grids_) {
+            for (auto& r : g->queue) {
+                ServeResult o;
+                o.ninst = r.ninst;
+                o.ok = false;
+                o.error = "ServeLoop closed before the request ran";
+                done_[r.id] = std::move(o);
+            }
+            g->queue.clear();
+        }
+    }
+    cv_done_.notify_all();
     for (auto& g : grids_) {
-        for (void* p : {(void*)g->dY, (void*)g->dS0, (void*)g->dV0, (void*)g->dS, (void*)g->dVin, (void*)g->dV,
+        for (void* p : {(void*)g->dY, (void*)g->dS0, (void*)g->dV0, (void*)g->dS, (void*)g->dV,
                         (void*)g->ddv, (void*)g->ddp, (void*)g->ddq, (void*)g->dIt, (void*)g->dSt})
             if (p) pfgpu_dev_free(g->ctx, p);
         if (g->grid) pfgpu_grid_destroy(g->grid);
@@ -66,7 +80,9 @@ int ServeLoop::add_grid(const CscC& Y, const CplxVect& Sbus, const CplxVect& V0,
     check_rc(pfgpu_grid_get_info(g->grid, &info), "pfgpu_grid_get_info");
     // per-instance device bytes of a launch: Ybus values (contingencies), Va/Vm/Ibus, spilled
     // value slots (upper bound), Sbus / V in / V out, iterations and status
-    g->bytes_per_inst = (size_t)g->nnzY * 16 + (size_t)n * (16 + 16 + 16 * 3) + (size_t)info.n_slots * 8 + 64;
+    // transient device bytes one instance of a launch may take in the engine (at most): Va, Vm and
+    // Ibus rows, FP64 spilled value slots, flags and fallback list
+    g->bytes_per_inst = (size_t)n * 32 + (size_t)info.n_slots * 8 + 64;
     auto alloc = [&](double** p, size_t bytes) {
         void* q = nullptr;
         check_rc(pfgpu_dev_alloc(g->ctx, bytes, &q), "pfgpu_dev_alloc");
@@ -112,9 +128,19 @@ uint64_t ServeLoop::submit(int grid, int kind, int ninst, const CplxVect& sbus, 
     r.dpos = dpos;
     r.dval = dval;
     r.t_submit = clk::now();
+    pending_[r.id] = {grid, ninst};
     g.queue.push_back(std::move(r));
     cv_work_.notify_all();
     return next_id_ - 1;
+}
+
+bool ServeLoop::request_shape(uint64_t id, int& ninst, int& n) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = pending_.find(id);
+    if (it == pending_.end()) return false;
+    ninst = it->second.second;
+    n = it->second.first < (int)grids_.size() ? grids_[it->second.first]->n : 0;
+    return true;
 }
 
 bool ServeLoop::ready(uint64_t id) {
@@ -124,12 +150,18 @@ bool ServeLoop::ready(uint64_t id) {
 
 bool ServeLoop::wait(uint64_t id, double timeout_s, ServeResult& out) {
     std::unique_lock<std::mutex> lk(mu_);
+    if (!pending_.count(id)) throw std::runtime_error("ServeLoop: unknown request id (never issued, or its result was taken)");
     auto pred = [&] { return done_.count(id) != 0; };
     if (timeout_s < 0) cv_done_.wait(lk, pred);
     else if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return false;
     out = std::move(done_[id]);
     done_.erase(id);
+    pending_.erase(id);
     return true;
+}
+
+size_t ServeLoop::buffer_bytes(const Grid& g, size_t ninst, size_t ndelta) {
+    return ninst * ((size_t)g.n * 32 + 12) + 4 + ndelta * 20;
 }
 
 void ServeLoop::ensure_capacity(Grid& g, size_t ninst, size_t ndelta) {
@@ -139,7 +171,7 @@ void ServeLoop::ensure_capacity(Grid& g, size_t ninst, size_t ndelta) {
         check_rc(pfgpu_dev_alloc(g.ctx, bytes, p), "pfgpu_dev_alloc");
     };
     if (ninst > g.cap) {
-        const size_t c = std::max(ninst, g.cap * 2);
+        const size_t c = ninst;
         grow((void**)&g.dS, c * g.n * 16);
         grow((void**)&g.dV, c * g.n * 16);
         grow((void**)&g.ddp, (c + 1) * 4);
@@ -148,11 +180,12 @@ void ServeLoop::ensure_capacity(Grid& g, size_t ninst, size_t ndelta) {
         g.cap = c;
     }
     if (ndelta > g.cap_d || !g.ddq) {
-        const size_t c = std::max<size_t>({ndelta, g.cap_d * 2, 16});
+        const size_t c = std::max<size_t>(ndelta, 16);
         grow((void**)&g.ddq, c * 4);
         grow((void**)&g.ddv, c * 16);
         g.cap_d = c;
     }
+    g.held = buffer_bytes(g, g.cap, g.cap_d);
 }
 
 void ServeLoop::run(int gi) {
@@ -166,16 +199,21 @@ void ServeLoop::run(int gi) {
             for (;;) {
                 if (stop_) return;
                 if (!g.queue.empty()) {
-                    // admit FIFO requests while they fit the instance cap and the HBM budget
+                    // admit FIFO requests while they fit the instance cap and the HBM budget: the
+                    // launch's transient workspace plus what its buffers would grow by
                     int total = 0;
-                    size_t need = 0;
+                    size_t nd = 0, need = 0;
                     while (!g.queue.empty()) {
                         const Req& r = g.queue.front();
-                        const size_t b = (size_t)r.ninst * g.bytes_per_inst;
                         if (!batch.empty() && total + r.ninst > max_inst_) break;
-                        if (in_use_.load() + need + b > budget_) break;
+                        const size_t grow = std::max(buffer_bytes(g, std::max(g.cap, (size_t)(total + r.ninst)),
+                                                                  std::max(g.cap_d, nd + r.dpos.size())),
+                                                     g.held) - g.held;
+                        const size_t b = (size_t)(total + r.ninst) * g.bytes_per_inst + grow;
+                        if (in_use_.load() + b > budget_) break;
                         total += r.ninst;
-                        need += b;
+                        nd += r.dpos.size();
+                        need = b;
                         batch.push_back(std::move(g.queue.front()));
                         g.queue.pop_front();
                     }
@@ -200,25 +238,30 @@ void ServeLoop::run(int gi) {
         ServeResult err_res;
         std::vector<ServeResult> res(batch.size());
         try {
+            const size_t held0 = g.held;
             ensure_capacity(g, M, nd);
-            // Sbus rows: resident base row (device copy) or the request's own rows (host copy)
+            // Sbus rows (the request's own, or the resident base row) packed on the host, one copy
             int row = 0;
             std::vector<int> dptr(1, 0);
             IntVect dpos;
             CplxVect dval;
+            CplxVect Srows((size_t)M * n);
             for (auto& r : batch) {
                 for (int k = 0; k < r.ninst; ++k, ++row) {
-                    double* dst = g.dS + (size_t)row * n * 2;
-                    if (r.kind == 2)
-                        check_rc(pfgpu_memcpy_h2d(g.ctx, dst, r.sbus.data() + (size_t)k * n, (size_t)n * 16), "h2d");
-                    else if (r.kind == 0 && !r.sbus.empty())
-                        check_rc(pfgpu_memcpy_h2d(g.ctx, dst, r.sbus.data(), (size_t)n * 16), "h2d");
-                    else
-                        check_rc(pfgpu_memcpy_d2d(g.ctx, dst, g.dS0, (size_t)n * 16), "d2d");
+                    const cplx* src = r.kind == 2 ? r.sbus.data() + (size_t)k * n
+                                      : (r.kind == 0 && !r.sbus.empty()) ? r.sbus.data() : g.Sbus.data();
+                    std::copy(src, src + n, Srows.begin() + (size_t)row * n);
                     if (r.kind == 1)
                         for (int q = r.dptr[k]; q < r.dptr[k + 1]; ++q) { dpos.push_back(r.dpos[q]); dval.push_back(r.dval[q]); }
                     dptr.push_back((int)dpos.size());
                 }
+            }
+            check_rc(pfgpu_memcpy_h2d(g.ctx, g.dS, Srows.data(), Srows.size() * 16), "h2d");
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                in_use_ += g.held - held0;  // kept until close
+                in_use_ -= std::min(reserved, g.held - held0);
+                reserved -= std::min(reserved, g.held - held0);
             }
             if (any_delta) {
                 check_rc(pfgpu_memcpy_h2d(g.ctx, g.ddp, dptr.data(), dptr.size() * 4), "h2d");

@@ -10,6 +10,9 @@
 // Request kinds: a single power flow (optionally with its own Sbus), a
 // contingency batch (per-outage Ybus deltas, warm start from the resident V), a
 // time-series chunk (one Sbus row per step, each warm-started from the resident V).
+//
+// Device memory: each grid's launch buffers grow with the largest launch it ran and are kept
+// (counted in in_use() until close); a launch is admitted when what it would add fits the budget.
 #pragma once
 
 #include <atomic>
@@ -53,10 +56,17 @@ public:
     uint64_t submit(int grid, int kind, int ninst, const CplxVect& sbus, const IntVect& dptr, const IntVect& dpos,
                     const CplxVect& dval);
     bool ready(uint64_t id);
-    // waits up to timeout_s (<0: forever); returns false on timeout
+    // waits up to timeout_s (<0: forever); returns false on timeout; throws for an id that was
+    // never issued or whose result was already taken
     bool wait(uint64_t id, double timeout_s, ServeResult& out);
+    // rows (instances) and bus count of a pending request, for callers that size result buffers
+    bool request_shape(uint64_t id, int& ninst, int& n);
     void close();
     int n_grids() const { return (int)grids_.size(); }
+    int grid_buses(int grid) const {
+        if (grid < 0 || grid >= (int)grids_.size()) throw std::runtime_error("ServeLoop: bad grid");
+        return grids_[grid]->n;
+    }
     size_t budget() const { return budget_; }
     size_t in_use() const { return in_use_.load(); }
     int launches() const { return launches_.load(); }
@@ -77,11 +87,12 @@ private:
         pfgpu_ctx* ctx = nullptr;
         pfgpu_grid* grid = nullptr;
         size_t bytes_per_inst = 0;
+        size_t held = 0;  // bytes of this grid's grow-only device buffers (counted in in_use_)
         // resident device data
         double *dY = nullptr, *dS0 = nullptr, *dV0 = nullptr;
         // grow-only launch buffers
         size_t cap = 0;
-        double *dS = nullptr, *dVin = nullptr, *dV = nullptr, *ddv = nullptr;
+        double *dS = nullptr, *dV = nullptr, *ddv = nullptr;
         int *ddp = nullptr, *ddq = nullptr, *dIt = nullptr, *dSt = nullptr;
         size_t cap_d = 0;
         std::deque<Req> queue;
@@ -89,6 +100,7 @@ private:
     };
     void run(int gi);
     void ensure_capacity(Grid& g, size_t ninst, size_t ndelta);
+    static size_t buffer_bytes(const Grid& g, size_t ninst, size_t ndelta);
 
     size_t budget_;
     int max_inst_;
@@ -97,6 +109,7 @@ private:
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
     std::map<uint64_t, ServeResult> done_;
+    std::map<uint64_t, std::pair<int, int>> pending_;  // id -> (grid, ninst) until its result is taken
     std::atomic<size_t> in_use_{0};
     std::atomic<int> launches_{0};
     uint64_t next_id_ = 1;

@@ -684,6 +684,62 @@ int oracle_lu_solve(int n, const int* Ap, const int* Ai, const double* Ax, doubl
     return e;
 }
 
+// DC power flow, BaseDCAlgo<KLU>::compute_pf (src/powerflow_algorithm/BaseDCAlgo.tpp:15-179):
+// my_pv = slack_ids[1:] ++ pv (BaseAlgo.hpp:201-216); the buses in neither my_pv nor pq are the
+// slack buses (BaseAlgo.cpp:143-181); the others are numbered in bus order (fill_mat_bus_id,
+// :196-205); B = Re(Ybus) and P = Re(Sbus) without the slack rows / columns (:207-233, :93-101);
+// B theta = P is factored (analyze + factor) and solved; theta must be finite with |theta|inf <
+// 1e6, else SolverSolve (:131-147); Va = theta (0 at the slack buses) + arg(V[first slack]),
+// Vm = |V|, V = Vm exp(j Va) (:154-177). V is in/out; returns the ErrorType code (0 converged).
+int oracle_dc(int n, const int* Yp, const int* Yi, const double* Yx, double* V, const double* Sbus,
+              const int* slack_ids, int nslack, const int* pv, int npv, const int* pq, int npq) {
+    std::vector<char> in_pv_pq(n, 0);
+    for (int s = 1; s < nslack; ++s) in_pv_pq[slack_ids[s]] = 1;
+    for (int k = 0; k < npv; ++k) in_pv_pq[pv[k]] = 1;
+    for (int k = 0; k < npq; ++k) in_pv_pq[pq[k]] = 1;
+    std::vector<int> id(n, -1), slack;
+    int m = 0;
+    for (int b = 0; b < n; ++b) {
+        if (in_pv_pq[b]) id[b] = m++;
+        else slack.push_back(b);
+    }
+    if (slack.empty()) return SolverAnalyze;
+    std::vector<int> Bp(m + 1, 0), Bi;
+    std::vector<double> Bx;
+    for (int c = 0; c < n; ++c) {
+        if (id[c] < 0) continue;
+        for (int q = Yp[c]; q < Yp[c + 1]; ++q) {
+            if (id[Yi[q]] < 0) continue;
+            Bi.push_back(id[Yi[q]]);
+            Bx.push_back(Yx[2 * q]);
+        }
+        Bp[id[c] + 1] = (int)Bi.size();
+    }
+    std::vector<double> theta(m);
+    for (int b = 0; b < n; ++b)
+        if (id[b] >= 0) theta[id[b]] = Sbus[2 * b];
+    KluLikeLU lu;
+    int e = lu.analyze(m, Bp, Bi);
+    if (e == NoError) e = lu.factor(Bp, Bi, Bx);
+    if (e != NoError) return e == SingularMatrix ? SolverFactor : e;
+    if (lu.solve(theta) != NoError) return SolverSolve;
+    double tmax = 0.0;
+    bool finite = true;
+    for (double t : theta) {
+        finite = finite && std::isfinite(t);
+        tmax = std::max(tmax, std::fabs(t));
+    }
+    if (!finite || tmax >= 1e6) return SolverSolve;
+    const double va_s = std::atan2(V[2 * slack[0] + 1], V[2 * slack[0]]);
+    for (int b = 0; b < n; ++b) {
+        const double vm = std::hypot(V[2 * b], V[2 * b + 1]);
+        const double va = (id[b] >= 0 ? theta[id[b]] : 0.0) + va_s;
+        V[2 * b] = vm * std::cos(va);
+        V[2 * b + 1] = vm * std::sin(va);
+    }
+    return NoError;
+}
+
 // Contingency sweep (ContingencyAnalysis::compute loop). Each contingency c
 // subtracts its Ybus deltas dv[dptr[c]:dptr[c+1]] at Ybus value positions
 // dpos[...], checks connectivity, solves from Vinit and restores Ybus.
@@ -760,7 +816,7 @@ double oracle_time_series(int n, const int* Yp, const int* Yi, const double* Yx,
                           int T, const double* Sbuses, const int* slack_ids, int nslack, const double* slack_w,
                           const int* pv, int npv, const int* pq, int npq,
                           const double* Vinit, int single_slack, int max_iter, double tol, int nchains,
-                          double* Vout, int* iters, int* errs, int* convs, int* status_out) {
+                          int nthreads, double* Vout, int* iters, int* errs, int* convs, int* status_out) {
     auto t0 = std::chrono::steady_clock::now();
     if (nchains < 1) nchains = 1;
     CscC Y{n, Yp, Yi, reinterpret_cast<const cplx*>(Yx)};
@@ -791,13 +847,19 @@ double oracle_time_series(int n, const int* Yp, const int* Yi, const double* Yx,
     if (nchains == 1) {
         worker(0, T);
     } else {
-        std::vector<std::thread> th;
+        // chains of ceil(T / nchains) steps, handed out to nthreads threads (each chain keeps
+        // its own solver object: a fresh factorisation at its head, as the reference's threaded use)
         const int chunk = (T + nchains - 1) / nchains;
-        for (int c = 0; c < nchains; ++c) {
-            int s0 = c * chunk, s1 = std::min(T, s0 + chunk);
-            if (s0 >= s1) break;
-            th.emplace_back(worker, s0, s1);
-        }
+        const int nth = std::max(1, std::min(nthreads > 0 ? nthreads : nchains, nchains));
+        std::atomic<int> next(0);
+        std::vector<std::thread> th;
+        for (int k = 0; k < nth; ++k)
+            th.emplace_back([&]() {
+                for (int c = next++; c < nchains; c = next++) {
+                    const int s0 = c * chunk, s1 = std::min(T, s0 + chunk);
+                    if (s0 < s1) worker(s0, s1);
+                }
+            });
         for (auto& t : th) t.join();
     }
     const int ff = first_fail.load();

@@ -63,7 +63,10 @@ def lib():
         L.oracle_time_series.argtypes = [
             ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_int, _f64p, _i32p, ctypes.c_int, _f64p,
             _i32p, ctypes.c_int, _i32p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-            ctypes.c_int, _f64p, _i32p, _i32p, _i32p, _i32p]
+            ctypes.c_int, ctypes.c_int, _f64p, _i32p, _i32p, _i32p, _i32p]
+        L.oracle_dc.restype = ctypes.c_int
+        L.oracle_dc.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _i32p, ctypes.c_int, _i32p,
+                                ctypes.c_int, _i32p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -230,8 +233,9 @@ def contingency(Ybus, Sbus, slack_ids, slack_weights, pv, pq, Vinit, deltas, max
 
 
 def time_series(Ybus, Sbuses, slack_ids, slack_weights, pv, pq, Vinit, max_iter=10, tol=1e-8,
-                single_slack=False, nchains=1):
+                single_slack=False, nchains=1, nthreads=0):
     """Time series on the oracle (warm start from the previous step inside each
+    chain; nchains chains of ceil(T / nchains) steps on nthreads threads, 0: one per
     chain). Returns (V, iters, errs, conv, status, wall_seconds)."""
     Y = _csc(Ybus)
     n = Y.shape[0]
@@ -251,8 +255,24 @@ def time_series(Ybus, Sbuses, slack_ids, slack_weights, pv, pq, Vinit, max_iter=
     conv = np.zeros(T, np.int32)
     status = np.zeros(1, np.int32)
     wall = lib().oracle_time_series(n, pYp, pYi, pYx, T, pSb, psl, len(sl), pw, ppv, len(pv_), ppq, len(pq_),
-                                    pV0, int(single_slack), int(max_iter), float(tol), int(nchains),
+                                    pV0, int(single_slack), int(max_iter), float(tol), int(nchains), int(nthreads),
                                     Vout.ctypes.data_as(_f64p), iters.ctypes.data_as(_i32p),
                                     errs.ctypes.data_as(_i32p), conv.ctypes.data_as(_i32p),
                                     status.ctypes.data_as(_i32p))
     return Vout, iters, errs, conv.astype(bool), int(status[0]), wall
+
+
+def dc_pf(Ybus, V0, Sbus, slack_ids, pv, pq):
+    """DC power flow on the oracle (BaseDCAlgo<KLU>::compute_pf). Returns (V, error)."""
+    Y = _csc(Ybus)
+    n = Y.shape[0]
+    Yp, pYp = _i(Y.indptr)
+    Yi, pYi = _i(Y.indices)
+    Yx, pYx = _c(Y.data)
+    V = np.array(V0, dtype=np.complex128, copy=True)
+    S, pS = _c(Sbus)
+    sl, psl = _i(np.atleast_1d(slack_ids))
+    pv_, ppv = _i(pv)
+    pq_, ppq = _i(pq)
+    err = lib().oracle_dc(n, pYp, pYi, pYx, V.ctypes.data_as(_f64p), pS, psl, len(sl), ppv, len(pv_), ppq, len(pq_))
+    return V, int(err)

@@ -25,4 +25,4 @@ def test_library_exports_every_declared_symbol():
     for s in declared_symbols():
         assert hasattr(L, s), s
     assert set(declared_symbols()) == set(abi.SIGNATURES)
-    assert L.pfgpu_abi_version() == 1
+    assert L.pfgpu_abi_version() == 2

@@ -160,13 +160,15 @@ def test_contingency_analysis_cpp(oracle_mod, solver_type):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("chain", [8, 48])
-def test_time_series_cpp(oracle_mod, chain):
+@pytest.mark.parametrize("solver_type", ["AMDGPU", "AMDGPUMixed"])
+@pytest.mark.parametrize("chain", [0, 8])
+def test_time_series_cpp(oracle_mod, chain, solver_type):
+    """chain 0 (the default): the reference's single warm-started chain; 8: chains of 8 steps."""
     import lightsim2grid_amd as L
     from lightsim2grid_amd import gridgen
     g = gridgen.make("case118_synth")
     T = 48
-    _say("TimeSeriesCPP chain", chain)
+    _say("TimeSeriesCPP chain", chain, solver_type)
     Sb = gridgen.time_series_sbus(g, T, seed=3)
     gen_bus = np.r_[g.slack, g.pv]
     load_bus = g.pq
@@ -174,13 +176,17 @@ def test_time_series_cpp(oracle_mod, chain):
     load_p = -Sb[:, load_bus].real
     load_q = -Sb[:, load_bus].imag
     ts = L.TimeSeriesCPP(gridgen.to_gridmodel(g))
-    ts.set_chain_length(chain)
+    ts.change_solver(getattr(L.SolverType, solver_type))
+    assert ts.get_chain_length() == 0
+    if chain:
+        ts.set_chain_length(chain)
     st = ts.compute_Vs(gen_p, np.zeros((T, 0)), load_p, load_q, g.v0(), 10, 1e-8)
     assert st == 1 and ts.get_status() == 1
+    assert ts.nb_solved() == T
     Vg = ts.get_voltages()
     d = g.solver_inputs()
     Vr, itr, _, convr, status, _ = oracle_mod.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
-                                                          d["pq"], g.v0(), 10, 1e-8, False, T // chain)
+                                                          d["pq"], g.v0(), 10, 1e-8, False, T // chain if chain else 1)
     assert status == 1 and convr.all()
-    assert np.max(np.abs(Vg - Vr)) <= 1e-8
-    assert np.allclose(ts.get_sbuses(), Sb)
+    assert np.max(np.abs(Vg - Vr)) <= (1e-8 if solver_type == "AMDGPU" else 5e-8)
+    assert np.allclose(ts.get_sbuses(), Sb, rtol=1e-14, atol=1e-14)

@@ -126,3 +126,54 @@ def test_oracle_error_paths(oracle_mod):
     args[2] = S
     assert not s2.compute_pf(*args, 10, 1e-8)
     assert s2.get_error() == 3  # InifiniteValue
+
+
+def _dc_ybus(g):
+    """DC Ybus of a synthetic grid: line stamps 1/x (TwoSidesContainer_rxh_A.hpp:934-940), no shunts."""
+    y = 1.0 / g.x
+    rows = np.r_[g.f, g.t, g.f, g.t]
+    cols = np.r_[g.f, g.t, g.t, g.f]
+    vals = np.r_[y, y, -y, -y].astype(np.complex128)
+    Y = sp.csc_matrix((vals, (rows, cols)), shape=(g.n, g.n))
+    Y.sum_duplicates()
+    Y.sort_indices()
+    return Y
+
+
+@pytest.mark.parametrize("name", ["case14_synth", "case118_synth", "case2000_synth"])
+def test_oracle_dc_matches_reduced_solve(oracle_mod, name):
+    """DC power flow: theta solves Re(Ybus) theta = Re(Sbus) without the slack row / column (only the
+    first slack is removed under a distributed slack), shifted by the slack angle, |V| kept."""
+    import scipy.sparse.linalg as spla
+    from lightsim2grid_amd import gridgen
+    g = gridgen.make(name)
+    d = g.solver_inputs()
+    Y = _dc_ybus(g)
+    V0 = d["V0"] * np.exp(1j * 0.1)  # a non-zero slack angle
+    V, err = oracle_mod.dc_pf(Y, V0, d["Sbus"], d["slack_ids"], d["pv"], d["pq"])
+    assert err == 0
+    s0 = int(np.atleast_1d(d["slack_ids"])[0])
+    keep = np.array([b for b in range(g.n) if b != s0])
+    B = Y.real.tocsc()[keep][:, keep]
+    th = spla.spsolve(B.tocsc(), d["Sbus"].real[keep])
+    Va = np.zeros(g.n)
+    Va[keep] = th
+    Va += np.angle(V0[s0])
+    assert np.max(np.abs(V - np.abs(V0) * np.exp(1j * Va))) <= 1e-9  # angles may exceed pi: compare V
+    assert np.max(np.abs(np.abs(V) - np.abs(V0))) <= 1e-14
+
+
+def test_oracle_dc_singular_and_huge(oracle_mod):
+    from lightsim2grid_amd import gridgen
+    g = gridgen.make("case14_synth")
+    d = g.solver_inputs()
+    Y = _dc_ybus(g).tolil()
+    b = int(d["pq"][2])
+    Y[b, :] = 0.0
+    Y[:, b] = 0.0
+    Y = sp.csc_matrix(Y)
+    V, err = oracle_mod.dc_pf(Y, d["V0"], d["Sbus"], d["slack_ids"], d["pv"], d["pq"])
+    assert err == 5  # SolverFactor: singular B
+    Y2 = _dc_ybus(g) * 1e-12
+    V, err = oracle_mod.dc_pf(Y2, d["V0"], d["Sbus"], d["slack_ids"], d["pv"], d["pq"])
+    assert err == 7  # SolverSolve: |theta| >= 1e6

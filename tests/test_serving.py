@@ -93,3 +93,54 @@ def test_serving_rejects_oversized_request():
     r = sl.result(sl.submit_pf(gid, None), 60.0)
     assert r["status"][0] == 0
     sl.close()
+
+
+def test_serving_c_abi_submit_poll(oracle_mod):
+    """pfgpu_server_* / pfgpu_submit / pfgpu_poll through ctypes: results equal the oracle's,
+    a poll without waiting returns not-ready or the result, and an unknown id is an error."""
+    from lightsim2grid_amd import abi, gridgen
+    _say("serving C ABI")
+    g, d = _grid("case118_synth")
+    o = oracle_mod.OracleNR(False)
+    assert o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    Vb = o.get_V()
+    srv = abi.Server(0, 4096, abi.FP64)
+    gid = srv.add_grid(d["Ybus"], d["Sbus"], Vb, d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    nb = np.nonzero(~gridgen.bridges(g))[0][:30]
+    deltas = g.ybus_deltas([[int(k)] for k in nb])
+    ids = [srv.submit_contingencies(gid, *deltas), srv.submit_pf(gid), srv.submit_time_series(gid, gridgen.time_series_sbus(g, 5, seed=2))]
+    first = srv.poll(ids[0], 0.0)
+    res = [first if first is not None else srv.poll(ids[0], 60.0)] + [srv.poll(i, 60.0) for i in ids[1:]]
+    Vr, itr, _, convr, _, _ = oracle_mod.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"],
+                                                      d["pq"], Vb, deltas, 10, 1e-8, False, False, 2)
+    assert np.array_equal(res[0]["status"] == 0, convr)
+    assert np.max(np.abs(res[0]["V"][convr] - Vr[convr])) <= 1e-8
+    assert res[1]["status"][0] == 0 and np.max(np.abs(res[1]["V"][0] - Vb)) <= 1e-8
+    assert np.all(res[2]["status"] == 0)
+    with pytest.raises(abi.PfgpuError):
+        srv._rows[999999] = (1, g.n)
+        srv.poll(999999, 0.0)
+    srv.close()
+
+
+def test_serving_close_fails_pending_requests():
+    """close() with requests still queued: their results are errors, no caller waits forever, and a
+    retired or unknown id is an error."""
+    import lightsim2grid_amd as L
+    from lightsim2grid_amd import gridgen
+    _say("serving close with pending")
+    g, d = _grid("case2000_synth")
+    sl = L.lightsim2grid_amd_cpp.ServeLoop(0, 64, 0)
+    gid = sl.add_grid(d["Ybus"], d["Sbus"], d["V0"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    ids = [sl.submit_time_series(gid, gridgen.time_series_sbus(g, 64, seed=k)) for k in range(6)]
+    sl.close()
+    outcomes = []
+    for i in ids:
+        try:
+            outcomes.append(sl.result(i, 30.0) is not None)
+        except RuntimeError as e:
+            assert "closed" in str(e)
+            outcomes.append("closed")
+    assert len(outcomes) == len(ids)
+    with pytest.raises(RuntimeError):
+        sl.result(ids[0], 1.0)  # already retired
