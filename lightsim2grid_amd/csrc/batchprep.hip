@@ -103,15 +103,16 @@ __global__ void k_sbus_rows(int T, int ns, const int* __restrict__ gptr, const i
 }
 
 // The four stamps of branch k (pi model, line charging split in halves): ys = 1 / (r + j x)
-// as (r - j x) / (r^2 + x^2); yff = ytt = ys + h / 2, yft = ytf = -ys. No contraction: the host
-// mirror evaluates the same operations in the same order.
+// as (r - j x) / (r^2 + x^2); yff = ytt = ys + h / 2, yft = ytf = -ys. Rounded operation by
+// operation with no fused multiply-add, as the host mirror evaluates them.
 __device__ __forceinline__ void branch_stamp(double r, double x, double hr, double hi, int which, double& yr,
                                              double& yi) {
-    const double d = __dadd_rn(__dmul_rn(r, r), __dmul_rn(x, x));
-    const double sr = __ddiv_rn(r, d), si = __ddiv_rn(-x, d);
+#pragma clang fp contract(off)
+    const double d = r * r + x * x;
+    const double sr = r / d, si = -x / d;
     if (which < 2) {
-        yr = __dadd_rn(sr, __dmul_rn(0.5, hr));
-        yi = __dadd_rn(si, __dmul_rn(0.5, hi));
+        yr = sr + 0.5 * hr;
+        yi = si + 0.5 * hi;
     } else {
         yr = -sr;
         yi = -si;
@@ -131,8 +132,8 @@ __global__ void k_ybus_values(int nnz, const int* __restrict__ cptr, const int* 
             if (!on[k]) continue;
             double vr, vi;
             branch_stamp(r[k], x[k], h[2 * k], h[2 * k + 1], c & 3, vr, vi);
-            yr = __dadd_rn(yr, vr);
-            yi = __dadd_rn(yi, vi);
+            yr += vr;
+            yi += vi;
         }
         Yx[2 * e] = yr;
         Yx[2 * e + 1] = yi;

@@ -69,7 +69,8 @@ struct pfgpu_ctx {
     // host-buffer entry points: device copies of the caller's arrays (grow-only)
     void* stage = nullptr;
     size_t stage_bytes = 0;
-    long long last_rows = 0;  // rows of the last solve call (their flags are in aux)
+    long long last_rows = 0;            // rows of the last solve call
+    const int* last_flags = nullptr;    // and their flags (device)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
 };

@@ -44,6 +44,7 @@ std::string& pfgpu_engine::last_error() {
     return msg;
 }
 
+
 namespace {
 
 using pfgpu_engine::ApiError;
@@ -1646,10 +1647,23 @@ AuxLayout aux_layout(long long nrows, long long ninst) {
     return L;
 }
 
+// A workspace given to the call instead of allocated by it (the no-allocation entry point): the
+// per-call rows at aux, the kept initial voltages at io, the launch area at ws (ws_bytes).
+struct FixedScratch {
+    char* aux = nullptr;
+    char* io = nullptr;
+    char* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
 // One NR launch (k_init_v, k_gather_y for a shared Ybus, k_nr) over `nunits` launch units;
-// imap (device, or null) maps them to the caller's instances / chains.
+// imap (device, or null) maps them to the caller's instances / chains. With a fixed workspace the
+// launch allocates nothing: the schedule of its launch shape must already be on the device, and
+// the number of workgroups is capped to what the workspace holds (each workgroup takes groups of
+// instances from the work queue, so fewer workgroups only means less concurrency).
 void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precision, int G_req, int NT_req,
-                 int nunits, const int* imap, const AuxLayout& AL, bool fresh, pfgpu_stats* st) {
+                 int nunits, const int* imap, const AuxLayout& AL, bool fresh, pfgpu_stats* st,
+                 const FixedScratch* fx = nullptr) {
     const pfgpu::DevicePlan& P = gr->P;
     const int n = P.n;
     const int fsize = precision == PFGPU_FP64 ? 8 : 4;
@@ -1659,20 +1673,38 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
         NT_req = gr->tuned_NT[pslot];
     }
     LaunchChoice lc = choose_launch(ctx, P, nunits, fsize, G_req, NT_req);
-    ensure_streams(gr, lc.G, lc.NT, fsize, lc.lean);
+    if (fx) {
+        const int ix = shape_index(lc.G, lc.NT, fsize, lc.lean);
+        if ((int)gr->streams.size() <= ix || !gr->streams[ix])
+            throw ApiError(PFGPU_E_INVALID, "the schedule of this launch shape is not on the device yet: solve a batch "
+                                            "of this shape once with pfgpu_solve_batch_device first");
+    } else {
+        ensure_streams(gr, lc.G, lc.NT, fsize, lc.lean);
+    }
     const long long ngroups = ((long long)nunits + lc.G - 1) / lc.G;
-    const int nres = (int)std::min<long long>(ngroups, resident_blocks(ctx, fsize, lc.G, lc.NT, lc.lean, lc.lds));
+    int nres = (int)std::min<long long>(ngroups, resident_blocks(ctx, fsize, lc.G, lc.NT, lc.lean, lc.lds));
+    const bool gather = q.Yx_stride == 0;
+    const size_t gather_bytes = gather ? align_up((size_t)gr->ell_w * n * 16) + align_up(P.ye_pos.size() * 16) : 0;
+    if (fx) {
+        // workgroups whose rows fit the fixed workspace (4 * 256 bytes covers the roundings)
+        const size_t per_block = (size_t)lc.G * ((size_t)n * 16 + (size_t)std::max(P.nyd, 1) * 16) +
+                                 (size_t)lc.nspill * lc.G * fsize;
+        const size_t room = fx->ws_bytes > gather_bytes + 1024 ? fx->ws_bytes - gather_bytes - 1024 : 0;
+        const long long cap = per_block ? (long long)(room / per_block) : nres;
+        if (cap < 1) throw ApiError(PFGPU_E_BUDGET, "the grid's per-workgroup rows do not fit the fixed scratch");
+        nres = (int)std::min<long long>(nres, cap);
+    }
     const long long slots = (long long)nres * lc.G;  // workspace rows: one per resident instance
     size_t off = 0;
     const size_t o_va = off; off += align_up((size_t)slots * n * 8);
     const size_t o_vm = off; off += align_up((size_t)slots * n * 8);
     const size_t o_ib = off; off += align_up((size_t)slots * std::max(P.nyd, 1) * 16);
     const size_t o_x = off; off += align_up((size_t)nres * lc.nspill * lc.G * fsize);
-    const bool gather = q.Yx_stride == 0;
     const size_t o_yell = off; off += gather ? align_up((size_t)gr->ell_w * n * 16) : 0;
     const size_t o_yye = off; off += gather ? align_up(P.ye_pos.size() * 16) : 0;
-    char* ws = static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
-    char* aux = static_cast<char*>(ctx->aux);
+    if (fx && off > fx->ws_bytes) throw ApiError(PFGPU_E_INTERNAL, "fixed scratch layout overflow");
+    char* ws = fx ? fx->ws : static_cast<char*>(ws_get(ctx, std::max<size_t>(off, 256)));
+    char* aux = fx ? fx->aux : static_cast<char*>(ctx->aux);
     hipStream_t stream = ctx->stream;
     int* queue = reinterpret_cast<int*>(aux + AL.o_queue);
     unsigned long long* phase = st ? reinterpret_cast<unsigned long long*>(aux + AL.o_phase) : nullptr;
@@ -1755,7 +1787,8 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
 // Core solve on device buffers: the launch in the requested form; with FP32 factors (unless
 // strict), the rows that stalled or broke down are re-solved from their initial voltages with
 // FP64 factors (chains from their heads).
-void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_stats* st) {
+void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_stats* st,
+                  const FixedScratch* fx = nullptr) {
     const pfgpu::DevicePlan& P = gr->P;
     const int n = P.n;
     if (q0.ninst <= 0 || q0.nrows <= 0) return;
@@ -1771,21 +1804,24 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     if (q.dptr && q.Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
     if (gr->dc && q.precision != PFGPU_FP64) throw ApiError(PFGPU_E_INVALID, "a DC grid is solved with FP64 factors");
     const AuxLayout AL = aux_layout(q.nrows, q.ninst);
-    char* aux = static_cast<char*>(grow_buffer(ctx->aux, ctx->aux_bytes, AL.bytes));
+    char* aux = fx ? fx->aux : static_cast<char*>(grow_buffer(ctx->aux, ctx->aux_bytes, AL.bytes));
     const bool fp32 = q.precision != PFGPU_FP64;
     const bool fallback = q.precision == PFGPU_FP32;
     // in place with an FP64 re-solve possible: keep the initial voltages
     const double* V0_keep = q.V0;
     if (fallback && q.V0 == q.V && q.v0_stride != 0) {
         const size_t bytes = (size_t)q.ninst * n * 16;
-        double* keep = static_cast<double*>(grow_buffer(ctx->io, ctx->io_bytes, bytes));
+        double* keep = fx ? reinterpret_cast<double*>(fx->io)
+                          : static_cast<double*>(grow_buffer(ctx->io, ctx->io_bytes, bytes));
+        if (!keep) throw ApiError(PFGPU_E_INTERNAL, "no room kept for the initial voltages");
         HIP_OK(hipMemcpyAsync(keep, q.V0, bytes, hipMemcpyDeviceToDevice, ctx->stream));
         V0_keep = keep;
     }
     const bool fresh = gr->fresh;
-    launch_pass(ctx, gr, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st);
+    launch_pass(ctx, gr, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st, fx);
     gr->fresh = false;
     ctx->last_rows = q.nrows;
+    ctx->last_flags = reinterpret_cast<const int*>(aux + AL.o_flags);
     if (!fallback) return;
     int* list = reinterpret_cast<int*>(aux + AL.o_list);
     int* count = reinterpret_cast<int*>(aux + AL.o_count);
@@ -1800,7 +1836,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     r.V0 = V0_keep;
     r.Jx_last = nullptr;
     pfgpu_stats s2{};
-    launch_pass(ctx, gr, r, PFGPU_FP64, 0, 0, nfb, list, AL, fresh, st ? &s2 : nullptr);
+    launch_pass(ctx, gr, r, PFGPU_FP64, 0, 0, nfb, list, AL, fresh, st ? &s2 : nullptr, fx);
     if (st) {
         st->fallback_fp64 = nfb;
         st->kernel_s += s2.kernel_s;
@@ -2162,8 +2198,7 @@ int pfgpu_ctx_last_flags(pfgpu_ctx* ctx, int* flags, long long nrows) {
         HIP_OK(hipSetDevice(ctx->device));
         if (nrows > ctx->last_rows) throw ApiError(PFGPU_E_INVALID, "more rows than the last solve call had");
         if (nrows == 0) return 0;
-        const AuxLayout AL = aux_layout(1, 1);
-        HIP_OK(hipMemcpy(flags, static_cast<char*>(ctx->aux) + AL.o_flags, (size_t)nrows * 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(flags, ctx->last_flags, (size_t)nrows * 4, hipMemcpyDeviceToHost));
         return 0;
     });
 }
