@@ -177,6 +177,25 @@ int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst,
                              int* d_iters, int* d_status, pfgpu_stats* stats);
 
 /*
+ * pfgpu_solve_batch_device without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments and results; the workspace is a 64 MiB device
+ * array the library holds per GPU until the process ends (per-call rows: 8 bytes per instance,
+ * plus 16 n per instance for an in-place FP32 call; the rest bounds how many workgroups run at
+ * once, not the batch). A batch whose rows do not fit fails with PFGPU_E_BUDGET. The launch
+ * shape's schedule must already be on the device: one pfgpu_solve_batch_device (or autotune) call
+ * with a batch of the same shape puts it there, else PFGPU_E_INVALID. Calls on one GPU share the
+ * array: the library runs them one at a time within the process (calls that overlapped on one
+ * GPU would race on it); each call keeps the array's last contents and clears what must start at
+ * zero itself.
+ */
+int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
+                                     const int* d_dptr, const int* d_dpos, const double* d_dval,
+                                     const double* d_Sbus, int sbus_stride, const double* d_V0, int v0_stride,
+                                     double* d_V, int max_iter, double tol, int precision, int instances_per_wg,
+                                     int* d_iters, int* d_status, pfgpu_stats* stats);
+
+/*
  * Time series (TimeSeries::compute_Vs, TimeSeries.cpp:80-103): nrows steps, each with its own
  * Sbus row, cut into chains of chain_len consecutive steps (chain_len >= nrows: the reference's
  * single chain). The head of chain c starts from V0 (row c if v0_stride == n, the shared row if

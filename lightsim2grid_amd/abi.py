@@ -83,6 +83,10 @@ SIGNATURES = {
     "pfgpu_solve_batch_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                                 ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.POINTER(Stats)]),
+    "pfgpu_solve_batch_device_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp,
+                                                        _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int,
+                                                        ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                                        ctypes.POINTER(Stats)]),
     "pfgpu_solve_chains": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p, ctypes.c_int,
                                           _f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p,
                                           _i32p, ctypes.POINTER(Stats)]),
@@ -395,11 +399,13 @@ class Grid:
         return bG.value, bNT.value, bt.value
 
     def solve_device(self, d_Yx, yx_stride, d_S, s_stride, d_V0, v0_stride, d_V, ninst, d_iters, d_status,
-                     max_iter=10, tol=1e-8, precision=FP64, deltas=None, instances_per_wg=0):
-        """Batched solve on device-resident arrays (DeviceArray); returns the stats dict."""
+                     max_iter=10, tol=1e-8, precision=FP64, deltas=None, instances_per_wg=0, noalloc=False):
+        """Batched solve on device-resident arrays (DeviceArray); returns the stats dict.
+        noalloc: the entry point that allocates nothing (pfgpu_solve_batch_device_noalloc)."""
         st = Stats()
         dp, dq, dv = deltas if deltas is not None else (None, None, None)
-        _check(self.ctx._L.pfgpu_solve_batch_device(
+        fn = self.ctx._L.pfgpu_solve_batch_device_noalloc if noalloc else self.ctx._L.pfgpu_solve_batch_device
+        _check(fn(
             self.ctx.h, self.h, int(ninst), d_Yx.ptr, int(yx_stride), dp.ptr if dp else None, dq.ptr if dq else None,
             dv.ptr if dv else None, d_S.ptr, int(s_stride), d_V0.ptr, int(v0_stride), d_V.ptr, int(max_iter),
             float(tol), int(precision), int(instances_per_wg), d_iters.ptr, d_status.ptr, ctypes.byref(st)))

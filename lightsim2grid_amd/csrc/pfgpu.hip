@@ -44,6 +44,35 @@ std::string& pfgpu_engine::last_error() {
     return msg;
 }
 
+// Scratch of the no-allocation entry point (pfgpu_solve_batch_device_noalloc): a device array that
+// every GPU the library runs on holds until the process ends. Its name is not a device address in
+// host code: the address is looked up once per device when the device's first context is created
+// (scratch_init) and kept per device; host code uses only that address.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+std::mutex g_noalloc_mu[kMaxDevices];  // no-allocation calls on one device run one at a time
+
+// With `device` current: the address of this device's copy of the scratch array.
+void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    g_scratch_done[device] = true;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    else
+        g_scratch_ptr[device] = p;
+}
+}  // namespace
 
 namespace {
 
@@ -1865,6 +1894,7 @@ int pfgpu_ctx_create(int device, size_t hbm_budget_bytes, pfgpu_ctx** out) {
         HIP_OK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) throw ApiError(PFGPU_E_INVALID, "no such device");
         HIP_OK(hipSetDevice(device));
+        scratch_init(device);
         auto c = std::make_unique<pfgpu_ctx>();
         c->device = device;
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -2232,6 +2262,49 @@ int pfgpu_solve_batch_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const 
         q.max_iter = max_iter; q.tol = tol; q.precision = precision; q.G_req = instances_per_wg;
         q.iters = d_iters; q.status = d_status;
         solve_device(ctx, grid, q, stats);
+        if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    });
+}
+
+int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const double* d_Yx, int Yx_stride,
+                                     const int* d_dptr, const int* d_dpos, const double* d_dval,
+                                     const double* d_Sbus, int sbus_stride, const double* d_V0, int v0_stride,
+                                     double* d_V, int max_iter, double tol, int precision, int instances_per_wg,
+                                     int* d_iters, int* d_status, pfgpu_stats* stats) {
+    return guarded([&]() -> int {
+        if (!ctx || !grid || !d_Yx || !d_Sbus || !d_V0 || !d_V || !d_iters || !d_status || ninst < 0)
+            throw ApiError(PFGPU_E_INVALID, "invalid solve arguments");
+        if ((d_dptr == nullptr) != (d_dpos == nullptr) || (d_dptr == nullptr) != (d_dval == nullptr))
+            throw ApiError(PFGPU_E_INVALID, "dptr/dpos/dval must be all set or all NULL");
+        if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch_ptr[ctx->device])
+            throw ApiError(PFGPU_E_DEVICE, "no scratch address for this device" +
+                                               (ctx->device >= 0 && ctx->device < kMaxDevices
+                                                    ? ": " + g_scratch_err[ctx->device] : std::string()));
+        std::lock_guard<std::mutex> lk_dev(g_noalloc_mu[ctx->device]);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        if (stats) *stats = pfgpu_stats{};
+        if (ninst == 0) return 0;
+        SolveArgs q;
+        q.ninst = ninst; q.nrows = ninst; q.Yx = d_Yx; q.Yx_stride = Yx_stride; q.dptr = d_dptr; q.dpos = d_dpos;
+        q.dval = d_dval; q.S = d_Sbus; q.sbus_stride = sbus_stride; q.V0 = d_V0; q.v0_stride = v0_stride; q.V = d_V;
+        q.max_iter = max_iter; q.tol = tol; q.precision = precision; q.G_req = instances_per_wg;
+        q.iters = d_iters; q.status = d_status;
+        // scratch layout: per-call rows, the kept initial voltages (in-place FP32 only), the launch area
+        const AuxLayout AL = aux_layout(ninst, ninst);
+        const bool keep = precision == PFGPU_FP32 && d_V0 == d_V && v0_stride != 0;
+        const size_t io_bytes = keep ? align_up((size_t)ninst * grid->P.n * 16) : 0;
+        if (AL.bytes + io_bytes + 4096 > kScratchBytes)
+            throw ApiError(PFGPU_E_BUDGET, "the batch does not fit the 64 MiB no-allocation scratch: fewer instances");
+        char* base = static_cast<char*>(g_scratch_ptr[ctx->device]);
+        FixedScratch fx;
+        fx.aux = base;
+        fx.io = keep ? base + AL.bytes : nullptr;
+        fx.ws = base + AL.bytes + io_bytes;
+        fx.ws_bytes = kScratchBytes - AL.bytes - io_bytes;
+        solve_device(ctx, grid, q, stats, &fx);
         if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return 0;
     });
