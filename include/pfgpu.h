@@ -49,16 +49,17 @@ extern "C" {
 
 /* precision of the linear solve (factor + triangular solves); the mismatch,
  * the Jacobian entries, the voltage update and the convergence test are FP64.
- * PFGPU_FP32: rows that do not converge with FP32 factors and stalled (their mismatch went
- * below sqrt(tol) on the way), lost a pivot to cancellation, met a zero / non-finite pivot or a
- * non-finite mismatch are re-solved with FP64 factors from their initial voltages (chains from
- * their heads); pfgpu_stats.fallback_fp64 counts them. PFGPU_FP32_STRICT: no re-solve. */
+ * PFGPU_FP32: rows that do not converge with FP32 factors and stalled (their last mismatch is
+ * below sqrt(tol): the inexact FP32 steps, not the Newton iteration, stopped the progress), or met
+ * a zero / non-finite pivot or a non-finite mismatch, are re-solved with FP64 factors from their
+ * initial voltages (chains from their heads); rows that diverged are not. pfgpu_stats.fallback_fp64
+ * counts them. PFGPU_FP32_STRICT: no re-solve. */
 #define PFGPU_FP64 0
 #define PFGPU_FP32 1
 #define PFGPU_FP32_STRICT 2
 
 /* per-row flags (pfgpu_ctx_last_flags) */
-#define PFGPU_FLAG_STALL 1   /* not converged, mismatch went below sqrt(tol) on the way */
+#define PFGPU_FLAG_STALL 1   /* not converged, last mismatch below sqrt(tol) */
 #define PFGPU_FLAG_GROWTH 2  /* a pivot kept less than 2^-10 (FP32) / 1e-10 (FP64) of its value */
 
 /* status of a chain step after the chain's first failure (not solved): NotInitError */

@@ -90,14 +90,14 @@ typedef __attribute__((address_space(3))) unsigned char lds_u8;
 
 // Per-workgroup state of the NR kernel, in dynamic LDS after the value slots and the level
 // table. The kernel declares no static LDS, so its value slots start at LDS address 0.
-// Layout: part[kWaves][G], slack[G], fbest[G], dval[G][kMaxEdits][2] (doubles), then done,
+// Layout: part[kWaves][G], slack[G], flast[G], dval[G][kMaxEdits][2] (doubles), then done,
 // it, status, piv, grow [G], dpos[G][kMaxEdits], nd, d0 [G], ndone, grp (ints).
 __host__ __device__ constexpr int wg_state_bytes(int G, int NT) {
     return 8 * G * (NT / 64 + 2 + 2 * kMaxEdits) + 4 * (G * (7 + kMaxEdits) + 2);
 }
 
 // Per-row diagnostic flags written by the NR kernel (pfgpu_stats / the fallback passes):
-constexpr int kFlagStall = 1;   // not converged, but the mismatch fell below the stall level on the way
+constexpr int kFlagStall = 1;   // not converged, but the last mismatch is below the stall level
 constexpr int kFlagGrowth = 2;  // a pivot lost more than the growth threshold to cancellation
 constexpr int kStatusNotRun = 8;  // chain step after the chain's first failure (NotInitError)
 
@@ -164,7 +164,7 @@ struct KArgs {
     int chain;
     long long nrows;     // rows of V / iters / status (ninst * chain, the last chain may be short)
     int fresh;           // 1: the first iteration factors a fresh solver (pivot failure -> SolverFactor)
-    double stall_tol;    // kFlagStall: the best mismatch of a failed solve went below this
+    double stall_tol;    // kFlagStall: the last mismatch of a failed solve is below this
     double growth_tau;   // kFlagGrowth: |pivot| < growth_tau * |its value before elimination|
     int* flags;          // per row (null: not written)
     unsigned long long* phase_t;  // workgroup 0's wall clock per phase (null: not measured)
@@ -361,8 +361,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
     lds_u8* const shb = (lds_u8*)lds_raw + a.sh_off;
     lds_dbl* const s_part = (lds_dbl*)shb;  // [kWaves][G]
     lds_dbl* const s_slack = s_part + kWaves * G;
-    lds_dbl* const s_fbest = s_slack + G;  // best mismatch of the current solve
-    lds_dbl* const s_dval = s_fbest + G;  // [G][kMaxEdits][2]: this instance's Ybus edits
+    lds_dbl* const s_flast = s_slack + G;  // mismatch of the current solve's last iteration
+    lds_dbl* const s_dval = s_flast + G;  // [G][kMaxEdits][2]: this instance's Ybus edits
     lds_int* const s_done = (lds_int*)(s_dval + G * kMaxEdits * 2);
     lds_int* const s_it = s_done + G;
     lds_int* const s_status = s_it + G;
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             s_status[g] = 0;
             s_piv[g] = 0;
             s_grow[g] = 0;
-            s_fbest[g] = INFINITY;
+            s_flast[g] = INFINITY;
             s_done[g] = run ? 0 : 1;
         }
         __syncthreads();
@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
 
     double fmx = mismatch();
     if (s == 0 && run && !s_done[g] && !a.dc) {  // DC: always one linear solve
-        s_fbest[g] = fmx;
+        s_flast[g] = fmx;
         if (fmx < a.tol) s_done[g] = 1;  // converged at iteration 0
         else if (a.max_iter <= 0) { s_status[g] = 2; s_done[g] = 1; }
     }
@@ -1237,7 +1237,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             s_status[g] = fmx < 1e6 ? 0 : 7;  // max |theta| (inf if not finite)
             s_done[g] = 1;
         } else if (s == 0 && act) {
-            s_fbest[g] = fmin(s_fbest[g], fmx);
+            s_flast[g] = fmx;
             if (!(fmx <= 1.7976931348623157e308)) { s_status[g] = 3; s_done[g] = 1; }
             else if (fmx < a.tol) { s_status[g] = 0; s_done[g] = 1; }
             else if (s_it[g] >= a.max_iter) { s_status[g] = 2; s_done[g] = 1; }
@@ -1257,7 +1257,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             a.iters[row] = s_it[g];
             a.status[row] = stt;
             if (a.flags)
-                a.flags[row] = (stt != 0 && s_fbest[g] < a.stall_tol ? kFlagStall : 0) | (s_grow[g] ? kFlagGrowth : 0);
+                a.flags[row] = (stt != 0 && s_flast[g] < a.stall_tol ? kFlagStall : 0) | (s_grow[g] ? kFlagGrowth : 0);
         }
         alive = s_status[g] == 0;
     } else if (valid && row < a.nrows) {  // a step after its chain's failure
@@ -1314,8 +1314,9 @@ __global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n
 }
 
 // Compaction of the rows to re-solve in FP64 after an FP32-factor launch, in row order, by one
-// workgroup of 1024 lanes: rows whose status is not 0 (and not a chain's not-run step) and that
-// stalled, lost a pivot to cancellation, met a zero / non-finite pivot or a non-finite mismatch.
+// workgroup of 1024 lanes: rows that did not converge and stalled (their last mismatch is below
+// the stall level: the FP32 steps, not the Newton iteration, stopped the progress), or met a zero
+// / non-finite pivot or a non-finite mismatch. A row that diverged is not re-solved.
 // Chain mode takes whole chains (a chain is re-run from its head). Writes the list and its
 // length.
 __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, const int* flags, long long nrows,
@@ -1335,7 +1336,7 @@ __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, co
                 if (r >= nrows) break;
                 const int st = status[r];
                 if (st == 0 || st == kStatusNotRun) continue;
-                take = st == 3 || st == 5 || st == 6 || (flags[r] & (kFlagStall | kFlagGrowth)) != 0;
+                take = st == 3 || st == 5 || st == 6 || (flags[r] & kFlagStall) != 0;
                 break;  // a chain's first failure decides
             }
         const unsigned long long m = __ballot(take);

@@ -36,7 +36,7 @@ def _n1(grid_name, max_cont=None, seed=0):
 
 def test_fp32_stall_falls_back_to_fp64(gpu_ctx, oracle_mod):
     """Rows whose FP32-factor solve needs one iteration more than FP64, run with max_iter = the
-    FP64 count: the FP32 launch flags them as stalled (mismatch below sqrt(tol) on the way), the
+    FP64 count: the FP32 launch flags them as stalled (last mismatch below sqrt(tol)), the
     FP64 pass re-solves them, and the statuses, iterations and voltages equal the oracle's."""
     from lightsim2grid_amd import abi
     found = None
