@@ -98,7 +98,14 @@ typedef struct pfgpu_stats {
     double phase_s[6];
     int block_size;           /* lanes per workgroup */
     int resident_workgroups;  /* workgroups launched (each loops over instance groups) */
+    int graph_launches;       /* launch sequences replayed from a captured HIP graph (see below) */
 } pfgpu_stats;
+
+/* Launch sequences: every solve call issues k_init_v, k_gather_y (shared Ybus) and the NR kernel on
+ * the context's stream. A grid keeps the last 8 distinct sequences (by their exact arguments: device
+ * pointers, sizes, launch shape) as instantiated HIP graphs: a sequence issued a second time is
+ * captured, and later identical calls replay the graph (one launch instead of three argument
+ * setups). PFGPU_GRAPH=0 in the environment turns this off. */
 
 /* Thread-local message of the last failing call. */
 const char* pfgpu_last_error(void);

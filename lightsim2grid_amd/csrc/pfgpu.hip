@@ -27,6 +27,7 @@
 #include <climits>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -80,6 +81,7 @@ using pfgpu_engine::ApiError;
 using pfgpu_engine::guarded;
 
 constexpr int kMaxBlock = 1024;
+constexpr int kMaxGraphs = 8;  // captured launch sequences kept per grid
 constexpr int kMaxEdits = 8;  // Ybus edits per instance kept in LDS
 constexpr int kMaxWaves = kMaxBlock / 64;
 constexpr int kMaxG = 32;
@@ -1432,14 +1434,32 @@ struct pfgpu_grid {
     // initialize, KLUSolver.cpp:24-45); cleared by a solve, set by load and pfgpu_grid_reset
     bool fresh = true;
     bool dc = false;  // a DC grid (pfgpu_grid_load_dc): one linear solve per instance
+    // Launch sequences (k_init_v, k_gather_y, k_nr) captured as HIP graphs, by the exact launch
+    // arguments: a sequence seen a second time is captured, and replayed from then on.
+    struct GraphEntry {
+        std::vector<unsigned char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    std::vector<GraphEntry> graphs;                 // at most kMaxGraphs, least recently used replaced
+    std::vector<std::vector<unsigned char>> seen;   // sequences launched once, not captured yet
+    unsigned long long clock = 0;
+    ~pfgpu_grid() {
+        for (auto& e : graphs) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+        }
+    }
 };
 
 namespace {
 
+// (the dynamic LDS limit of the variant is raised by resident_blocks before every launch, so that
+// a launch under stream capture issues nothing but the launch)
 template <typename FT, int G, int NT, bool LEAN>
 void launch_nr(const KArgs& a, int nblocks, size_t lds, hipStream_t st) {
     auto kfn = k_nr<FT, G, NT, LEAN>;
-    if (lds > 65536) HIP_OK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(NT), lds, st, a);
     HIP_OK(hipGetLastError());
 }
@@ -1740,11 +1760,8 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     unsigned long long* phase = st ? reinterpret_cast<unsigned long long*>(aux + AL.o_phase) : nullptr;
     // in place (V0 == V, one row per instance): nothing to copy, rows are read where they are
     const bool copy = !(q.V0 == q.V && q.v0_stride != 0 && q.chain == 1);
-    HIP_OK(hipEventRecord(ctx->ev0, stream));
-    hipLaunchKernelGGL(k_init_v, dim3(copy ? 1024 : 1), dim3(256), 0, stream, q.V0, (long long)q.v0_stride * 2, q.V, n,
-                       nunits, imap, q.chain, queue, phase, copy ? 1 : 0);
-    HIP_OK(hipGetLastError());
-    KArgs a{};
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));  // padding too: the struct's bytes key the graph cache
     a.n = n; a.N = P.N; a.nJ = P.nJ; a.nzero = (int)P.zero_slots.size(); a.nlev = P.nlev; a.nblev = P.nblev;
     a.m = P.m;
     if (lc.nlds < P.m * P.m + P.m) throw ApiError(PFGPU_E_INTERNAL, "dense tail does not fit in LDS");
@@ -1789,14 +1806,82 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     a.dc = gr->dc ? 1 : 0;
     a.slack0 = P.slack0;
     if (gr->dc) a.max_iter = 1;
-    if (gather) {
-        const int nell = gr->ell_w * n, nye = (int)P.ye_pos.size();
-        hipLaunchKernelGGL(k_gather_y, dim3((nell + nye + 255) / 256), dim3(256), 0, stream, q.Yx, a.yr_ell, nell,
-                           a.ye_pos, nye, reinterpret_cast<double*>(ws + o_yell), reinterpret_cast<double*>(ws + o_yye));
+    const int nell = gr->ell_w * n, nye = (int)P.ye_pos.size();
+    auto issue = [&]() {
+        hipLaunchKernelGGL(k_init_v, dim3(copy ? 1024 : 1), dim3(256), 0, stream, q.V0, (long long)q.v0_stride * 2, q.V,
+                           n, nunits, imap, q.chain, queue, phase, copy ? 1 : 0);
         HIP_OK(hipGetLastError());
+        if (gather) {
+            hipLaunchKernelGGL(k_gather_y, dim3((nell + nye + 255) / 256), dim3(256), 0, stream, q.Yx, a.yr_ell, nell,
+                               a.ye_pos, nye, reinterpret_cast<double*>(ws + o_yell),
+                               reinterpret_cast<double*>(ws + o_yye));
+            HIP_OK(hipGetLastError());
+        }
+        if (fsize == 4) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
+        else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
+    };
+    // the launch sequence's key: every argument of the three launches
+    struct Extra {
+        const double* V0;
+        long long v0s;
+        int copy, nres, G, NT, lean, fsize, gather;
+        size_t lds;
+    } ex;
+    std::memset(&ex, 0, sizeof(ex));
+    ex.V0 = q.V0; ex.v0s = (long long)q.v0_stride * 2; ex.copy = copy; ex.nres = nres; ex.G = lc.G; ex.NT = lc.NT;
+    ex.lean = lc.lean; ex.fsize = fsize; ex.gather = gather; ex.lds = lc.lds;
+    std::vector<unsigned char> key(sizeof(KArgs) + sizeof(Extra));
+    std::memcpy(key.data(), &a, sizeof(KArgs));
+    std::memcpy(key.data() + sizeof(KArgs), &ex, sizeof(Extra));
+    static const bool graphs_on = [] {
+        const char* e = std::getenv("PFGPU_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    bool replayed = false;
+    HIP_OK(hipEventRecord(ctx->ev0, stream));
+    if (graphs_on && !fx) {  // capturing allocates: not for the fixed-workspace entry point
+        pfgpu_grid::GraphEntry* hit = nullptr;
+        for (auto& e : gr->graphs)
+            if (e.key == key) hit = &e;
+        if (!hit && std::find(gr->seen.begin(), gr->seen.end(), key) != gr->seen.end()) {
+            // second launch of this sequence: capture it
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                issue();
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(stream, &g));
+            hipGraphExec_t exg = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&exg, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                throw ApiError(PFGPU_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            }
+            if ((int)gr->graphs.size() >= kMaxGraphs) {  // replace the least recently used
+                auto lru = std::min_element(gr->graphs.begin(), gr->graphs.end(),
+                                            [](const auto& x, const auto& y) { return x.used < y.used; });
+                (void)hipGraphExecDestroy(lru->exec);
+                (void)hipGraphDestroy(lru->graph);
+                gr->graphs.erase(lru);
+            }
+            gr->graphs.push_back({key, g, exg, 0});
+            hit = &gr->graphs.back();
+            gr->seen.erase(std::find(gr->seen.begin(), gr->seen.end(), key));
+        }
+        if (hit) {
+            hit->used = ++gr->clock;
+            HIP_OK(hipGraphLaunch(hit->exec, stream));
+            replayed = true;
+        } else {
+            if (gr->seen.size() >= 2 * kMaxGraphs) gr->seen.erase(gr->seen.begin());
+            gr->seen.push_back(key);
+        }
     }
-    if (fsize == 4) launch_nr_any<float>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
-    else launch_nr_any<double>(lc.G, lc.NT, lc.lean, a, nres, lc.lds, stream);
+    if (!replayed) issue();
     HIP_OK(hipEventRecord(ctx->ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     if (st) {
@@ -1808,6 +1893,7 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
         st->lds_slots = lc.nlds;
         st->workgroups = (int)std::min<long long>(ngroups, INT_MAX);
         st->resident_workgroups = nres;
+        st->graph_launches += replayed ? 1 : 0;
         unsigned long long ph[8];
         HIP_OK(hipMemcpy(ph, phase, sizeof(ph), hipMemcpyDeviceToHost));
         for (int k = 0; k < 6; ++k) st->phase_s[k] += (double)ph[k] / ctx->wall_hz;

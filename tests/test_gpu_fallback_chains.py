@@ -238,3 +238,39 @@ def test_timers_jacobian_of_host_solver(oracle_mod):
     fx, solve, refactor, init, check, dsbus, fillj, vavm, pre, total = t
     assert fx > 0 and solve > 0 and refactor > 0 and fillj > 0 and vavm > 0 and total > 0
     assert fx + solve + refactor + fillj + vavm <= total
+
+
+def test_graph_replays_equal_eager_launches(gpu_ctx):
+    """The launch sequence of a repeated call is captured as a HIP graph on its second call and
+    replayed afterwards: every replay, with other (eager) solves and synchronisations between the
+    replays, gives the eager result bit for bit."""
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make("case118_synth")
+    d = g.solver_inputs()
+    _say("graph replays")
+    nb = np.nonzero(~gridgen.bridges(g))[0][:64]
+    deltas = g.ybus_deltas([[int(k)] for k in nb])
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    n, m = g.n, len(nb)
+    dY, dS, dV0 = gpu_ctx.to_device(Yx), gpu_ctx.to_device(np.ascontiguousarray(d["Sbus"])), gpu_ctx.to_device(d["V0"])
+    dV = gpu_ctx.empty((m, n), np.complex128)
+    dIt, dSt = gpu_ctx.empty(m, np.int32), gpu_ctx.empty(m, np.int32)
+    dd = tuple(gpu_ctx.to_device(x) for x in deltas)
+    Sb = gridgen.time_series_sbus(g, 12, seed=9)
+    for prec in (FP64, FP32):
+        ref = None
+        replays = 0
+        for rep in range(6):
+            dV.copy_from(np.zeros((m, n), np.complex128))
+            st = gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, m, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
+            out = (dV.to_host(), dIt.to_host(), dSt.to_host())
+            replays += st["graph_launches"]
+            if ref is None:
+                assert st["graph_launches"] == 0  # the first call runs eagerly
+                ref = out
+            else:
+                assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]) and np.array_equal(out[2], ref[2])
+            # eager work of another shape between the replays
+            gr.solve_chains(Yx, Sb[: 3 + rep], d["V0"], chain_len=None, precision=prec)
+        assert replays >= 4
