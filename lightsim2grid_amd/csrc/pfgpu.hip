@@ -1952,7 +1952,8 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     r.V0 = V0_keep;
     r.Jx_last = nullptr;
     pfgpu_stats s2{};
-    launch_pass(ctx, gr, r, PFGPU_FP64, 0, 0, nfb, list, AL, fresh, st ? &s2 : nullptr, fx);
+    // the caller's launch shape when it fixed one (results independent of which rows re-solve)
+    launch_pass(ctx, gr, r, PFGPU_FP64, q.G_req, q.NT_req, nfb, list, AL, fresh, st ? &s2 : nullptr, fx);
     if (st) {
         st->fallback_fp64 = nfb;
         st->kernel_s += s2.kernel_s;

@@ -271,9 +271,11 @@ void ServeLoop::run(int gi) {
                 }
             }
             pfgpu_stats st{};
+            // one instance per workgroup whatever the launch size: the lane schedule, hence the
+            // rounding, of a request does not depend on what else rides in its launch
             check_rc(pfgpu_solve_batch_device(g.ctx, g.grid, M, g.dY, 0, any_delta ? g.ddp : nullptr,
                                               any_delta ? g.ddq : nullptr, any_delta ? g.ddv : nullptr, g.dS, n, g.dV0,
-                                              0, g.dV, g.max_iter, g.tol, precision_, 0, g.dIt, g.dSt, &st),
+                                              0, g.dV, g.max_iter, g.tol, precision_, 1, g.dIt, g.dSt, &st),
                      "pfgpu_solve_batch_device");
             CplxVect Vall((size_t)M * n);
             IntVect it(M), stt(M);
