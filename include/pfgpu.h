@@ -99,6 +99,7 @@ typedef struct pfgpu_stats {
     int block_size;           /* lanes per workgroup */
     int resident_workgroups;  /* workgroups launched (each loops over instance groups) */
     int graph_launches;       /* launch sequences replayed from a captured HIP graph (see below) */
+    int repivot_rows;         /* rows (or chains) re-solved on pivots chosen on the numbers (see below) */
 } pfgpu_stats;
 
 /* Launch sequences: every solve call issues k_init_v, k_gather_y (shared Ybus) and the NR kernel on
@@ -136,7 +137,16 @@ int pfgpu_grid_load_dc(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, cons
 int pfgpu_grid_destroy(pfgpu_grid* grid);
 /* The next solve on the grid factors afresh: a zero / non-finite pivot in its first
  * iteration is SolverFactor (5), later ones SolverReFactor (6), as with klu_factor then
- * klu_refactor. A loaded grid starts fresh. */
+ * klu_refactor. A loaded grid starts fresh.
+ * Pivots chosen on the numbers (host re-pivot, the partial pivoting of klu_factor,
+ * KLUSolver.cpp:31-45): when the first factorisation of a fresh solve meets a zero or
+ * non-finite diagonal pivot, the library takes J at the first such instance's initial voltages,
+ * chooses its pivot rows on the host (threshold partial pivoting: the diagonal row while it
+ * holds at least 1e-3 of its column's largest candidate), re-solves those instances in FP64 on
+ * a plan with these pivots (pfgpu_stats.repivot_rows) and keeps that plan for the grid's later
+ * solves, as klu_refactor keeps klu_factor's pivots, until this call drops it. A numerically
+ * singular J keeps SolverFactor. PFGPU_FP32_STRICT and pfgpu_solve_batch_device_noalloc do not
+ * re-pivot. */
 int pfgpu_grid_reset(pfgpu_grid* grid);
 int pfgpu_grid_get_info(const pfgpu_grid* grid, pfgpu_grid_info* info);
 /* J pattern in the reference layout: Jp[n_jac+1], Ji[nnz_jac]. */

@@ -46,7 +46,7 @@ class Stats(ctypes.Structure):
                 ("d2h_s", ctypes.c_double), ("instances_per_wg", ctypes.c_int), ("lds_slots", ctypes.c_int),
                 ("workgroups", ctypes.c_int), ("fallback_fp64", ctypes.c_int), ("phase_s", ctypes.c_double * 6),
                 ("block_size", ctypes.c_int), ("resident_workgroups", ctypes.c_int),
-                ("graph_launches", ctypes.c_int)]
+                ("graph_launches", ctypes.c_int), ("repivot_rows", ctypes.c_int)]
 
     PHASES = ("jfill", "factor", "tail", "backward", "update", "mismatch")
 

@@ -109,7 +109,7 @@ static std::vector<int> bus_block_order(const JPattern& J) {
     std::vector<std::vector<int>> adj(n);
     for (int c = 0; c < N; ++c)
         for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
-            const int a = blk[J.Ji[q]], b = blk[c];
+            const int a = blk[J.row_of(J.Ji[q])], b = blk[c];
             if (a == b) continue;
             adj[a].push_back(b);
             adj[b].push_back(a);
@@ -164,17 +164,26 @@ static inline int find_sorted(const std::vector<int>& v, int b, int e, int x) {
 // S_k = struct(A(k+1:,k)) U  union over etree children c of (S_c \ {k}),
 // parent(k) = min S_k (George & Liu's symbolic Cholesky on A + A^T).
 // ---------------------------------------------------------------------------
-SymbolicLU symbolic_lu(const JPattern& J, int max_tail) {
+std::vector<int> column_order(const JPattern& J) { return bus_block_order(J); }
+
+SymbolicLU symbolic_lu(const JPattern& J, int max_tail, const std::vector<int>* perm) {
     SymbolicLU S;
     const int N = J.N;
     S.N = N;
-    S.perm = bus_block_order(J);
+    if (perm) {
+        if ((int)perm->size() != N) throw std::runtime_error("pfgpu: elimination order of the wrong size");
+        S.perm = *perm;
+    } else {
+        S.perm = bus_block_order(J);
+    }
     S.pinv.assign(N, -1);
     for (int k = 0; k < N; ++k) S.pinv[S.perm[k]] = k;
+    for (int k = 0; k < N; ++k)
+        if (S.pinv[k] < 0) throw std::runtime_error("pfgpu: the elimination order is not a permutation");
     std::vector<std::vector<int>> low(N);
     for (int c = 0; c < N; ++c)
         for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
-            const int a = S.pinv[J.Ji[q]], b = S.pinv[c];
+            const int a = S.pinv[J.row_of(J.Ji[q])], b = S.pinv[c];
             if (a == b) continue;
             low[std::min(a, b)].push_back(std::max(a, b));
         }
@@ -204,7 +213,7 @@ SymbolicLU symbolic_lu(const JPattern& J, int max_tail) {
     S.jslot.resize(J.Ji.size());
     for (int c = 0; c < N; ++c)
         for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
-            const int a = S.pinv[J.Ji[q]], b = S.pinv[c];
+            const int a = S.pinv[J.row_of(J.Ji[q])], b = S.pinv[c];
             int slot;
             if (a == b) slot = a;
             else if (a > b) slot = (int)S.off_L + find_sorted(S.ci, S.cp[b], S.cp[b + 1], a);
@@ -443,10 +452,11 @@ DevicePlan build_device_plan(const YbusPattern& Y, const JPattern& J, const Symb
     }
     // --- F rows and unknowns --------------------------------------------
     const int npvpq = (int)J.pvpq.size(), npq = (int)J.pq.size(), off = J.off;
+    // the F entry of J row r is the right-hand side of factored row row_of(r)
     P.fslot_p.assign(n, -1); P.fslot_q.assign(n, -1);
-    if (off) P.fslot_p[J.slack0] = ys(S.pinv[0]);
-    for (int k = 0; k < npvpq; ++k) P.fslot_p[J.pvpq[k]] = ys(S.pinv[k + off]);
-    for (int k = 0; k < npq; ++k) P.fslot_q[J.pq[k]] = ys(S.pinv[k + npvpq + off]);
+    if (off) P.fslot_p[J.slack0] = ys(S.pinv[J.row_of(0)]);
+    for (int k = 0; k < npvpq; ++k) P.fslot_p[J.pvpq[k]] = ys(S.pinv[J.row_of(k + off)]);
+    for (int k = 0; k < npq; ++k) P.fslot_q[J.pq[k]] = ys(S.pinv[J.row_of(k + npvpq + off)]);
     P.ux_bus.resize(N); P.ux_kind.resize(N); P.ux_slot.resize(N);
     if (off) { P.ux_bus[0] = J.slack0; P.ux_kind[0] = 0; P.ux_slot[0] = ys(S.pinv[0]); }
     for (int k = 0; k < npvpq; ++k) { P.ux_bus[k + off] = J.pvpq[k]; P.ux_kind[k + off] = 1; P.ux_slot[k + off] = ys(S.pinv[k + off]); }
@@ -609,7 +619,7 @@ void build_jfill_table(DevicePlan& P) {
 // l, nlev + 1 = dense tail, nlev + 2 + l = backward level l, nlev + 2 + nblev =
 // update. A value lives from its first write to its last read. J entries are
 // written at 0; a fill-in entry (neither a J entry nor in the dense tile) is
-// assigned by its own task (kind 3) instead of being zeroed at 0, so it lives from
+// assigned by its own task (kTaskAssign) instead of being zeroed at 0, so it lives from
 // its task's level only. The tile and the right-hand side / solution slots (written
 // by the mismatch of the previous iteration, read by the update) keep their slots;
 // every other value takes a physical slot freed by a value whose last read came at
@@ -640,7 +650,7 @@ void recycle_slots(DevicePlan& P) {
         zero[s] = !fixed[s] && !isj[s] && !written[s];
     }
     for (size_t t = 0; t < P.tasks.size() / 4; ++t)
-        if (fill[P.tasks[4 * t]]) P.tasks[4 * t + 3] = 3;
+        if (fill[P.tasks[4 * t]]) P.tasks[4 * t + 3] |= kTaskAssign;
     {
         std::vector<int> z;
         for (int s : P.zero_slots)
@@ -1062,8 +1072,8 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
                 it[2] = P.pairs[4 * p + 2] * sc;
             }
             if (last)
-                it[3] = P.tasks[4 * t] * sc | (lt << kStreamTeamShift) | (P.tasks[4 * t + 3] == 1 ? kStreamRecip : 0) |
-                        (P.tasks[4 * t + 3] == 3 ? kStreamAssign : 0) | kStreamEnd;
+                it[3] = P.tasks[4 * t] * sc | (lt << kStreamTeamShift) | ((P.tasks[4 * t + 3] & 3) == 1 ? kStreamRecip : 0) |
+                        ((P.tasks[4 * t + 3] & kTaskAssign) ? kStreamAssign : 0) | kStreamEnd;
         },
         3, fpad, L.f_lv, L.f_lc, L.fs);
     lay(
@@ -1089,6 +1099,98 @@ LaneStreams build_lane_streams(const DevicePlan& P, int G, int nslot, const Lean
             L.bs.insert(L.bs.end(), bpad, bpad + 4);
         }
     return L;
+}
+
+// ---------------------------------------------------------------------------
+// Host re-pivot: threshold partial pivoting on J's numbers (see analysis.hpp).
+// Left-looking LU of J(:, perm): column k is the sparse triangular solve L x = J(:, perm[k]) over
+// the columns its nonzeros reach (depth-first search, reverse post-order), then the pivot row is
+// chosen among the rows not pivotal yet. Only L is kept: the pivots are what this is for.
+// ---------------------------------------------------------------------------
+std::vector<int> threshold_pivot_rows(const JPattern& J, const std::vector<double>& Jx, const std::vector<int>& perm,
+                                      double tau) {
+    const int N = J.N;
+    if ((int)Jx.size() != (int)J.Ji.size() || (int)perm.size() != N)
+        throw std::runtime_error("pfgpu: re-pivot inputs of the wrong size");
+    std::vector<int> step_of_row(N, -1), pivrow(N, -1);  // row -> step it pivots, step -> row
+    std::vector<int> Lp(1, 0), Li;                          // L by steps, original row indices
+    std::vector<double> Lx;
+    std::vector<double> x(N, 0.0);
+    // inz: row is a nonzero of the current column; vis: step reached by the current search
+    std::vector<int> inz(N, -1), vis(N, -1), nz, topo, stack, pos;
+    nz.reserve(N);
+    for (int k = 0; k < N; ++k) {
+        const int c = perm[k];
+        nz.clear();
+        topo.clear();
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            const int r = J.Ji[q];
+            x[r] += Jx[q];
+            if (inz[r] != k) { inz[r] = k; nz.push_back(r); }
+        }
+        // the steps the column reaches: depth-first from the pivotal rows of the column through
+        // the graph of L, each step finished (appended to topo) after every step it reaches
+        const size_t n0 = nz.size();
+        for (size_t t = 0; t < n0; ++t) {
+            const int j0 = step_of_row[nz[t]];
+            if (j0 < 0 || vis[j0] == k) continue;
+            vis[j0] = k;
+            stack.assign(1, j0);
+            pos.assign(1, Lp[j0]);
+            while (!stack.empty()) {
+                const int j = stack.back();
+                bool pushed = false;
+                while (pos.back() < Lp[j + 1]) {
+                    const int r = Li[pos.back()++];
+                    if (inz[r] != k) { inz[r] = k; nz.push_back(r); }
+                    const int js = step_of_row[r];
+                    if (js >= 0 && vis[js] != k) {
+                        vis[js] = k;
+                        stack.push_back(js);
+                        pos.push_back(Lp[js]);
+                        pushed = true;
+                        break;
+                    }
+                }
+                if (!pushed) {
+                    topo.push_back(j);
+                    stack.pop_back();
+                    pos.pop_back();
+                }
+            }
+        }
+        // the steps in reverse post-order: x -= L(:, j) * x[pivrow[j]]
+        for (auto it = topo.rbegin(); it != topo.rend(); ++it) {
+            const int j = *it;
+            const double xj = x[pivrow[j]];
+            if (xj == 0.0) continue;
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) x[Li[p]] -= Lx[p] * xj;
+        }
+        // pivot: the diagonal row c while it holds at least tau times the largest candidate
+        int best = -1;
+        double amax = 0.0;
+        for (int r : nz)
+            if (step_of_row[r] < 0 && std::fabs(x[r]) > amax) { amax = std::fabs(x[r]); best = r; }
+        if (best < 0 || !(amax > 0.0) || !std::isfinite(amax)) {
+            for (int r : nz) x[r] = 0.0;
+            return {};
+        }
+        if (step_of_row[c] < 0 && x[c] != 0.0 && std::fabs(x[c]) >= tau * amax) best = c;
+        const double piv = x[best];
+        step_of_row[best] = k;
+        pivrow[k] = best;
+        for (int r : nz) {
+            if (step_of_row[r] < 0 && x[r] != 0.0) {
+                Li.push_back(r);
+                Lx.push_back(x[r] / piv);
+            }
+            x[r] = 0.0;
+        }
+        Lp.push_back((int)Li.size());
+    }
+    std::vector<int> rowpos(N, -1);
+    for (int k = 0; k < N; ++k) rowpos[pivrow[k]] = perm[k];
+    return rowpos;
 }
 
 }  // namespace pfgpu

@@ -37,6 +37,10 @@ struct JPattern {
     std::vector<int> src;        // Ybus CSC value index per J entry (-1 for constants)
     std::vector<uint8_t> sel;    // JSel per J entry
     std::vector<double> cval;    // constant value per J entry (slack-weight column)
+    // Row of the factored matrix that J row r becomes (a row permutation chosen by pivoting on
+    // the numbers, threshold_pivot_rows); empty: J is factored with its own diagonal.
+    std::vector<int> rowpos;
+    int row_of(int r) const { return rowpos.empty() ? r : rowpos[r]; }
 };
 
 // Ybus in CSC (rows sorted per column), as the reference takes it.
@@ -72,8 +76,22 @@ struct SymbolicLU {
     double flops = 0;
 };
 
-// max_tail: largest dense tail (0 disables it)
-SymbolicLU symbolic_lu(const JPattern& J, int max_tail = 96);
+// max_tail: largest dense tail (0 disables it); perm: a given elimination order (perm[k] = J
+// column eliminated k-th) instead of the bus-block minimum degree
+SymbolicLU symbolic_lu(const JPattern& J, int max_tail = 96, const std::vector<int>* perm = nullptr);
+
+// The bus-block minimum-degree elimination order of J (perm[k] = J column eliminated k-th).
+std::vector<int> column_order(const JPattern& J);
+
+// Host re-pivot: a left-looking sparse LU with threshold partial pivoting (Gilbert and Peierls'
+// algorithm, the pivot rule of KLU: the diagonal row while its magnitude is at least tau times the
+// column's largest candidate, else the largest) of J's values Jx, columns in the order `perm`.
+// Returns rowpos (J row -> the J column it is the pivot of), or an empty vector when a column has
+// no nonzero candidate (numerically singular).
+std::vector<int> threshold_pivot_rows(const JPattern& J, const std::vector<double>& Jx, const std::vector<int>& perm,
+                                      double tau = 1e-3);
+
+constexpr int kTaskAssign = 4;  // DevicePlan task kind flag: the task assigns its value
 
 struct Schedule {
     // Factor + forward-solve tasks, grouped by level.
@@ -132,8 +150,9 @@ struct DevicePlan {
     // dense-tile slots that hold no J entry (fill-in): zeroed by the J-fill phase
     std::vector<int> zero_slots;
     // factor + forward tasks (level-major), 4 ints each: {out slot, pair begin, pair end, kind};
-    // kind 0 = L/U entry, 1 = pivot (stores the reciprocal), 2 = forward-solve entry,
-    // 3 = fill-in entry outside the dense tile (assigns -sum: its slot is recycled).
+    // kind & 3: 0 = L/U entry, 1 = pivot (stores the reciprocal), 2 = forward-solve entry;
+    // kind & kTaskAssign: a fill-in entry outside the dense tile (no J entry: it assigns -sum,
+    // its slot is recycled; under a row permutation this can be a pivot as well).
     // Other tasks start from the value already in their slot (J entry, 0 for tile
     // fill-in, or F) and subtract sum X[l]*X[u]*X[d] over their pairs. Tasks without
     // pairs are not listed. Value slots are shared by values with disjoint lifetimes.

@@ -102,6 +102,9 @@ __host__ __device__ constexpr int wg_state_bytes(int G, int NT) {
 constexpr int kFlagStall = 1;   // not converged, but the last mismatch is below the stall level
 constexpr int kFlagGrowth = 2;  // a pivot lost more than the growth threshold to cancellation
 constexpr int kStatusNotRun = 8;  // chain step after the chain's first failure (NotInitError)
+// Host re-pivot threshold: a diagonal pivot is kept while it holds at least this fraction of its
+// column's largest candidate (klu_defaults' tol)
+constexpr double kRepivotTau = 1e-3;
 
 // ---------------------------------------------------------------------------
 // Kernel arguments
@@ -155,7 +158,8 @@ struct KArgs {
     void* Xg;            // spill [nblocks][nspill][G]
     int* iters;
     int* status;
-    double* Jx_last;     // optional (row 0)
+    double* Jx_last;     // optional: the J values (reference CSC order) of row jx_row at its J fills
+    long long jx_row;
     // launch instance i solves the caller's instance imap[i] (null: i): the index of its Ybus
     // values / edits, its V0 and V rows, its Sbus rows and its outputs
     const int* imap;
@@ -637,7 +641,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         if (act) {
             auto put = [&](int slot, int jidx, double jv) {
                 if (slot < 0) return;
-                if (a.Jx_last != nullptr && row == 0) a.Jx_last[jidx] = jv;
+                if (a.Jx_last != nullptr && row == a.jx_row) a.Jx_last[jidx] = jv;
                 FT v = (FT)jv;
                 if (slot & (1 << 30)) {  // pivot with nothing to eliminate
                     if (bad_pivot(v)) s_piv[g] = 1;
@@ -681,7 +685,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     const int e = e0 + u * nslot;
                     if (e >= a.nye) break;
                     if (a.dc) {  // DC: the J entry of a P row and an angle column is Re(Y)
-                        put(sl[u].x, (a.Jx_last != nullptr && row == 0) ? a.ye_jidx[e].x : 0, y[u].re);
+                        put(sl[u].x, (a.Jx_last != nullptr && row == a.jx_row) ? a.ye_jidx[e].x : 0, y[u].re);
                         continue;
                     }
                     const bool diag = e < a.nyd;
@@ -693,7 +697,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     }
                     const c64 dva = cmul(cconj({-t.re, -t.im}), {-vr[u].im, vr[u].re});  // * (j V_r)
                     int4 jx = {0, 0, 0, 0};
-                    if (a.Jx_last != nullptr && row == 0) jx = a.ye_jidx[e];
+                    if (a.Jx_last != nullptr && row == a.jx_row) jx = a.ye_jidx[e];
                     put(sl[u].x, jx.x, dva.re);
                     put(sl[u].y, jx.y, dva.im);
                     put(sl[u].z, jx.z, dvm.re);
@@ -1321,8 +1325,11 @@ __global__ void k_init_v(const double* V0, long long v0_stride, double* V, int n
 // / non-finite pivot or a non-finite mismatch. A row that diverged is not re-solved.
 // Chain mode takes whole chains (a chain is re-run from its head). Writes the list and its
 // length.
+// Lists the units (instances, or chains) whose first failure calls for another pass: mode 0, the
+// FP32 form's FP64 re-solve (breakdown or stall); mode 1, the host re-pivot (a pivot failure on the
+// first factorisation of a fresh solve, SolverFactor).
 __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, const int* flags, long long nrows,
-                                                            int chain, int* list, int* count) {
+                                                            int chain, int mode, int* list, int* count) {
     __shared__ int wsum[16];
     __shared__ int base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1338,7 +1345,7 @@ __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, co
                 if (r >= nrows) break;
                 const int st = status[r];
                 if (st == 0 || st == kStatusNotRun) continue;
-                take = st == 3 || st == 5 || st == 6 || (flags[r] & kFlagStall) != 0;
+                take = mode == 0 ? (st == 3 || st == 5 || st == 6 || (flags[r] & kFlagStall) != 0) : st == 5;
                 break;  // a chain's first failure decides
             }
         const unsigned long long m = __ballot(take);
@@ -1402,6 +1409,14 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    T* get_or_alloc(size_t count) {
+        if (n < count) {
+            release();
+            HIP_OK(hipMalloc(&p, count * sizeof(T)));
+            n = count;
+        }
+        return p;
+    }
     ~DevBuf() { release(); }
 };
 
@@ -1434,6 +1449,14 @@ struct pfgpu_grid {
     // initialize, KLUSolver.cpp:24-45); cleared by a solve, set by load and pfgpu_grid_reset
     bool fresh = true;
     bool dc = false;  // a DC grid (pfgpu_grid_load_dc): one linear solve per instance
+    std::vector<double> w_h;  // slack weights (host copy)
+    // Pivots chosen on the numbers (host re-pivot): a plan of the same grid whose factor takes
+    // its pivots off the diagonal where the first factorisation of a fresh solve met a zero or
+    // non-finite pivot; once built it serves every solve until pfgpu_grid_reset, as klu_refactor
+    // reuses the pivots of klu_factor (KLUSolver.cpp:24-59)
+    std::unique_ptr<pfgpu_grid> alt;
+    bool alt_active = false;
+    DevBuf<double> jx_probe;  // J values of the instance the re-pivot chooses its pivots on
     // Launch sequences (k_init_v, k_gather_y, k_nr) captured as HIP graphs, by the exact launch
     // arguments: a sequence seen a second time is captured, and replayed from then on.
     struct GraphEntry {
@@ -1667,6 +1690,7 @@ struct SolveArgs {
     int* iters = nullptr;
     int* status = nullptr;
     double* Jx_last = nullptr;
+    long long jx_row = 0;   // the row whose J values Jx_last receives
 };
 
 // Resident workgroups of a launch shape (occupancy of the kernel variant at its LDS size).
@@ -1789,7 +1813,7 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     a.V = q.V; a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Xg = ws + o_x;
     a.ye_diag_of = gr->ye_diag_of.p; a.nyd = P.nyd;
-    a.iters = q.iters; a.status = q.status; a.Jx_last = q.Jx_last;
+    a.iters = q.iters; a.status = q.status; a.Jx_last = q.Jx_last; a.jx_row = q.jx_row;
     a.v_off = (int)lc.v_off;
     a.Yell = gather ? reinterpret_cast<const double*>(ws + o_yell) : nullptr;
     a.Yye = gather ? reinterpret_cast<const double*>(ws + o_yye) : nullptr;
@@ -1900,6 +1924,10 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     }
 }
 
+// Computes the grid's symbolic factor, schedule and device plan from g->Y and g->J (elimination
+// order `perm`, or the bus-block minimum degree) and uploads its tables.
+void build_plan(pfgpu_grid* g, const std::vector<int>* perm);
+
 // Core solve on device buffers: the launch in the requested form; with FP32 factors (unless
 // strict), the rows that stalled or broke down are re-solved from their initial voltages with
 // FP64 factors (chains from their heads).
@@ -1934,30 +1962,83 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
         V0_keep = keep;
     }
     const bool fresh = gr->fresh;
-    launch_pass(ctx, gr, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st, fx);
+    pfgpu_grid* pg = gr->alt_active ? gr->alt.get() : gr;  // the plan in use
+    launch_pass(ctx, pg, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st, fx);
     gr->fresh = false;
     ctx->last_rows = q.nrows;
     ctx->last_flags = reinterpret_cast<const int*>(aux + AL.o_flags);
-    if (!fallback) return;
     int* list = reinterpret_cast<int*>(aux + AL.o_list);
     int* count = reinterpret_cast<int*>(aux + AL.o_count);
+    if (fallback) {
+        hipLaunchKernelGGL(k_collect_fallback, dim3(1), dim3(1024), 0, ctx->stream, q.status,
+                           reinterpret_cast<const int*>(aux + AL.o_flags), q.nrows, q.chain, 0, list, count);
+        HIP_OK(hipGetLastError());
+        int nfb = 0;
+        HIP_OK(hipMemcpyAsync(&nfb, count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (nfb > 0) {
+            SolveArgs r = q;
+            r.V0 = V0_keep;
+            r.Jx_last = nullptr;
+            pfgpu_stats s2{};
+            // the caller's launch shape when it fixed one (results independent of which rows re-solve)
+            launch_pass(ctx, pg, r, PFGPU_FP64, q.G_req, q.NT_req, nfb, list, AL, fresh, st ? &s2 : nullptr, fx);
+            if (st) {
+                st->fallback_fp64 = nfb;
+                st->kernel_s += s2.kernel_s;
+                for (int k = 0; k < 6; ++k) st->phase_s[k] += s2.phase_s[k];
+            }
+        }
+    }
+    // host re-pivot: only a fresh solve's first factorisation chooses pivots (klu_factor); the
+    // strict FP32 form and the allocation-free entry point leave their SolverFactor rows as they are
+    if (!fresh || fx || gr->alt_active || q.precision == PFGPU_FP32_STRICT) return;
     hipLaunchKernelGGL(k_collect_fallback, dim3(1), dim3(1024), 0, ctx->stream, q.status,
-                       reinterpret_cast<const int*>(aux + AL.o_flags), q.nrows, q.chain, list, count);
+                       reinterpret_cast<const int*>(aux + AL.o_flags), q.nrows, q.chain, 1, list, count);
     HIP_OK(hipGetLastError());
-    int nfb = 0;
-    HIP_OK(hipMemcpyAsync(&nfb, count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int head[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(&head[0], count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(&head[1], list, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    if (nfb <= 0) return;
+    const int nrp = head[0];
+    if (nrp <= 0) return;
+    // J at the first listed unit's initial voltages (one FP64 iteration): the matrix whose first
+    // factorisation failed (of a chain: its first step's)
+    double* jx = gr->jx_probe.get_or_alloc((size_t)std::max(P.nJ, 1));
+    SolveArgs p = q;
+    p.V0 = V0_keep;
+    p.max_iter = 1;
+    p.Jx_last = jx;
+    p.jx_row = (long long)head[1] * q.chain;
+    launch_pass(ctx, gr, p, PFGPU_FP64, q.G_req, q.NT_req, 1, list, AL, true, nullptr, nullptr);
+    std::vector<double> Jx((size_t)P.nJ);
+    HIP_OK(hipMemcpyAsync(Jx.data(), jx, Jx.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    const std::vector<int> order = pfgpu::column_order(gr->J);
+    std::vector<int> rowpos = pfgpu::threshold_pivot_rows(gr->J, Jx, order, kRepivotTau);
+    if (rowpos.empty()) return;  // numerically singular: SolverFactor stands, as klu_factor's failure
+    auto alt = std::make_unique<pfgpu_grid>();
+    alt->ctx = ctx;
+    alt->Y = gr->Y;
+    alt->J = gr->J;
+    alt->J.rowpos = std::move(rowpos);
+    alt->dc = gr->dc;
+    alt->info = gr->info;
+    alt->w_h = gr->w_h;
+    build_plan(alt.get(), &order);
+    gr->alt = std::move(alt);
+    gr->alt_active = true;
+    // the listed units again, from their initial voltages, on the re-pivoted plan
     SolveArgs r = q;
     r.V0 = V0_keep;
     r.Jx_last = nullptr;
-    pfgpu_stats s2{};
-    // the caller's launch shape when it fixed one (results independent of which rows re-solve)
-    launch_pass(ctx, gr, r, PFGPU_FP64, q.G_req, q.NT_req, nfb, list, AL, fresh, st ? &s2 : nullptr, fx);
+    pfgpu_stats s3{};
+    launch_pass(ctx, gr->alt.get(), r, PFGPU_FP64, q.G_req, q.NT_req, nrp, list, AL, true, st ? &s3 : nullptr,
+                nullptr);
     if (st) {
-        st->fallback_fp64 = nfb;
-        st->kernel_s += s2.kernel_s;
-        for (int k = 0; k < 6; ++k) st->phase_s[k] += s2.phase_s[k];
+        st->repivot_rows = nrp;
+        st->kernel_s += s3.kernel_s;
+        for (int k = 0; k < 6; ++k) st->phase_s[k] += s3.phase_s[k];
     }
 }
 
@@ -2064,6 +2145,66 @@ int pfgpu_grid_load_dc(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, cons
 }
 
 namespace {
+void build_plan(pfgpu_grid* g, const std::vector<int>* perm) {
+    pfgpu::SymbolicLU S = pfgpu::symbolic_lu(g->J, 96, perm);
+    pfgpu::Schedule sch = pfgpu::build_schedule(S);
+    g->P = pfgpu::build_device_plan(g->Y, g->J, S, sch);
+    const auto& P = g->P;
+    g->info.n_bus = g->Y.n;
+    g->info.n_jac = P.N;
+    g->info.nnz_ybus = P.nnzY;
+    g->info.nnz_jac = P.nJ;
+    g->info.nnz_lu = P.nnzLU;
+    g->info.n_slots = P.nvals;
+    g->info.n_levels = P.nlev;
+    g->info.n_back_levels = P.nblev;
+    g->info.n_pairs = (int64_t)P.pairs.size() / 4;
+    g->info.flops = P.flops;
+    g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
+    {
+        std::vector<int> pc(2 * P.yr_pos.size());
+        for (size_t k = 0; k < P.yr_pos.size(); ++k) { pc[2 * k] = P.yr_pos[k]; pc[2 * k + 1] = P.yr_col[k]; }
+        g->yr_pc.upload(pc.empty() ? std::vector<int>(2, 0) : pc);
+        // column-major row table, width the longest row rounded up to kEll columns
+        const int n = (int)P.yr_ptr.size() - 1;
+        int w = 1;
+        for (int r = 0; r < n; ++r) w = std::max(w, P.yr_ptr[r + 1] - P.yr_ptr[r]);
+        w = (w + kEll - 1) / kEll * kEll;
+        g->ell_w = w;
+        std::vector<int> ell(2 * (size_t)w * std::max(n, 1), 0);
+        for (int r = 0; r < n; ++r)
+            for (int k = 0; k < w; ++k) {
+                const int q = P.yr_ptr[r] + k;
+                const bool in = q < P.yr_ptr[r + 1];
+                ell[2 * ((size_t)k * n + r)] = in ? P.yr_pos[q] : -1;
+                ell[2 * ((size_t)k * n + r) + 1] = in ? P.yr_col[q] : r;
+            }
+        g->yr_ell.upload(ell);
+    }
+    auto up_or = [](auto& d, const auto& v, size_t minsz) {
+        auto w = v;
+        if (w.size() < minsz) w.resize(minsz);
+        d.upload(w);
+    };
+    up_or(g->ye_pos, P.ye_pos, 1); up_or(g->ye_rc, P.ye_rc, 2); up_or(g->ye_slot, P.ye_slot, 4);
+    up_or(g->ye_jidx, P.ye_jidx, 4); up_or(g->ye_diag_of, P.ye_diag_of, 1); up_or(g->jc_slot, P.jc_slot, 1); up_or(g->jc_jidx, P.jc_jidx, 1);
+    up_or(g->jc_val, P.jc_val, 1);
+    g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
+    g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
+    {
+        std::vector<int> bd(2 * (size_t)std::max(P.n, 1), -1);
+        for (size_t j = 0; j < P.ux_bus.size(); ++j) {
+            if (P.ux_kind[j] == 0) g->slack_dx = P.ux_slot[j];
+            else bd[2 * (size_t)P.ux_bus[j] + (P.ux_kind[j] == 1 ? 0 : 1)] = P.ux_slot[j];
+        }
+        g->bus_dx.upload(bd);
+    }
+    g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
+    g->j_cval.upload(P.j_cval); g->j_slot.upload(P.j_slot);
+    g->zero_slots.upload(P.zero_slots.empty() ? std::vector<int>(1, 0) : P.zero_slots);
+    g->w.upload(g->w_h);
+}
+
 int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
               const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack, bool dc,
               pfgpu_grid** out) {
@@ -2098,66 +2239,11 @@ int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* sl
         for (int k = 0; k < npv; ++k)
             if (pv[k] < 0 || pv[k] >= n) throw ApiError(PFGPU_E_INVALID, "pv bus out of range");
         g->J = pfgpu::build_jacobian_pattern(g->Y, slack_ids, nslack, slack_weights, pv, npv, pq, npq, single_slack != 0);
-        pfgpu::SymbolicLU S = pfgpu::symbolic_lu(g->J);
-        pfgpu::Schedule sch = pfgpu::build_schedule(S);
-        g->P = pfgpu::build_device_plan(g->Y, g->J, S, sch);
-        const auto& P = g->P;
-        g->info.n_bus = n;
-        g->info.n_jac = P.N;
-        g->info.nnz_ybus = P.nnzY;
-        g->info.nnz_jac = P.nJ;
-        g->info.nnz_lu = P.nnzLU;
-        g->info.n_slots = P.nvals;
-        g->info.n_levels = P.nlev;
-        g->info.n_back_levels = P.nblev;
-        g->info.n_pairs = (int64_t)P.pairs.size() / 4;
-        g->info.flops = P.flops;
+        g->w_h.assign(slack_weights, slack_weights + n);
         g->info.single_slack = single_slack ? 1 : 0;
         g->info.dc = dc ? 1 : 0;
         g->dc = dc;
-        g->yr_ptr.upload(P.yr_ptr); g->yr_col.upload(P.yr_col); g->yr_pos.upload(P.yr_pos);
-        {
-            std::vector<int> pc(2 * P.yr_pos.size());
-            for (size_t k = 0; k < P.yr_pos.size(); ++k) { pc[2 * k] = P.yr_pos[k]; pc[2 * k + 1] = P.yr_col[k]; }
-            g->yr_pc.upload(pc.empty() ? std::vector<int>(2, 0) : pc);
-            // column-major row table, width the longest row rounded up to kEll columns
-            const int n = (int)P.yr_ptr.size() - 1;
-            int w = 1;
-            for (int r = 0; r < n; ++r) w = std::max(w, P.yr_ptr[r + 1] - P.yr_ptr[r]);
-            w = (w + kEll - 1) / kEll * kEll;
-            g->ell_w = w;
-            std::vector<int> ell(2 * (size_t)w * std::max(n, 1), 0);
-            for (int r = 0; r < n; ++r)
-                for (int k = 0; k < w; ++k) {
-                    const int q = P.yr_ptr[r] + k;
-                    const bool in = q < P.yr_ptr[r + 1];
-                    ell[2 * ((size_t)k * n + r)] = in ? P.yr_pos[q] : -1;
-                    ell[2 * ((size_t)k * n + r) + 1] = in ? P.yr_col[q] : r;
-                }
-            g->yr_ell.upload(ell);
-        }
-        auto up_or = [](auto& d, const auto& v, size_t minsz) {
-            auto w = v;
-            if (w.size() < minsz) w.resize(minsz);
-            d.upload(w);
-        };
-        up_or(g->ye_pos, P.ye_pos, 1); up_or(g->ye_rc, P.ye_rc, 2); up_or(g->ye_slot, P.ye_slot, 4);
-        up_or(g->ye_jidx, P.ye_jidx, 4); up_or(g->ye_diag_of, P.ye_diag_of, 1); up_or(g->jc_slot, P.jc_slot, 1); up_or(g->jc_jidx, P.jc_jidx, 1);
-        up_or(g->jc_val, P.jc_val, 1);
-        g->fslot_p.upload(P.fslot_p); g->fslot_q.upload(P.fslot_q);
-        g->ux_bus.upload(P.ux_bus); g->ux_slot.upload(P.ux_slot); g->ux_kind.upload(P.ux_kind);
-        {
-            std::vector<int> bd(2 * (size_t)std::max(P.n, 1), -1);
-            for (size_t j = 0; j < P.ux_bus.size(); ++j) {
-                if (P.ux_kind[j] == 0) g->slack_dx = P.ux_slot[j];
-                else bd[2 * (size_t)P.ux_bus[j] + (P.ux_kind[j] == 1 ? 0 : 1)] = P.ux_slot[j];
-            }
-            g->bus_dx.upload(bd);
-        }
-        g->j_ypos.upload(P.j_ypos); g->j_rbus.upload(P.j_rbus); g->j_cbus.upload(P.j_cbus); g->j_sel.upload(P.j_sel);
-        g->j_cval.upload(P.j_cval); g->j_slot.upload(P.j_slot);
-        g->zero_slots.upload(P.zero_slots.empty() ? std::vector<int>(1, 0) : P.zero_slots);
-        g->w.upload(std::vector<double>(slack_weights, slack_weights + n));
+        build_plan(g.get(), nullptr);
         *out = g.release();
         return 0;
     });
@@ -2193,7 +2279,12 @@ int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji) {
 int pfgpu_grid_reset(pfgpu_grid* grid) {
     return guarded([&]() -> int {
         if (!grid) throw ApiError(PFGPU_E_INVALID, "grid is NULL");
+        // the next solve factors afresh: pivots are chosen again (the re-pivoted plan is dropped)
+        HIP_OK(hipSetDevice(grid->ctx->device));
+        HIP_OK(hipStreamSynchronize(grid->ctx->stream));
         grid->fresh = true;
+        grid->alt_active = false;
+        grid->alt.reset();
         return 0;
     });
 }
@@ -2220,9 +2311,10 @@ int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const dou
         q.max_iter = max_iter; q.tol = tol; q.iters = d_iters; q.status = d_status;
         // the kernel alone is timed: no FP64 re-solve pass
         q.precision = precision == PFGPU_FP64 ? PFGPU_FP64 : PFGPU_FP32_STRICT;
+        pfgpu_grid* pg = grid->alt_active ? grid->alt.get() : grid;  // the plan in use
         for (int G = 1; G <= kMaxG; G <<= 1) {
             for (int NT : {256, 1024}) {
-                LaunchChoice lc = choose_launch(ctx, grid->P, ninst, fsize, G, NT);
+                LaunchChoice lc = choose_launch(ctx, pg->P, ninst, fsize, G, NT);
                 if (lc.G != G) continue;  // the dense tail did not fit at this G
                 q.G_req = G;
                 q.NT_req = NT;
@@ -2235,8 +2327,9 @@ int pfgpu_autotune_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, const dou
                 if (tmin < best) { best = tmin; bG = G; bNT = NT; }
             }
         }
-        grid->tuned_G[pslot] = bG;
-        grid->tuned_NT[pslot] = bNT;
+        pg = grid->alt_active ? grid->alt.get() : grid;
+        pg->tuned_G[pslot] = bG;
+        pg->tuned_NT[pslot] = bNT;
         if (best_G) *best_G = bG;
         if (best_block) *best_block = bNT;
         if (best_seconds) *best_seconds = best;

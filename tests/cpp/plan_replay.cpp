@@ -16,7 +16,51 @@
 
 using namespace pfgpu;
 
+namespace {
+// row permutation for the next replays (empty: J's own diagonal), set by replay_set_rowpos
+std::vector<int> g_rowpos;
+// DC form for the next replays (replay_set_dc): J = Re(Ybus) over the angle unknowns,
+// F = J theta - Re(Sbus), one linear solve (the kernel's DC mode)
+int g_dc = 0;
+}
+
 extern "C" {
+
+// Pivots chosen on the numbers: the product's threshold_pivot_rows on J values Jx (the
+// reference's CSC order), columns in the bus-block order. Writes rowpos (N ints) and returns N,
+// or 0 when J is numerically singular.
+int replay_repivot(int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack, const double* w,
+                   const int* pv, int npv, const int* pq, int npq, int single_slack, const double* Jx, double tau,
+                   int* rowpos_out) {
+    YbusPattern Y;
+    Y.n = n;
+    Y.p.assign(Yp, Yp + n + 1);
+    Y.i.assign(Yi, Yi + Yp[n]);
+    JPattern J = build_jacobian_pattern(Y, slack_ids, nslack, w, pv, npv, pq, npq, single_slack != 0);
+    const std::vector<double> x(Jx, Jx + J.Ji.size());
+    const std::vector<int> rp = threshold_pivot_rows(J, x, column_order(J), tau);
+    if (rp.empty()) return 0;
+    std::copy(rp.begin(), rp.end(), rowpos_out);
+    return J.N;
+}
+
+// The bus-block elimination order of J (perm[k] = J column eliminated k-th); returns N.
+int replay_column_order(int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack, const double* w,
+                        const int* pv, int npv, const int* pq, int npq, int single_slack, int* perm_out) {
+    YbusPattern Y;
+    Y.n = n;
+    Y.p.assign(Yp, Yp + n + 1);
+    Y.i.assign(Yi, Yi + Yp[n]);
+    JPattern J = build_jacobian_pattern(Y, slack_ids, nslack, w, pv, npv, pq, npq, single_slack != 0);
+    const std::vector<int> o = column_order(J);
+    std::copy(o.begin(), o.end(), perm_out);
+    return J.N;
+}
+
+void replay_set_dc(int dc) { g_dc = dc; }
+
+// Row permutation used by the following replay_solve calls (N = 0 clears it).
+void replay_set_rowpos(const int* rowpos, int N) { g_rowpos.assign(rowpos, rowpos + N); }
 
 // Returns iterations (>= 0) or a negative value on analysis failure. status_out: 0 ok, 2 max iter, 3 non-finite.
 // Also reports write-write / read-before-write conflicts found in the level schedule (conflicts_out).
@@ -29,7 +73,12 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     Y.p.assign(Yp, Yp + n + 1);
     Y.i.assign(Yi, Yi + Yp[n]);
     JPattern J = build_jacobian_pattern(Y, slack_ids, nslack, w, pv, npv, pq, npq, single_slack != 0);
-    SymbolicLU S = symbolic_lu(J);
+    const std::vector<int> order = column_order(J);
+    if (!g_rowpos.empty()) {
+        if ((int)g_rowpos.size() != J.N) return -1;
+        J.rowpos = g_rowpos;
+    }
+    SymbolicLU S = symbolic_lu(J, 96, &order);
     Schedule sch = build_schedule(S);
     DevicePlan P = build_device_plan(Y, J, S, sch);
     // lean != 0: replay the lean streams (LDS byte offsets, aligned task ends) of the kernel's
@@ -127,7 +176,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
                         key += fwd ? (long)P.pairs[4 * p] * 7 + (long)P.pairs[4 * p + 1] * 13 + (long)P.pairs[4 * p + 2] * 17
                                    : (long)P.bpairs[2 * p] * 7 + (long)P.bpairs[2 * p + 1] * 13;
                     if (npair_seen[o] != tasks[4 * t + 2] - tasks[4 * t + 1] || ends[o] != team_of[o] || pair_sum[o] != key ||
-                        recip_of[o] != (fwd && tasks[4 * t + 3] == 1 ? 1 : 0))
+                        recip_of[o] != (fwd && (tasks[4 * t + 3] & 3) == 1 ? 1 : 0))
                         ++conflicts;
                 }
             }
@@ -137,8 +186,25 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
         *conflicts_out = conflicts;
     }
 
+    std::vector<char> has_va(n, 0);
+    for (int j = 0; j < P.N; ++j)
+        if (P.ux_kind[j] == 1) has_va[P.ux_bus[j]] = 1;
     auto mismatch = [&](double& fmax) {
         fmax = 0;
+        if (g_dc) {
+            const double vas = Va[P.slack0];
+            for (int r = 0; r < n; ++r) {
+                double acc = 0;
+                for (int q = P.yr_ptr[r]; q < P.yr_ptr[r + 1]; ++q)
+                    if (has_va[P.yr_col[q]]) acc += Yc[P.yr_pos[q]].real() * (Va[P.yr_col[q]] - vas);
+                if (P.fslot_p[r] >= 0) {
+                    X[P.fslot_p[r]] = rnd(acc - Sc[r].real());
+                    fmax = std::max(fmax, std::fabs(Va[r] - vas));
+                    if (!std::isfinite(Va[r])) fmax = NAN;
+                }
+            }
+            return;
+        }
         for (int r = 0; r < n; ++r) {
             cplx acc(0, 0);
             for (int q = P.yr_ptr[r]; q < P.yr_ptr[r + 1]; ++q) acc += Yc[P.yr_pos[q]] * Vc[P.yr_col[q]];
@@ -151,6 +217,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     };
     auto jval = [&](int q) -> double {
         const int sel = P.j_sel[q];
+        if (g_dc) return Yc[P.j_ypos[q]].real();
         if (sel == SEL_CONST) return P.j_cval[q];
         const int r = P.j_rbus[q], c = P.j_cbus[q];
         const cplx y = Yc[P.j_ypos[q]];
@@ -167,7 +234,7 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
     };
     double fmax;
     mismatch(fmax);
-    bool conv = fmax < tol;
+    bool conv = g_dc ? false : fmax < tol;
     int it = 0, status = 0;
     while (!conv && it < max_iter) {
         ++it;
@@ -254,6 +321,11 @@ int replay_solve(int n, const int* Yp, const int* Yi, const double* Yx, const do
         for (int b = 0; b < n; ++b) { Vc[b] = Vm[b] * cplx(std::cos(Va[b]), std::sin(Va[b])); vmin = std::min(vmin, Vm[b]); }
         if (vmin < 0) for (int b = 0; b < n; ++b) { Vm[b] = std::abs(Vc[b]); Va[b] = std::arg(Vc[b]); }
         mismatch(fmax);
+        if (g_dc) {  // one linear solve; SolverSolve if theta is not finite or too large
+            status = fmax < 1e6 ? 0 : 7;
+            conv = status == 0;
+            break;
+        }
         if (!std::isfinite(fmax)) { status = 3; break; }
         conv = fmax < tol;
     }
