@@ -164,7 +164,12 @@ int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji);
  *   v0_stride n: V holds one initial vector per instance; 0: V[0:n] is the
  *             initial vector of every instance (V still receives ninst rows)
  *   precision PFGPU_FP64, PFGPU_FP32 or PFGPU_FP32_STRICT (see above)
- *   instances_per_wg  0 = automatic, else G in {1,2,4,8,16,32}
+ *   instances_per_wg  0 = automatic, else G in {1,2,4,8,16,32}; -1 = wide launch: the
+ *             instances one after the other, each spread over 32 workgroups that run the NR
+ *             phases and each elimination level together, separated by grid-wide barriers
+ *             (cooperative launch). Automatic for a single instance (or a single chain) of a
+ *             plan with at least 200000 factor pairs; pfgpu_stats.instances_per_wg is -1 then.
+ *             Not for DC grids; PFGPU_WIDE_BLOCKS sets the workgroup count.
  *   iters/status      [ninst] outputs: NR iterations (linear solves) and the
  *             ErrorType (0 converged)
  *   Jx_last   optional [nnz_jac] J values of instance 0's last iteration (or NULL)

@@ -31,7 +31,7 @@ def _run(cmd, **kw):
 
 # translation units of libpfgpu.so: the HIP kernels and their C ABI, the host analysis, the
 # host mirror of the solver surface (over the C ABI) and the serving loop with its C ABI
-LIB_SOURCES = ["pfgpu.hip", "batchprep.hip", "analysis.cpp", "gridmodel.cpp", "serve.cpp", "serve_capi.cpp"]
+LIB_SOURCES = ["pfgpu.hip", "wide.hip", "batchprep.hip", "analysis.cpp", "gridmodel.cpp", "serve.cpp", "serve_capi.cpp"]
 
 
 def build_lib() -> str:

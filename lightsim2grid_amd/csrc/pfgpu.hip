@@ -39,6 +39,7 @@
 #include "../../include/pfgpu.h"
 #include "analysis.hpp"
 #include "engine.hpp"
+#include "wide.hpp"
 
 std::string& pfgpu_engine::last_error() {
     thread_local std::string msg;
@@ -1457,6 +1458,12 @@ struct pfgpu_grid {
     std::unique_ptr<pfgpu_grid> alt;
     bool alt_active = false;
     DevBuf<double> jx_probe;  // J values of the instance the re-pivot chooses its pivots on
+    // Wide launches (wide.hpp): the plan's tasks and pairs as flat tables and the per-level team
+    // sizes for `blocks` workgroups (uploaded by the first wide launch)
+    struct Wide {
+        DevBuf<int> f_lvl, b_lvl, f_team, b_team, tasks, pairs, btasks, bpairs;
+        int blocks = 0;
+    } wide;
     // Launch sequences (k_init_v, k_gather_y, k_nr) captured as HIP graphs, by the exact launch
     // arguments: a sequence seen a second time is captured, and replayed from then on.
     struct GraphEntry {
@@ -1730,6 +1737,120 @@ struct FixedScratch {
     size_t ws_bytes = 0;
 };
 
+// Wide launches: one instance at a time over every workgroup of the device (wide.hpp). Taken for
+// instances_per_wg == -1, and by default for a single launch unit (one instance, or one chain of
+// time steps) on a plan of at least kWidePairs factor pairs, where one workgroup per instance would
+// leave the device idle. Not for DC grids, J captures or the allocation-free entry point.
+constexpr long long kWidePairs = 200000;
+constexpr int kWideBlocks = 32;  // workgroups of a wide launch (PFGPU_WIDE_BLOCKS overrides)
+
+bool wide_wanted(const pfgpu_grid* gr, const SolveArgs& q, int G_req, int nunits, const FixedScratch* fx) {
+    if (gr->dc || q.Jx_last || fx) return false;
+    if (G_req == -1) return true;
+    return G_req == 0 && nunits == 1 && (long long)gr->P.pairs.size() / 4 >= kWidePairs;
+}
+
+// log2 of the team size of each level: the largest power of two (at most 64, at most the level's
+// longest task rounded up) with which the level's tasks still take one lane team each
+std::vector<int> level_teams(const std::vector<int>& lvl, const std::vector<int>& tasks, long long lanes) {
+    std::vector<int> out;
+    for (size_t l = 0; l + 1 < lvl.size(); ++l) {
+        const long long nt = lvl[l + 1] - lvl[l];
+        int maxp = 1;
+        for (int t = lvl[l]; t < lvl[l + 1]; ++t) maxp = std::max(maxp, tasks[4 * t + 2] - tasks[4 * t + 1]);
+        int lt = 0;
+        while (lt < 6 && (1 << lt) < maxp && nt * (2LL << lt) <= lanes) ++lt;
+        out.push_back(lt);
+    }
+    return out;
+}
+
+void launch_wide(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precision, int nunits, const int* imap,
+                 const AuxLayout& AL, bool fresh, pfgpu_stats* st) {
+    const pfgpu::DevicePlan& P = gr->P;
+    const int n = P.n;
+    const bool fp32 = precision != PFGPU_FP64;
+    const int fsize = fp32 ? 4 : 8;
+    const size_t lds = pfgpu_wide::lds_bytes(P.m, fp32);
+    if (lds > (size_t)ctx->lds_max) throw ApiError(PFGPU_E_INVALID, "the dense tail does not fit a wide launch's LDS");
+    int nb = pfgpu_wide::resident_blocks(ctx->device, lds, fp32);
+    if (nb <= 0) throw ApiError(PFGPU_E_DEVICE, "the device cannot run a cooperative (wide) launch");
+    // a grid barrier costs about 0.1 us per workgroup (one device-scope atomic each): a few dozen
+    // workgroups balance barrier cost against the lanes a wide level needs
+    static const int wide_blocks = [] {
+        const char* e = std::getenv("PFGPU_WIDE_BLOCKS");
+        return e ? std::max(1, std::atoi(e)) : kWideBlocks;
+    }();
+    nb = std::min(nb, wide_blocks);
+    auto& W = gr->wide;
+    if (W.blocks != nb) {
+        const long long lanes = (long long)nb * pfgpu_wide::kThreads;
+        W.f_lvl.upload(P.f_lvl);
+        W.b_lvl.upload(P.b_lvl);
+        W.f_team.upload(level_teams(P.f_lvl, P.tasks, lanes));
+        W.b_team.upload(level_teams(P.b_lvl, P.btasks, lanes));
+        auto nonempty = [](const std::vector<int>& v, size_t k) { return v.empty() ? std::vector<int>(k, 0) : v; };
+        W.tasks.upload(nonempty(P.tasks, 4));
+        W.pairs.upload(nonempty(P.pairs, 4));
+        W.btasks.upload(nonempty(P.btasks, 4));
+        W.bpairs.upload(nonempty(P.bpairs, 2));
+        W.blocks = nb;
+    }
+    size_t off = 0;
+    const size_t o_x = off; off += align_up((size_t)P.nvals * fsize);
+    const size_t o_va = off; off += align_up((size_t)n * 8);
+    const size_t o_vm = off; off += align_up((size_t)n * 8);
+    const size_t o_ib = off; off += align_up((size_t)n * 16);
+    const size_t o_yw = off; off += align_up((size_t)P.nnzY * 16);
+    const size_t o_red = off; off += align_up((size_t)4 * nb * 8);
+    const size_t o_ctl = off; off += align_up(64 * sizeof(int));
+    char* ws = static_cast<char*>(ws_get(ctx, off));
+    char* aux = static_cast<char*>(ctx->aux);
+    hipStream_t stream = ctx->stream;
+    const bool copy = !(q.V0 == q.V && q.v0_stride != 0 && q.chain == 1);
+    pfgpu_wide::WArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = n; a.N = P.N; a.nJ = P.nJ; a.nzero = (int)P.zero_slots.size(); a.nlev = P.nlev; a.nblev = P.nblev; a.m = P.m;
+    a.single_slack = gr->info.single_slack; a.max_iter = q.max_iter; a.chain = q.chain; a.fresh = fresh ? 1 : 0;
+    a.nblocks = nb; a.nunits = nunits; a.nrows = q.nrows; a.tol = q.tol; a.stall_tol = std::sqrt(q.tol);
+    a.yr_ptr = gr->yr_ptr.p; a.yr_col = gr->yr_col.p; a.yr_pos = gr->yr_pos.p;
+    a.fslot_p = gr->fslot_p.p; a.fslot_q = gr->fslot_q.p;
+    a.bus_dx = reinterpret_cast<const int2*>(gr->bus_dx.p); a.slack_dx = gr->slack_dx;
+    a.j_ypos = gr->j_ypos.p; a.j_rbus = gr->j_rbus.p; a.j_cbus = gr->j_cbus.p; a.j_slot = gr->j_slot.p;
+    a.zero_slots = gr->zero_slots.p; a.j_sel = gr->j_sel.p; a.j_cval = gr->j_cval.p;
+    a.f_lvl = W.f_lvl.p; a.tasks = reinterpret_cast<const int4*>(W.tasks.p); a.pairs = reinterpret_cast<const int4*>(W.pairs.p);
+    a.f_team = W.f_team.p;
+    a.b_lvl = W.b_lvl.p; a.btasks = reinterpret_cast<const int4*>(W.btasks.p);
+    a.bpairs = reinterpret_cast<const int2*>(W.bpairs.p); a.b_team = W.b_team.p;
+    a.w = gr->w.p;
+    a.Yx = q.Yx; a.y_stride = (long long)q.Yx_stride * 2;
+    a.dptr = q.dptr; a.dpos = q.dpos; a.dval = q.dval;
+    a.S = q.S; a.s_stride = (long long)q.sbus_stride * 2;
+    a.V = q.V; a.iters = q.iters; a.status = q.status; a.flags = reinterpret_cast<int*>(aux + AL.o_flags);
+    a.imap = imap;
+    a.X = ws + o_x;
+    a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
+    a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Yw = reinterpret_cast<double*>(ws + o_yw);
+    a.red = reinterpret_cast<double*>(ws + o_red); a.ctl = reinterpret_cast<int*>(ws + o_ctl);
+    HIP_OK(hipEventRecord(ctx->ev0, stream));
+    hipLaunchKernelGGL(k_init_v, dim3(copy ? 1024 : 1), dim3(256), 0, stream, q.V0, (long long)q.v0_stride * 2, q.V, n,
+                       nunits, imap, q.chain, reinterpret_cast<int*>(aux + AL.o_queue), nullptr, copy ? 1 : 0);
+    HIP_OK(hipGetLastError());
+    HIP_OK(pfgpu_wide::launch(a, fp32, lds, stream));
+    HIP_OK(hipEventRecord(ctx->ev1, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    if (st) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        st->kernel_s += ms * 1e-3;
+        st->instances_per_wg = -1;
+        st->block_size = pfgpu_wide::kThreads;
+        st->lds_slots = 0;
+        st->workgroups = nb;
+        st->resident_workgroups = nb;
+    }
+}
+
 // One NR launch (k_init_v, k_gather_y for a shared Ybus, k_nr) over `nunits` launch units;
 // imap (device, or null) maps them to the caller's instances / chains. With a fixed workspace the
 // launch allocates nothing: the schedule of its launch shape must already be on the device, and
@@ -1738,10 +1859,15 @@ struct FixedScratch {
 void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precision, int G_req, int NT_req,
                  int nunits, const int* imap, const AuxLayout& AL, bool fresh, pfgpu_stats* st,
                  const FixedScratch* fx = nullptr) {
+    if (wide_wanted(gr, q, G_req, nunits, fx)) {
+        launch_wide(ctx, gr, q, precision, nunits, imap, AL, fresh, st);
+        return;
+    }
     const pfgpu::DevicePlan& P = gr->P;
     const int n = P.n;
     const int fsize = precision == PFGPU_FP64 ? 8 : 4;
     const int pslot = precision == PFGPU_FP64 ? 0 : 1;
+    if (G_req == -1) G_req = 0;  // a wide launch was not possible here (DC grid, J capture, fixed scratch)
     if (G_req == 0 && NT_req == 0 && gr->tuned_G[pslot]) {
         G_req = gr->tuned_G[pslot];
         NT_req = gr->tuned_NT[pslot];
