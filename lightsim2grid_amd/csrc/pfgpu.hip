@@ -1804,6 +1804,7 @@ void launch_wide(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     const size_t o_yw = off; off += align_up((size_t)P.nnzY * 16);
     const size_t o_red = off; off += align_up((size_t)4 * nb * 8);
     const size_t o_ctl = off; off += align_up(64 * sizeof(int));
+    const size_t o_bar = off; off += align_up(pfgpu_wide::kBarrierWords * sizeof(unsigned int));
     char* ws = static_cast<char*>(ws_get(ctx, off));
     char* aux = static_cast<char*>(ctx->aux);
     hipStream_t stream = ctx->stream;
@@ -1832,6 +1833,7 @@ void launch_wide(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     a.Va = reinterpret_cast<double*>(ws + o_va); a.Vm = reinterpret_cast<double*>(ws + o_vm);
     a.Ib = reinterpret_cast<double*>(ws + o_ib); a.Yw = reinterpret_cast<double*>(ws + o_yw);
     a.red = reinterpret_cast<double*>(ws + o_red); a.ctl = reinterpret_cast<int*>(ws + o_ctl);
+    a.bar = reinterpret_cast<unsigned int*>(ws + o_bar);
     HIP_OK(hipEventRecord(ctx->ev0, stream));
     hipLaunchKernelGGL(k_init_v, dim3(copy ? 1024 : 1), dim3(256), 0, stream, q.V0, (long long)q.v0_stride * 2, q.V, n,
                        nunits, imap, q.chain, reinterpret_cast<int*>(aux + AL.o_queue), nullptr, copy ? 1 : 0);

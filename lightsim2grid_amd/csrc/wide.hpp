@@ -5,7 +5,8 @@
 // compute unit. The wide kernel spreads each phase of one NR iteration over all resident
 // workgroups instead: the J fill, the mismatch and the voltage update by entry / bus, and the
 // factorisation and the backward solve level by level, each level's tasks split across the whole
-// grid (a team of lanes per task). Grid-wide barriers (cooperative launch) separate the levels.
+// grid (a team of lanes per task). Grid-wide barriers separate the levels (a cooperative launch
+// keeps every workgroup resident).
 // The dense tail of the factor (the top chain of the elimination tree) runs on workgroup 0.
 // The value array lives in global memory (one per launch, reused by the instances in turn).
 // Reference: the same per-iteration phases as BaseNRAlgo::compute_pf (BaseNRAlgo.tpp:106-167).
@@ -60,7 +61,10 @@ struct WArgs {
     void* X;
     double *Va, *Vm, *Ib, *Yw, *red;
     int* ctl;
+    unsigned int* bar;  // kBarrierWords words of grid-barrier state (cleared by launch())
 };
+
+constexpr int kBarrierWords = 10 * 32;
 
 // Launches the wide kernel (cooperative launch) for FT = float (fp32 != 0) or double.
 hipError_t launch(const WArgs& a, bool fp32, size_t lds_bytes, hipStream_t stream);
