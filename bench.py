@@ -315,11 +315,17 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
         out[nm] = {"wall_s": wall, "kernel_ms": 1e3 * st["kernel_s"], "converged": conv, "solves_per_s": conv / wall,
                    "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"],
                    "fallback_fp64": st["fallback_fp64"], "resident_workgroups": st["resident_workgroups"]}
-        lat = []
-        for _ in range(5):
-            r = gr.solve(Yx, d["Sbus"], Vb, 10, 1e-8, precision=prec, deltas=(dptr[:2], dpos[:dptr[1]], dval[:dptr[1]]))
-            lat.append(r["stats"]["kernel_s"])
-        out[nm]["single_solve_kernel_ms"] = 1e3 * float(np.median(lat))
+        # one solve: the default launch (a wide launch: every workgroup on the one instance, level
+        # by level) and the batched kernel's one workgroup per instance
+        one = (dptr[:2], dpos[:dptr[1]], dval[:dptr[1]])
+        for G, key in ((0, "single_solve_kernel_ms"), (1, "single_solve_one_workgroup_kernel_ms")):
+            lat = []
+            for _ in range(5):
+                r = gr.solve(Yx, d["Sbus"], Vb, 10, 1e-8, precision=prec, deltas=one, instances_per_wg=G)
+                lat.append(r["stats"]["kernel_s"])
+            out[nm][key] = 1e3 * float(np.median(lat))
+            if G == 0:
+                out[nm]["single_solve_launch"] = "wide" if r["stats"]["instances_per_wg"] == -1 else "batched"
     if with_cpu:
         from oracle import oracle
         k = 16 * nthreads
@@ -401,6 +407,11 @@ def main():
         iters = dIt.to_host()
         status = dSt.to_host()
         gpu_rows[name] = (dV.to_host(), iters, status)
+        # the same sweep with the work queue in index order (no iteration counts of an earlier
+        # sweep to take the longest solves first), untimed by the rate above
+        gr.reset()
+        first = sweep(prec)
+        assert np.array_equal(dIt.to_host(), iters) and np.array_equal(dSt.to_host(), status)
         conv = status == 0
         wall = t1 - t0
         w = 4 if prec == abi.FP32 else 8
@@ -413,6 +424,7 @@ def main():
             lds_slots=st["lds_slots"], instances_per_wg=st["instances_per_wg"], block_size=st["block_size"],
             resident_workgroups=st["resident_workgroups"], fallback_fp64_per_step=fb / args.steps,
             status_hist=np.bincount(status, minlength=9).tolist(), autotune=tuned.get(name),
+            kernel_ms_index_order=1e3 * first["kernel_s"],
             phase_ms_wg0={k: 1e3 * v for k, v in st["phase_s"].items()})
 
     cinfo = cpu_info()

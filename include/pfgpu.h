@@ -146,7 +146,11 @@ int pfgpu_grid_destroy(pfgpu_grid* grid);
  * a plan with these pivots (pfgpu_stats.repivot_rows) and keeps that plan for the grid's later
  * solves, as klu_refactor keeps klu_factor's pivots, until this call drops it. A numerically
  * singular J keeps SolverFactor. PFGPU_FP32_STRICT and pfgpu_solve_batch_device_noalloc do not
- * re-pivot. */
+ * re-pivot.
+ * Work-queue order: a batch of at least 2 x (compute units) instances (one per row, no chains)
+ * takes its instances longest first, by their iteration counts in the previous batch of the same
+ * size on this grid (index order for the first one and after this call). The results do not
+ * depend on the order. */
 int pfgpu_grid_reset(pfgpu_grid* grid);
 int pfgpu_grid_get_info(const pfgpu_grid* grid, pfgpu_grid_info* info);
 /* J pattern in the reference layout: Jp[n_jac+1], Ji[nnz_jac]. */

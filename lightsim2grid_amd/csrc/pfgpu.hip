@@ -1367,6 +1367,43 @@ __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, co
     if (tid == 0) *count = base;
 }
 
+// Work-queue order for the next launch of a batch of the same size: the instances by decreasing
+// iteration count of this launch (longest first), ties in index order. Taking the longest solves
+// first shortens the end of a launch, where a few long solves (diverging outages run max_iter
+// iterations) would otherwise start late and leave the other workgroups idle. One workgroup of
+// 1024 lanes, one compaction pass per iteration count.
+__global__ void __launch_bounds__(1024) k_order_by_iters(const int* iters, long long n, int nkeys, int* perm) {
+    __shared__ int wsum[16];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int k = nkeys - 1; k >= 0; --k) {
+        for (long long u0 = 0; u0 < n; u0 += 1024) {
+            const long long u = u0 + tid;
+            bool take = false;
+            if (u < n) {
+                const int it = iters[u];
+                take = (it >= nkeys - 1 ? nkeys - 1 : (it < 0 ? 0 : it)) == k;
+            }
+            const unsigned long long m = __ballot(take);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[wave] = __popcll(m);
+            __syncthreads();
+            int off = base;
+            for (int w = 0; w < wave; ++w) off += wsum[w];
+            if (take) perm[off + before] = (int)u;
+            __syncthreads();
+            if (tid == 0) {
+                int t = 0;
+                for (int w = 0; w < 16; ++w) t += wsum[w];
+                base += t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Branch flows at the origin side, one (instance, branch) per thread.
 __global__ void k_branch_flows(int ninst, int n, const double* __restrict__ V, int nbr, const int* __restrict__ f,
                                const int* __restrict__ t, const double* __restrict__ yff,
@@ -1458,6 +1495,9 @@ struct pfgpu_grid {
     std::unique_ptr<pfgpu_grid> alt;
     bool alt_active = false;
     DevBuf<double> jx_probe;  // J values of the instance the re-pivot chooses its pivots on
+    // work-queue order of the next batch of order_n instances (k_order_by_iters of the last one)
+    DevBuf<int> order;
+    long long order_n = 0;
     // Wide launches (wide.hpp): the plan's tasks and pairs as flat tables and the per-level team
     // sizes for `blocks` workgroups (uploaded by the first wide launch)
     struct Wide {
@@ -2091,7 +2131,19 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     }
     const bool fresh = gr->fresh;
     pfgpu_grid* pg = gr->alt_active ? gr->alt.get() : gr;  // the plan in use
-    launch_pass(ctx, pg, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, nullptr, AL, fresh, st, fx);
+    // large batches take their instances longest first, by the iterations of the previous batch of
+    // the same size (any order gives the same results: instances are independent)
+    const bool ordered = !fx && q.chain == 1 && q.ninst >= 2LL * std::max(1, ctx->n_cu) &&
+                         !wide_wanted(pg, q, q.G_req, q.ninst, fx);
+    const int* qorder = ordered && gr->order_n == q.ninst ? gr->order.p : nullptr;
+    launch_pass(ctx, pg, q, fp32 ? PFGPU_FP32 : PFGPU_FP64, q.G_req, q.NT_req, q.ninst, qorder, AL, fresh, st, fx);
+    if (ordered) {
+        int* perm = gr->order.get_or_alloc((size_t)q.ninst);
+        hipLaunchKernelGGL(k_order_by_iters, dim3(1), dim3(1024), 0, ctx->stream, q.iters, (long long)q.ninst,
+                           std::min(std::max(q.max_iter, 0), 62) + 1, perm);
+        HIP_OK(hipGetLastError());
+        gr->order_n = q.ninst;
+    }
     gr->fresh = false;
     ctx->last_rows = q.nrows;
     ctx->last_flags = reinterpret_cast<const int*>(aux + AL.o_flags);
@@ -2413,6 +2465,7 @@ int pfgpu_grid_reset(pfgpu_grid* grid) {
         grid->fresh = true;
         grid->alt_active = false;
         grid->alt.reset();
+        grid->order_n = 0;  // and the next batch takes its instances in index order
         return 0;
     });
 }
