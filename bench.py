@@ -339,6 +339,30 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
     return out
 
 
+def single_solves(name: str, reps: int = 5) -> dict:
+    """One solve of a synthetic grid from its V0, reps times per form: the default launch (a wide
+    launch, every workgroup on the one instance, for a large plan) and one workgroup per instance."""
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make(name)
+    d = g.solver_inputs()
+    ctx = abi.Context(0)
+    gr = abi.Grid(ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    out = {"workload": f"{name}, one solve from V0, tol 1e-8, max 10 iterations"}
+    for prec, nm in ((abi.FP64, "fp64"), (abi.FP32_STRICT, "fp32_strict")):
+        for G, key in ((0, "default"), (1, "one_workgroup")):
+            ms = []
+            for _ in range(reps):
+                r = gr.solve(Yx, d["Sbus"], d["V0"], 10, 1e-8, precision=prec, instances_per_wg=G)
+                ms.append(1e3 * r["stats"]["kernel_s"])
+            out[f"{nm}_{key}"] = {"launch": "wide" if r["stats"]["instances_per_wg"] == -1 else "batched",
+                                  "iters": int(r["iters"][0]), "status": int(r["status"][0]),
+                                  "kernel_ms": ms}
+    gr.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +375,9 @@ def main():
     ap.add_argument("--instances-per-wg", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
+    ap.add_argument("--single-solve", metavar="GRID", default=None,
+                    help="only time single solves of GRID from its V0 (default launch, then one workgroup) "
+                         "and print one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the N-1 sweep would shard by "
@@ -358,6 +385,10 @@ def main():
         sys.exit(2)
 
     from lightsim2grid_amd import abi, gridgen
+
+    if args.single_solve:
+        print(json.dumps(single_solves(args.single_solve)))
+        return
 
     g = gridgen.make(args.grid)
     d = g.solver_inputs()
