@@ -252,7 +252,7 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
     load_p, load_q = -Sb[:, g.pq].real, -Sb[:, g.pq].imag
     out = {"workload": "case118_synth (1 slack, 53 PV), %d hourly steps, warm start from the previous step" % T}
     for chain, lname in ((0, "single_chain"), (8, "chains_of_8")):
-        res = {}
+        res, gpu_V = {}, {}
         for st, nm in ((L.SolverType.AMDGPUMixed, "fp32"), (L.SolverType.AMDGPU, "fp64")):
             ts = L.TimeSeriesCPP(gridgen.to_gridmodel(g))
             ts.change_solver(st)
@@ -263,15 +263,24 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
             wall = time.perf_counter() - t0
             res[nm] = {"status": int(status), "steps": T, "wall_s": wall, "solves_per_s": T / wall,
                        "solver_time_s": ts.solver_time()}
+            gpu_V[nm] = np.asarray(ts.get_voltages())
         if with_cpu:
             from oracle import oracle
             d = g.solver_inputs()
             nch = 1 if chain == 0 else (T + chain - 1) // chain
-            _, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"],
-                                                             d["pv"], d["pq"], g.v0(), 10, 1e-8, False, nch,
-                                                             1 if nch == 1 else nthreads)
+            Vr, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"],
+                                                              d["pv"], d["pq"], g.v0(), 10, 1e-8, False, nch,
+                                                              1 if nch == 1 else nthreads)
             res["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": 1 if nch == 1 else nthreads,
                                    "kind": "port", "wall_s": wall}
+            # every step against the oracle on the same chain layout (TimeSeriesCPP keeps no
+            # per-step iteration counts: flags and voltages only)
+            res["parity"] = {}
+            for nm, Vg in gpu_V.items():
+                ok = np.all(np.isfinite(Vg), axis=1) & (np.abs(Vg).sum(axis=1) > 0)
+                both = ok & conv
+                res["parity"][nm] = {"flags_equal": bool(np.array_equal(ok, conv)),
+                                     "max_dV": float(np.max(np.abs(Vg[both] - Vr[both]))) if both.any() else None}
         out[lname] = res
     return out
 
@@ -330,10 +339,16 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
         from oracle import oracle
         k = 16 * nthreads
         sub = (dptr[:k + 1], dpos[:dptr[k]], dval[:dptr[k]])
-        _, _, _, conv, _, wall = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"],
-                                                    d["pq"], Vb, sub, 10, 1e-8, False, False, nthreads)
+        Vr, itr, _, conv, _, wall = oracle.contingency(d["Ybus"], d["Sbus"], d["slack_ids"], d["slack_weights"],
+                                                       d["pv"], d["pq"], Vb, sub, 10, 1e-8, False, False, nthreads)
         out["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": nthreads, "kind": "port",
                                "sample": "%d outage solves (base injections), %.2f s" % (k, wall)}
+        # the same sample on the GPU, both forms, against the oracle
+        out["parity"] = {}
+        for prec, nm in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+            r = gr.solve(Yx, d["Sbus"], Vb, 10, 1e-8, precision=prec, deltas=sub)
+            out["parity"][nm] = parity(r["V"], r["iters"], r["status"], Vr, itr, conv)
+            out["parity"][nm]["sample"] = "the %d outages of the CPU sample" % k
     gr.close()
     ctx.close()
     return out
