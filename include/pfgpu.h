@@ -81,6 +81,7 @@ typedef struct pfgpu_grid_info {
     double flops;       /* flops of one refactorisation */
     int single_slack;
     int dc;             /* loaded with pfgpu_grid_load_dc */
+    int fdpf;           /* loaded with pfgpu_grid_load_fdpf */
 } pfgpu_grid_info;
 
 /* Per-call timings (seconds) and launch choices. */
@@ -152,6 +153,20 @@ int pfgpu_grid_destroy(pfgpu_grid* grid);
  * size on this grid (index order for the first one and after this call). The results do not
  * depend on the order. */
 int pfgpu_grid_reset(pfgpu_grid* grid);
+
+/* A fast-decoupled grid (BaseFDPFAlgo<LinearSolver, XB_BX>::compute_pf, BaseFDPFAlgo.tpp:12-160;
+ * the reference builds grid_Bp / grid_Bpp with GridModel::fillBp_Bpp, GridModel.cpp:1037-1060):
+ * Bp_* / Bpp_* are the n x n matrices B' and B'' (CSC, any row order, duplicates summed). The
+ * grid takes my_pv = slack_ids[1:] followed by pv as angle unknowns (retrieve_pv_with_slack) and
+ * factors the block-diagonal [B'[pvpq, pvpq] 0; 0 B''[pq, pq]] with the NR kernels' factor and
+ * solve phases. A solve alternates the P half-step (Va[pvpq] -= B'^-1 P) and the Q half-step
+ * (Vm[pq] -= B''^-1 Q), P and Q being the mismatch (slack_absorbed = Re(sum Sbus) kept fixed)
+ * divided by |V|, with the convergence test (max |P|, |Q| < tol) after each; iters counts P
+ * half-steps, max_iter bounds them. Solved with PFGPU_FP64 only. */
+int pfgpu_grid_load_fdpf(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+                         const double* slack_weights, const int* pv, int npv, const int* pq, int npq, const int* Bp_p,
+                         const int* Bp_i, const double* Bp_x, const int* Bpp_p, const int* Bpp_i, const double* Bpp_x,
+                         pfgpu_grid** out);
 int pfgpu_grid_get_info(const pfgpu_grid* grid, pfgpu_grid_info* info);
 /* J pattern in the reference layout: Jp[n_jac+1], Ji[nnz_jac]. */
 int pfgpu_grid_get_J_pattern(const pfgpu_grid* grid, int* Jp, int* Ji);

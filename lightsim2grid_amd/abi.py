@@ -35,7 +35,8 @@ class GridInfo(ctypes.Structure):
     _fields_ = [("n_bus", ctypes.c_int), ("n_jac", ctypes.c_int), ("nnz_ybus", ctypes.c_int),
                 ("nnz_jac", ctypes.c_int), ("nnz_lu", ctypes.c_int64), ("n_slots", ctypes.c_int),
                 ("n_levels", ctypes.c_int), ("n_back_levels", ctypes.c_int), ("n_pairs", ctypes.c_int64),
-                ("flops", ctypes.c_double), ("single_slack", ctypes.c_int), ("dc", ctypes.c_int)]
+                ("flops", ctypes.c_double), ("single_slack", ctypes.c_int), ("dc", ctypes.c_int),
+                ("fdpf", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +75,9 @@ SIGNATURES = {
                                        ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "pfgpu_grid_load_dc": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, _i32p, ctypes.c_int,
                                           _i32p, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "pfgpu_grid_load_fdpf": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, _f64p, _i32p,
+                                            ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p, _f64p, _i32p, _i32p,
+                                            _f64p, ctypes.POINTER(_vp)]),
     "pfgpu_grid_destroy": (ctypes.c_int, [_vp]),
     "pfgpu_grid_reset": (ctypes.c_int, [_vp]),
     "pfgpu_grid_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(GridInfo)]),
@@ -257,10 +261,11 @@ def _csc(Ybus):
 
 class Grid:
     """A loaded grid (pfgpu_grid): J pattern, ordering, symbolic LU and schedule on the device.
-    dc=True loads the DC power flow of the same buses (pfgpu_grid_load_dc; slack_weights unused)."""
+    dc=True loads the DC power flow of the same buses (pfgpu_grid_load_dc; slack_weights unused);
+    fdpf=(Bp, Bpp) the fast-decoupled power flow with these n x n matrices (pfgpu_grid_load_fdpf)."""
 
     def __init__(self, ctx: Context, Ybus, slack_ids, slack_weights, pv, pq, single_slack: bool = False,
-                 dc: bool = False):
+                 dc: bool = False, fdpf=None):
         self.ctx = ctx
         Y = _csc(Ybus)
         self.n = Y.shape[0]
@@ -273,6 +278,12 @@ class Grid:
         if dc:
             _check(ctx._L.pfgpu_grid_load_dc(ctx.h, self.n, pYp, pYi, psl, len(sl), ppv, len(pv_), ppq, len(pq_),
                                              ctypes.byref(h)))
+        elif fdpf is not None:
+            w, pw = _d(slack_weights)
+            Bp, Bpp = _csc(fdpf[0]), _csc(fdpf[1])
+            keep = [_i(Bp.indptr), _i(Bp.indices), _d(Bp.data), _i(Bpp.indptr), _i(Bpp.indices), _d(Bpp.data)]
+            _check(ctx._L.pfgpu_grid_load_fdpf(ctx.h, self.n, pYp, pYi, psl, len(sl), pw, ppv, len(pv_), ppq,
+                                               len(pq_), *[k[1] for k in keep], ctypes.byref(h)))
         else:
             w, pw = _d(slack_weights)
             _check(ctx._L.pfgpu_grid_load(ctx.h, self.n, pYp, pYi, psl, len(sl), pw, ppv, len(pv_), ppq, len(pq_),

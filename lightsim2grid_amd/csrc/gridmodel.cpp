@@ -23,14 +23,20 @@ std::string solver_type_name(SolverType t) {
         "KLUSingleSlack", "NICSLUSingleSlack", "KLUDC", "NICSLUDC", "CKTSO", "CKTSOSingleSlack", "CKTSODC",
         "FDPF_XB_SparseLU", "FDPF_BX_SparseLU", "FDPF_XB_KLU", "FDPF_BX_KLU", "FDPF_XB_NICSLU", "FDPF_BX_NICSLU",
         "FDPF_XB_CKTSO", "FDPF_BX_CKTSO", "AMDGPU", "AMDGPUSingleSlack", "AMDGPUMixed", "AMDGPUMixedSingleSlack",
-        "AMDGPUDC"};
+        "AMDGPUDC", "AMDGPUFDPF_XB", "AMDGPUFDPF_BX"};
     const int k = (int)t;
     return (k >= 0 && k < (int)(sizeof(names) / sizeof(names[0]))) ? names[k] : "Unknown";
 }
 
 bool is_gpu_type(SolverType t) {
     return t == SolverType::AMDGPU || t == SolverType::AMDGPUSingleSlack || t == SolverType::AMDGPUMixed ||
-           t == SolverType::AMDGPUMixedSingleSlack || t == SolverType::AMDGPUDC;
+           t == SolverType::AMDGPUMixedSingleSlack || t == SolverType::AMDGPUDC || t == SolverType::AMDGPUFDPF_XB ||
+           t == SolverType::AMDGPUFDPF_BX;
+}
+int fdpf_kind(SolverType t) {
+    if (t == SolverType::AMDGPUFDPF_XB) return 1 + (int)FDPFMethod::XB;
+    if (t == SolverType::AMDGPUFDPF_BX) return 1 + (int)FDPFMethod::BX;
+    return 0;
 }
 // ChooseSolver::is_dc (ChooseSolver.hpp:88-97)
 bool is_dc_type(SolverType t) {
@@ -106,14 +112,24 @@ RealVect GpuNRSolver::get_Vm() const {
 
 void GpuNRSolver::ensure_grid(const CscC& Y, const IntVect& slack_ids, const RealVect& w, const IntVect& pv,
                               const IntVect& pq) {
+    if (fdpf_ && (Bp_.ncol != Y.ncol || Bpp_.ncol != Y.ncol))
+        throw std::runtime_error("BaseFDPFAlgo::compute_pf: B' and B'' are built by GridModel::ac_pf (fillBp_Bpp); "
+                                 "this solver needs a GridModel");
+    auto same = [](const CscR& a, const CscR& b) { return a.p == b.p && a.i == b.i && a.x == b.x; };
     if (grid_ && Y.p == key_p_ && Y.i == key_i_ && slack_ids == key_slack_ && pv == key_pv_ && pq == key_pq_ &&
-        w == key_w_)
+        w == key_w_ && (!fdpf_ || (same(Bp_, key_Bp_) && same(Bpp_, key_Bpp_))))
         return;
     auto t0 = clk::now();
     if (grid_) pfgpu_grid_destroy(grid_);
     grid_ = nullptr;
     pfgpu_grid* g = nullptr;
-    if (dc_)
+    if (fdpf_)
+        check_rc(pfgpu_grid_load_fdpf(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(),
+                                      (int)slack_ids.size(), w.data(), pv.data(), (int)pv.size(), pq.data(),
+                                      (int)pq.size(), Bp_.p.data(), Bp_.i.data(), Bp_.x.data(), Bpp_.p.data(),
+                                      Bpp_.i.data(), Bpp_.x.data(), &g),
+                 "pfgpu_grid_load_fdpf");
+    else if (dc_)
         check_rc(pfgpu_grid_load_dc(engine_ctx(), Y.ncol, Y.p.data(), Y.i.data(), slack_ids.data(),
                                     (int)slack_ids.size(), pv.data(), (int)pv.size(), pq.data(), (int)pq.size(), &g),
                  "pfgpu_grid_load_dc");
@@ -124,6 +140,8 @@ void GpuNRSolver::ensure_grid(const CscC& Y, const IntVect& slack_ids, const Rea
     grid_ = g;
     check_rc(pfgpu_grid_get_info(grid_, &info_), "pfgpu_grid_get_info");
     key_p_ = Y.p; key_i_ = Y.i; key_slack_ = slack_ids; key_pv_ = pv; key_pq_ = pq; key_w_ = w;
+    key_Bp_ = Bp_;
+    key_Bpp_ = Bpp_;
     timer_initialize_ += since(t0);
 }
 
@@ -240,11 +258,14 @@ ChooseSolver::ChooseSolver() : type_(SolverType::AMDGPU) {
     s_[2] = std::make_unique<GpuNRSolver>(false, PFGPU_FP32);
     s_[3] = std::make_unique<GpuNRSolver>(true, PFGPU_FP32);
     s_[4] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64, true);
+    s_[5] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64, false, 1 + (int)FDPFMethod::XB);
+    s_[6] = std::make_unique<GpuNRSolver>(true, PFGPU_FP64, false, 1 + (int)FDPFMethod::BX);
 }
 
 std::vector<SolverType> ChooseSolver::available_solvers() const {
     return {SolverType::AMDGPU, SolverType::AMDGPUSingleSlack, SolverType::AMDGPUMixed,
-            SolverType::AMDGPUMixedSingleSlack, SolverType::AMDGPUDC};
+            SolverType::AMDGPUMixedSingleSlack, SolverType::AMDGPUDC, SolverType::AMDGPUFDPF_XB,
+            SolverType::AMDGPUFDPF_BX};
 }
 
 void ChooseSolver::change_solver(SolverType t) {
@@ -254,7 +275,7 @@ void ChooseSolver::change_solver(SolverType t) {
         std::ostringstream os;
         os << "Impossible to use the solver " << solver_type_name(t)
            << ": it is not available in this build (available: AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, "
-              "AMDGPUMixedSingleSlack, AMDGPUDC).";
+              "AMDGPUMixedSingleSlack, AMDGPUDC, AMDGPUFDPF_XB, AMDGPUFDPF_BX).";
         throw std::runtime_error(os.str());
     }
     current().reset();
@@ -453,8 +474,59 @@ CplxVect GridModel::dc_pf(const CplxVect& Vinit, int max_iter, double tol) {
     return res;
 }
 
+// GridModel::fillBp_Bpp for this subset (GridModel.cpp:1037-1060): per connected line,
+// TwoSidesContainer_rxh_A::get_fdpf_coeffs (TwoSidesContainer_rxh_A.hpp:818-843) with ys_bp = 1/(jx)
+// (XB) or 1/(r + jx) (BX) and ys_bpp the other one, the charging split half and half between the
+// ends (LineContainer.cpp:13-34), placed as fillBp_Bpp does (:503-598): B' gets -Im(ys_bp) on both
+// diagonals and +Im(ys_bp) off them, B'' -(Im(ys_bpp) + Im(h)/2) on the diagonals and +Im(ys_bpp)
+// off them. Duplicates are summed and rows sorted (setFromTriplets).
+void GridModel::fill_Bp_Bpp(const SolverInput& in, FDPFMethod method, CscR& Bp, CscR& Bpp) const {
+    const int n = (int)in.id_solver_to_me.size();
+    std::vector<std::vector<std::pair<int, double>>> cp(n), cpp(n);
+    for (size_t k = 0; k < line_r_.size(); ++k) {
+        if (!line_status_[k]) continue;
+        const int f = in.id_me_to_solver[line_from_[k]], t = in.id_me_to_solver[line_to_[k]];
+        if (f < 0 || t < 0) continue;
+        const double r = line_r_[k], x = line_x_[k];
+        const double y_jx = std::imag(1.0 / cplx(0.0, x)), y_rx = std::imag(1.0 / cplx(r, x));
+        const double ybp = method == FDPFMethod::XB ? y_jx : y_rx;
+        const double ybpp = method == FDPFMethod::XB ? y_rx : y_jx;
+        const double half = 0.5 * line_h_[k].imag();
+        cp[f].push_back({f, -ybp});
+        cp[t].push_back({t, -ybp});
+        cp[t].push_back({f, ybp});
+        cp[f].push_back({t, ybp});
+        cpp[f].push_back({f, -(ybpp + half)});
+        cpp[t].push_back({t, -(ybpp + half)});
+        cpp[t].push_back({f, ybpp});
+        cpp[f].push_back({t, ybpp});
+    }
+    auto build = [n](std::vector<std::vector<std::pair<int, double>>>& cols, CscR& M) {
+        M.nrow = M.ncol = n;
+        M.p.assign(n + 1, 0);
+        M.i.clear();
+        M.x.clear();
+        for (int c = 0; c < n; ++c) {
+            auto& v = cols[c];
+            std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (size_t q = 0; q < v.size(); ++q) {
+                if ((int)M.i.size() > M.p[c] && M.i.back() == v[q].first) M.x.back() += v[q].second;
+                else { M.i.push_back(v[q].first); M.x.push_back(v[q].second); }
+            }
+            M.p[c + 1] = (int)M.i.size();
+        }
+    };
+    build(cp, Bp);
+    build(cpp, Bpp);
+}
+
 CplxVect GridModel::ac_pf(const CplxVect& Vinit, int max_iter, double tol) {
     last_ = pre_process_solver(Vinit);
+    if (const int kind = fdpf_kind(solver_->get_type())) {
+        CscR Bp, Bpp;
+        fill_Bp_Bpp(last_, (FDPFMethod)(kind - 1), Bp, Bpp);
+        solver_->current().set_fdpf_matrices(std::move(Bp), std::move(Bpp));
+    }
     CplxVect V = last_.V;
     const bool conv = solver_->compute_pf(last_.Ybus, V, last_.Sbus, last_.slack_ids, last_.slack_weights, last_.pv,
                                           last_.pq, max_iter, tol / sn_mva_);

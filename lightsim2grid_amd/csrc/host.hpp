@@ -40,9 +40,12 @@ enum class SolverType : int {
     FDPF_XB_SparseLU, FDPF_BX_SparseLU, FDPF_XB_KLU, FDPF_BX_KLU,
     FDPF_XB_NICSLU, FDPF_BX_NICSLU, FDPF_XB_CKTSO, FDPF_BX_CKTSO,
     // MI355X batched Newton-Raphson (distributed / single slack; FP64 or FP32 factors), and the DC
-    // power flow on the same factor / solve kernels
-    AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, AMDGPUMixedSingleSlack, AMDGPUDC
+    // and fast-decoupled (XB, BX) power flows on the same factor / solve kernels
+    AMDGPU, AMDGPUSingleSlack, AMDGPUMixed, AMDGPUMixedSingleSlack, AMDGPUDC, AMDGPUFDPF_XB, AMDGPUFDPF_BX
 };
+
+// FDPFMethod (the reference's enum: Utils.hpp)
+enum class FDPFMethod : int { XB, BX };
 
 enum class ErrorType : int {
     NoError, SingularMatrix, TooManyIterations, InifiniteValue, SolverAnalyze, SolverFactor,
@@ -52,6 +55,8 @@ enum class ErrorType : int {
 std::string solver_type_name(SolverType t);
 bool is_gpu_type(SolverType t);
 bool is_dc_type(SolverType t);
+// 0: not a fast-decoupled type; 1 + FDPFMethod otherwise
+int fdpf_kind(SolverType t);
 
 // CSC matrix as the solvers take it (column pointers, sorted row indices, values)
 struct CscC {
@@ -101,9 +106,11 @@ private:
 // ---------------------------------------------------------------------------
 class GpuNRSolver {
 public:
-    // dc: the DC power flow (BaseDCAlgo) on the same kernels; max_iter and tol are ignored
-    GpuNRSolver(bool single_slack, int precision, bool dc = false)
-        : single_(single_slack), precision_(precision), dc_(dc) {}
+    // dc: the DC power flow (BaseDCAlgo) on the same kernels; max_iter and tol are ignored.
+    // fdpf: 1 + FDPFMethod for the fast-decoupled power flow (BaseFDPFAlgo); its B' and B'' come
+    // from set_fdpf_matrices (GridModel::ac_pf builds them, as BaseFDPFAlgo asks its GridModel)
+    GpuNRSolver(bool single_slack, int precision, bool dc = false, int fdpf = 0)
+        : single_(single_slack), precision_(precision), dc_(dc), fdpf_(fdpf) {}
     ~GpuNRSolver();
     GpuNRSolver(const GpuNRSolver&) = delete;
     GpuNRSolver& operator=(const GpuNRSolver&) = delete;
@@ -152,6 +159,12 @@ public:
     CscR get_J() const;
     bool single_slack() const { return single_; }
     bool is_dc() const { return dc_; }
+    int fdpf() const { return fdpf_; }
+    // B' and B'' (n x n, solver bus order) of the next fast-decoupled solves
+    void set_fdpf_matrices(CscR Bp, CscR Bpp) {
+        Bp_ = std::move(Bp);
+        Bpp_ = std::move(Bpp);
+    }
     int precision() const { return precision_; }
     const pfgpu_grid_info* grid_info() const { return grid_ ? &info_ : nullptr; }
 
@@ -161,6 +174,8 @@ private:
     bool single_;
     int precision_;
     bool dc_ = false;
+    int fdpf_ = 0;
+    CscR Bp_, Bpp_, key_Bp_, key_Bpp_;
     pfgpu_grid* grid_ = nullptr;
     pfgpu_grid_info info_{};
     IntVect key_p_, key_i_, key_slack_, key_pv_, key_pq_;
@@ -191,7 +206,7 @@ public:
 
 private:
     SolverType type_;
-    std::unique_ptr<GpuNRSolver> s_[5];
+    std::unique_ptr<GpuNRSolver> s_[7];
 };
 
 // ---------------------------------------------------------------------------
@@ -233,6 +248,9 @@ public:
     // DC power flow (GridModel::dc_pf, GridModel.cpp:676-759): DC Ybus from the lines' 1 / x
     // stamps, solved by the GPU DC solver; empty when it fails
     CplxVect dc_pf(const CplxVect& Vinit, int max_iter, double tol);
+    // B' and B'' of the fast-decoupled power flow (GridModel::fillBp_Bpp, GridModel.cpp:1037-1060),
+    // n x n in the solver bus order of `in`
+    void fill_Bp_Bpp(const SolverInput& in, FDPFMethod method, CscR& Bp, CscR& Bpp) const;
     GpuNRSolver& get_dc_solver() { return *dc_solver_; }
 
     int total_bus() const { return (int)bus_vn_kv_.size(); }

@@ -72,6 +72,12 @@ py::object cscc_to_py(const CscC& Y) {
                                   py::arg("shape") = py::make_tuple(Y.nrow, Y.ncol));
 }
 
+// the solver bus order of the last power flow, or of a flat start when none ran yet
+SolverInput bus_order_of(const GridModel& g) {
+    if (!g.last_solver_input().id_solver_to_me.empty()) return g.last_solver_input();
+    return g.pre_process_solver(CplxVect(g.total_bus(), cplx(1.0, 0.0)));
+}
+
 template <bool SINGLE, int PREC, bool DC = false>
 struct SolverT : GpuNRSolver {
     SolverT() : GpuNRSolver(SINGLE, PREC, DC) {}
@@ -118,9 +124,14 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
     m.attr("version") = py::str("0.1.0");
 
     py::enum_<SolverType> st(m, "SolverType", "This enum controls the solver you want to use.");
-    for (int k = 0; k <= (int)SolverType::AMDGPUDC; ++k)
+    for (int k = 0; k <= (int)SolverType::AMDGPUFDPF_BX; ++k)
         st.value(solver_type_name((SolverType)k).c_str(), (SolverType)k);
     st.export_values();
+
+    py::enum_<FDPFMethod>(m, "FDPFMethod", "Fast-decoupled variant (XB or BX).")
+        .value("XB", FDPFMethod::XB)
+        .value("BX", FDPFMethod::BX)
+        .export_values();
 
     py::enum_<ErrorType>(m, "ErrorType", "This enum controls the error encountered in the solver")
         .value("NoError", ErrorType::NoError)
@@ -210,6 +221,16 @@ PYBIND11_MODULE(lightsim2grid_amd_cpp, m) {
             }
             return to_np(r);
         })
+        .def("debug_get_Bp_python", [](const GridModel& g, FDPFMethod m) {
+            CscR Bp, Bpp;
+            g.fill_Bp_Bpp(bus_order_of(g), m, Bp, Bpp);
+            return csc_to_py(Bp);
+        }, "B' of the fast-decoupled power flow in the solver bus order (fillBp_Bpp).")
+        .def("debug_get_Bpp_python", [](const GridModel& g, FDPFMethod m) {
+            CscR Bp, Bpp;
+            g.fill_Bp_Bpp(bus_order_of(g), m, Bp, Bpp);
+            return csc_to_py(Bpp);
+        }, "B'' of the fast-decoupled power flow in the solver bus order (fillBp_Bpp).")
         .def("get_Ybus_solver", [](const GridModel& g) { return cscc_to_py(g.last_solver_input().Ybus); })
         .def("get_Sbus_solver", [](const GridModel& g) { return to_np(g.last_solver_input().Sbus); })
         .def("get_pv_solver", [](const GridModel& g) { return to_np(g.last_solver_input().pv); })

@@ -184,6 +184,14 @@ struct KArgs {
     // the status is SolverSolve (7) if theta is not finite or |theta| >= 1e6
     int dc;
     int slack0;
+    // Fast-decoupled power flow (BaseFDPFAlgo::compute_pf, BaseFDPFAlgo.tpp:12-160) on the same
+    // phases: the grid is loaded in the single-slack layout with every J entry a constant, B' on
+    // the P-row / angle-column block, B'' on the Q-row / magnitude-column block and 0 elsewhere, so
+    // that the factor is that of the block-diagonal [B' 0; 0 B'']. F = (V conj(Ybus V) - Sbus +
+    // Re(sum Sbus) w) / |V|. The loop alternates half-steps: odd ones apply only the angle steps
+    // (P half), even ones only the magnitude steps (Q half), with the convergence test after each;
+    // s_it counts half-steps, the reported iterations are P half-steps, max_iter bounds them.
+    int fdpf;
 };
 
 struct c64 {
@@ -587,7 +595,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 // the row's own data, loaded with its first entries
                 const int len = a.yr_ptr[r + 1] - a.yr_ptr[r];
                 const c64 sr = ld_c(S, r);
-                const double wr = a.single_slack ? 0.0 : a.w[r];
+                const double wr = (a.single_slack && !a.fdpf) ? 0.0 : a.w[r];
                 const int fp = a.fslot_p[r], fq = a.fslot_q[r], ed = a.ye_diag_of[r];
                 for (int k0 = 0; k0 == 0 || k0 < len; k0 += kEll) {
                     int2 pc[kEll];
@@ -608,7 +616,13 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                     Ib[2 * ed + 1] = acc.im;
                 }
                 c64 mis = csub(cmul(ldV(r), cconj(acc)), sr);
-                if (!a.single_slack) mis.re += sa * wr;
+                if (!a.single_slack || a.fdpf) mis.re += sa * wr;
+                if (a.fdpf) {  // P and Q of the fast-decoupled method: the mismatch divided by |V|
+                    const c64 vr = ldV(r);
+                    const double ivm = 1.0 / sqrt(vr.re * vr.re + vr.im * vr.im);
+                    mis.re *= ivm;
+                    mis.im *= ivm;
+                }
                 if (fp >= 0) { XS(fp, (FT)mis.re); fm = fmax(fm, inf_if_nonfinite(fabs(mis.re))); }
                 if (fq >= 0) { XS(fq, (FT)mis.im); fm = fmax(fm, inf_if_nonfinite(fabs(mis.im))); }
             }
@@ -1017,7 +1031,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // zero / non-finite pivot: klu_factor failure on the first factorisation of a fresh
         // solver (KLUSolver.cpp:38-44), klu_refactor failure afterwards (:50-57)
         if (s == 0 && act && s_piv[g]) {
-            s_status[g] = (a.fresh && s_it[g] == 1) ? 5 : 6;
+            s_status[g] = (a.fresh && s_it[g] == 1) ? 5 : 6;  // (FDPF: the first half-step)
             s_done[g] = 1;
         }
         __syncthreads();
@@ -1222,8 +1236,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             if (s == 0 && a.slack_dx >= 0) s_slack[g] -= (double)XL(a.slack_dx);
             for (int b = s; b < n; b += nslot) {
                 const int2 us = a.bus_dx[b];
-                const double dva = us.x >= 0 ? (double)XL(us.x) : 0.0;
-                const double dvm = us.y >= 0 ? (double)XL(us.y) : 0.0;
+                // FDPF: the P half-step (odd s_it) moves the angles only, the Q half the magnitudes
+                const bool pstep = !a.fdpf || (s_it[g] & 1) != 0, qstep = !a.fdpf || (s_it[g] & 1) == 0;
+                const double dva = (us.x >= 0 && pstep) ? (double)XL(us.x) : 0.0;
+                const double dvm = (us.y >= 0 && qstep) ? (double)XL(us.y) : 0.0;
                 double vm;
                 if constexpr (LEAN) vm = update_bus(Vl, G, Va, Vm, b, dva, dvm, a.single_slack != 0);
                 else vm = update_bus(V, 1, Va, Vm, b, dva, dvm, a.single_slack != 0);
@@ -1247,7 +1263,10 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             s_flast[g] = fmx;
             if (!(fmx <= 1.7976931348623157e308)) { s_status[g] = 3; s_done[g] = 1; }
             else if (fmx < a.tol) { s_status[g] = 0; s_done[g] = 1; }
-            else if (s_it[g] >= a.max_iter) { s_status[g] = 2; s_done[g] = 1; }
+            else if (a.fdpf ? ((s_it[g] & 1) == 0 && s_it[g] / 2 >= a.max_iter) : s_it[g] >= a.max_iter) {
+                s_status[g] = 2;
+                s_done[g] = 1;
+            }
         }
         __syncthreads();
         tmark(5);
@@ -1261,7 +1280,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             }
         if (s == 0) {
             const int stt = s_status[g];
-            a.iters[row] = s_it[g];
+            a.iters[row] = a.fdpf ? (s_it[g] + 1) / 2 : s_it[g];
             a.status[row] = stt;
             if (a.flags)
                 a.flags[row] = (stt != 0 && s_flast[g] < a.stall_tol ? kFlagStall : 0) | (s_grow[g] ? kFlagGrowth : 0);
@@ -1487,6 +1506,7 @@ struct pfgpu_grid {
     // initialize, KLUSolver.cpp:24-45); cleared by a solve, set by load and pfgpu_grid_reset
     bool fresh = true;
     bool dc = false;  // a DC grid (pfgpu_grid_load_dc): one linear solve per instance
+    bool fdpf = false;  // a fast-decoupled grid (pfgpu_grid_load_fdpf): constant J, half-steps
     std::vector<double> w_h;  // slack weights (host copy)
     // Pivots chosen on the numbers (host re-pivot): a plan of the same grid whose factor takes
     // its pivots off the diagonal where the first factorisation of a fresh solve met a zero or
@@ -1785,7 +1805,7 @@ constexpr long long kWidePairs = 200000;
 constexpr int kWideBlocks = 32;  // workgroups of a wide launch (PFGPU_WIDE_BLOCKS overrides)
 
 bool wide_wanted(const pfgpu_grid* gr, const SolveArgs& q, int G_req, int nunits, const FixedScratch* fx) {
-    if (gr->dc || q.Jx_last || fx) return false;
+    if (gr->dc || gr->fdpf || q.Jx_last || fx) return false;
     if (G_req == -1) return true;
     return G_req == 0 && nunits == 1 && (long long)gr->P.pairs.size() / 4 >= kWidePairs;
 }
@@ -1997,6 +2017,7 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
     a.ngroups = ngroups;
     a.dc = gr->dc ? 1 : 0;
     a.slack0 = P.slack0;
+    a.fdpf = gr->fdpf ? 1 : 0;
     if (gr->dc) a.max_iter = 1;
     const int nell = gr->ell_w * n, nye = (int)P.ye_pos.size();
     auto issue = [&]() {
@@ -2115,6 +2136,8 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     if (q.chain > 1 && q.V0 == q.V) throw ApiError(PFGPU_E_INVALID, "chains need V0 apart from V");
     if (q.dptr && q.Yx_stride != 0) throw ApiError(PFGPU_E_INVALID, "Ybus deltas need a shared base (Yx_stride 0)");
     if (gr->dc && q.precision != PFGPU_FP64) throw ApiError(PFGPU_E_INVALID, "a DC grid is solved with FP64 factors");
+    if (gr->fdpf && q.precision != PFGPU_FP64)
+        throw ApiError(PFGPU_E_INVALID, "a fast-decoupled grid is solved with FP64 factors");
     const AuxLayout AL = aux_layout(q.nrows, q.ninst);
     char* aux = fx ? fx->aux : static_cast<char*>(grow_buffer(ctx->aux, ctx->aux_bytes, AL.bytes));
     const bool fp32 = q.precision != PFGPU_FP64;
@@ -2203,6 +2226,7 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
     alt->J = gr->J;
     alt->J.rowpos = std::move(rowpos);
     alt->dc = gr->dc;
+    alt->fdpf = gr->fdpf;
     alt->info = gr->info;
     alt->w_h = gr->w_h;
     build_plan(alt.get(), &order);
@@ -2225,9 +2249,16 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
 }  // namespace
 
 namespace {
+// B' and B'' of a fast-decoupled grid (n x n, CSC)
+struct FdpfB {
+    const int *Bp_p, *Bp_i;
+    const double* Bp_x;
+    const int *Bpp_p, *Bpp_i;
+    const double* Bpp_x;
+};
 int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
               const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack, bool dc,
-              pfgpu_grid** out);
+              pfgpu_grid** out, const FdpfB* fd = nullptr);
 }  // namespace
 
 extern "C" {
@@ -2324,6 +2355,24 @@ int pfgpu_grid_load_dc(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, cons
     });
 }
 
+int pfgpu_grid_load_fdpf(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
+                         const double* slack_weights, const int* pv, int npv, const int* pq, int npq, const int* Bp_p,
+                         const int* Bp_i, const double* Bp_x, const int* Bpp_p, const int* Bpp_i, const double* Bpp_x,
+                         pfgpu_grid** out) {
+    return guarded([&]() -> int {
+        if (!ctx || !out || !slack_ids || !slack_weights || n <= 0 || nslack < 1 || npv < 0 || npq < 0 ||
+            (npv > 0 && !pv) || (npq > 0 && !pq) || !Bp_p || !Bp_i || !Bp_x || !Bpp_p || !Bpp_i || !Bpp_x)
+            throw ApiError(PFGPU_E_INVALID, "invalid grid arguments");
+        // my_pv = slack_ids[1:] ++ pv (retrieve_pv_with_slack, BaseAlgo.hpp:201-216): the other slack
+        // buses carry their angle, the first one is the reference; the single-slack layout
+        std::vector<int> mypv(slack_ids + 1, slack_ids + nslack);
+        mypv.insert(mypv.end(), pv, pv + npv);
+        const FdpfB fd{Bp_p, Bp_i, Bp_x, Bpp_p, Bpp_i, Bpp_x};
+        return load_grid(ctx, n, Yp, Yi, slack_ids, 1, slack_weights, mypv.data(), (int)mypv.size(), pq, npq, 1, false,
+                         out, &fd);
+    });
+}
+
 namespace {
 void build_plan(pfgpu_grid* g, const std::vector<int>* perm) {
     pfgpu::SymbolicLU S = pfgpu::symbolic_lu(g->J, 96, perm);
@@ -2385,9 +2434,56 @@ void build_plan(pfgpu_grid* g, const std::vector<int>* perm) {
     g->w.upload(g->w_h);
 }
 
+// The J entries of a fast-decoupled grid become constants: B'(row bus, column bus) on the P-row /
+// angle-column block, B''(row bus, column bus) on the Q-row / magnitude-column block, 0 on the two
+// coupling blocks (fill_sparse_matrices, BaseFDPFAlgo.tpp:164-214: Bp = grid_Bp[pvpq, pvpq], Bpp =
+// grid_Bpp[pq, pq]). Every entry of those restrictions must lie in the Ybus pattern.
+void fdpf_constants(pfgpu::JPattern& J, int n, const FdpfB& fd) {
+    const int npvpq = (int)J.pvpq.size(), N = J.N;
+    auto bus = [&](int k) { return k < npvpq ? J.pvpq[k] : J.pq[k - npvpq]; };
+    auto lookup = [&](const int* Ap, const int* Ai, const double* Ax, int r, int c) -> double {
+        double v = 0.0;  // duplicates are summed, as setFromTriplets does
+        for (int q = Ap[c]; q < Ap[c + 1]; ++q)
+            if (Ai[q] == r) v += Ax[q];
+        return v;
+    };
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) {
+            const int r = J.Ji[q];
+            const bool prow = r < npvpq, acol = c < npvpq;
+            double v = 0.0;
+            if (prow && acol) v = lookup(fd.Bp_p, fd.Bp_i, fd.Bp_x, bus(r), bus(c));
+            if (!prow && !acol) v = lookup(fd.Bpp_p, fd.Bpp_i, fd.Bpp_x, bus(r), bus(c));
+            J.sel[q] = pfgpu::SEL_CONST;
+            J.src[q] = -1;
+            J.cval[q] = v;
+        }
+    // every stored entry of the restrictions has a J position
+    std::vector<int> kpos(n, -1);  // bus -> J index of its angle (P rows) / magnitude (Q rows)
+    std::vector<long long> keys;
+    for (int c = 0; c < N; ++c)
+        for (int q = J.Jp[c]; q < J.Jp[c + 1]; ++q) keys.push_back((long long)J.Ji[q] * N + c);
+    std::sort(keys.begin(), keys.end());
+    auto check = [&](const int* Ap, const int* Ai, int k0, int k1) {
+        std::fill(kpos.begin(), kpos.end(), -1);
+        for (int k = k0; k < k1; ++k) kpos[bus(k)] = k;
+        for (int c = 0; c < n; ++c) {
+            if (kpos[c] < 0) continue;
+            for (int q = Ap[c]; q < Ap[c + 1]; ++q) {
+                if (Ai[q] < 0 || Ai[q] >= n) throw ApiError(PFGPU_E_INVALID, "B' / B'' row index out of range");
+                if (kpos[Ai[q]] < 0) continue;
+                if (!std::binary_search(keys.begin(), keys.end(), (long long)kpos[Ai[q]] * N + kpos[c]))
+                    throw ApiError(PFGPU_E_INVALID, "B' / B'' hold an entry outside the Ybus pattern");
+            }
+        }
+    };
+    check(fd.Bp_p, fd.Bp_i, 0, npvpq);
+    check(fd.Bpp_p, fd.Bpp_i, npvpq, N);
+}
+
 int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* slack_ids, int nslack,
               const double* slack_weights, const int* pv, int npv, const int* pq, int npq, int single_slack, bool dc,
-              pfgpu_grid** out) {
+              pfgpu_grid** out, const FdpfB* fd) {
     return guarded([&]() -> int {
         if (!ctx || !out || !Yp || !Yi || !slack_ids || !slack_weights || n <= 0 || nslack < 1 || npv < 0 || npq < 0 ||
             (npv > 0 && !pv) || (npq > 0 && !pq))
@@ -2419,10 +2515,13 @@ int load_grid(pfgpu_ctx* ctx, int n, const int* Yp, const int* Yi, const int* sl
         for (int k = 0; k < npv; ++k)
             if (pv[k] < 0 || pv[k] >= n) throw ApiError(PFGPU_E_INVALID, "pv bus out of range");
         g->J = pfgpu::build_jacobian_pattern(g->Y, slack_ids, nslack, slack_weights, pv, npv, pq, npq, single_slack != 0);
+        if (fd) fdpf_constants(g->J, n, *fd);
         g->w_h.assign(slack_weights, slack_weights + n);
         g->info.single_slack = single_slack ? 1 : 0;
         g->info.dc = dc ? 1 : 0;
+        g->info.fdpf = fd ? 1 : 0;
         g->dc = dc;
+        g->fdpf = fd != nullptr;
         build_plan(g.get(), nullptr);
         *out = g.release();
         return 0;

@@ -740,6 +740,131 @@ int oracle_dc(int n, const int* Yp, const int* Yi, const double* Yx, double* V, 
     return NoError;
 }
 
+// Fast-decoupled power flow (BaseFDPFAlgo<KLU, XB_BX>::compute_pf, BaseFDPFAlgo.tpp:12-160,
+// has_converged / evaluate_mismatch BaseFDPFAlgo.hpp:64-164). grid_Bp / grid_Bpp are the n x n
+// matrices of GridModel::fillBp_Bpp (CSC); the solver keeps Bp = grid_Bp[pvpq, pvpq] and
+// Bpp = grid_Bpp[pq, pq] (fill_sparse_matrices, BaseFDPFAlgo.tpp:164-214) with my_pv = slack_ids[1:]
+// followed by pv (retrieve_pv_with_slack, BaseAlgo.hpp:201-216), factors both once and alternates
+// the P half-step (Va[pvpq] -= Bp^-1 P) and the Q half-step (Vm[pq] -= Bpp^-1 Q), P and Q being the
+// mismatch divided by Vm, with a convergence test after each half. slack_absorbed stays Re(sum Sbus).
+// Returns the ErrorType; V in/out, *iters = P half-steps done.
+int oracle_fdpf(int n, const int* Yp, const int* Yi, const double* Yx, double* V, const double* Sbus,
+                const int* slack_ids, int nslack, const double* slack_w, const int* pv, int npv, const int* pq, int npq,
+                const int* Bp_p, const int* Bp_i, const double* Bp_x, const int* Bpp_p, const int* Bpp_i,
+                const double* Bpp_x, int max_iter, double tol, int* iters) {
+    std::vector<int> pvpq;
+    for (int s = 1; s < nslack; ++s) pvpq.push_back(slack_ids[s]);
+    for (int k = 0; k < npv; ++k) pvpq.push_back(pv[k]);
+    const int npvpq_ = (int)pvpq.size();
+    for (int k = 0; k < npq; ++k) pvpq.push_back(pq[k]);
+    const int nvv = (int)pvpq.size();
+    std::vector<int> pvpq_inv(n, -1), pq_inv(n, -1);
+    for (int k = 0; k < nvv; ++k) pvpq_inv[pvpq[k]] = k;
+    for (int k = 0; k < npq; ++k) pq_inv[pq[k]] = k;
+    // sub-matrices in CSC with sorted rows, duplicates summed (setFromTriplets)
+    auto sub = [&](const int* Ap, const int* Ai, const double* Ax, const std::vector<int>& inv, int m,
+                   std::vector<int>& p, std::vector<int>& i, std::vector<double>& x) {
+        std::vector<std::vector<std::pair<int, double>>> cols(m);
+        for (int c = 0; c < n; ++c) {
+            if (inv[c] < 0) continue;
+            for (int q = Ap[c]; q < Ap[c + 1]; ++q)
+                if (inv[Ai[q]] >= 0) cols[inv[c]].push_back({inv[Ai[q]], Ax[q]});
+        }
+        p.assign(m + 1, 0);
+        i.clear();
+        x.clear();
+        for (int c = 0; c < m; ++c) {
+            auto& v = cols[c];
+            std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (size_t k = 0; k < v.size(); ++k) {
+                if (!i.empty() && (int)i.size() > p[c] && i.back() == v[k].first) x.back() += v[k].second;
+                else { i.push_back(v[k].first); x.push_back(v[k].second); }
+            }
+            p[c + 1] = (int)i.size();
+        }
+    };
+    std::vector<int> Pp, Pi, Qp, Qi;
+    std::vector<double> Px, Qx;
+    sub(Bp_p, Bp_i, Bp_x, pvpq_inv, nvv, Pp, Pi, Px);
+    sub(Bpp_p, Bpp_i, Bpp_x, pq_inv, npq, Qp, Qi, Qx);
+    (void)npvpq_;
+    const cplx* Yc = reinterpret_cast<const cplx*>(Yx);
+    const cplx* S = reinterpret_cast<const cplx*>(Sbus);
+    std::vector<cplx> Vc(reinterpret_cast<cplx*>(V), reinterpret_cast<cplx*>(V) + n);
+    std::vector<double> Vm(n), Va(n);
+    for (int b = 0; b < n; ++b) { Vm[b] = std::abs(Vc[b]); Va[b] = std::arg(Vc[b]); }
+    double slack_absorbed = 0.0;
+    for (int b = 0; b < n; ++b) slack_absorbed += S[b].real();
+    std::vector<double> p(nvv), q(npq);
+    int err = NoError;
+    // mis = (V conj(Ybus V) - Sbus + slack_absorbed w) / Vm; P = mis[pvpq].re, Q = mis[pq].im
+    auto mismatch = [&]() -> bool {
+        std::vector<cplx> I(n, cplx(0, 0));
+        for (int c = 0; c < n; ++c)
+            for (int k = Yp[c]; k < Yp[c + 1]; ++k) I[Yi[k]] += Yc[k] * Vc[c];
+        std::vector<cplx> mis(n);
+        bool finite = true;
+        for (int b = 0; b < n; ++b) {
+            mis[b] = (Vc[b] * std::conj(I[b]) - S[b] + slack_absorbed * slack_w[b]) / Vm[b];
+            finite = finite && std::isfinite(mis[b].real()) && std::isfinite(mis[b].imag());
+        }
+        if (!finite) return false;
+        for (int k = 0; k < nvv; ++k) p[k] = mis[pvpq[k]].real();
+        for (int k = 0; k < npq; ++k) q[k] = mis[pq[k]].imag();
+        return true;
+    };
+    auto check = [&]() -> bool {
+        double np = 0.0, nq = 0.0;
+        for (double v : p) np = std::max(np, std::fabs(v));
+        for (double v : q) nq = std::max(nq, std::fabs(v));
+        return np < tol && nq < tol;
+    };
+    // has_converged: V = Vm exp(j Va) (tmp_va), Vm and Va taken back from V, then the mismatch
+    auto has_converged = [&](const std::vector<cplx>& tmp_va) -> bool {
+        for (int b = 0; b < n; ++b) {
+            Vc[b] = Vm[b] * tmp_va[b];
+            Vm[b] = std::abs(Vc[b]);
+            Va[b] = std::arg(Vc[b]);
+        }
+        if (!mismatch()) {
+            err = InifiniteValue;
+            return false;
+        }
+        return check();
+    };
+    bool converged = mismatch() && check();
+    int it = 0;
+    KluLikeLU lu_p, lu_q;
+    bool factored = false;
+    while (!converged && it < max_iter) {
+        ++it;
+        if (!factored) {
+            int e = lu_p.analyze(nvv, Pp, Pi);
+            if (e == NoError) e = lu_p.factor(Pp, Pi, Px);
+            if (e == NoError) e = lu_q.analyze(npq, Qp, Qi);
+            if (e == NoError) e = lu_q.factor(Qp, Qi, Qx);
+            if (e != NoError) {
+                err = e == SingularMatrix ? SolverFactor : e;
+                break;
+            }
+            factored = true;
+        }
+        std::vector<double> x = p;
+        if (lu_p.solve(x) != NoError) { err = SolverSolve; break; }
+        for (int k = 0; k < nvv; ++k) Va[pvpq[k]] -= x[k];
+        std::vector<cplx> tmp_va(n);
+        for (int b = 0; b < n; ++b) tmp_va[b] = cplx(std::cos(Va[b]), std::sin(Va[b]));
+        if (has_converged(tmp_va)) { converged = true; break; }
+        if (lu_q.solve(q) != NoError) { err = SolverSolve; break; }
+        for (int k = 0; k < npq; ++k) Vm[pq[k]] -= q[k];
+        converged = has_converged(tmp_va);
+    }
+    if (!converged && err == NoError) err = TooManyIterations;
+    for (int b = 0; b < n; ++b) { V[2 * b] = Vc[b].real(); V[2 * b + 1] = Vc[b].imag(); }
+    *iters = it;
+    return err;
+}
+
 // Contingency sweep (ContingencyAnalysis::compute loop). Each contingency c
 // subtracts its Ybus deltas dv[dptr[c]:dptr[c+1]] at Ybus value positions
 // dpos[...], checks connectivity, solves from Vinit and restores Ybus.

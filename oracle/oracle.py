@@ -276,3 +276,67 @@ def dc_pf(Ybus, V0, Sbus, slack_ids, pv, pq):
     pq_, ppq = _i(pq)
     err = lib().oracle_dc(n, pYp, pYi, pYx, V.ctypes.data_as(_f64p), pS, psl, len(sl), ppv, len(pv_), ppq, len(pq_))
     return V, int(err)
+
+
+def fdpf_matrices(n, f, t, r, x, h, method, shunt_bus=(), shunt_q_mvar=(), sn_mva=1.0):
+    """grid_Bp and grid_Bpp (n x n CSC) of GridModel::fillBp_Bpp (GridModel.cpp:1037-1060) for a
+    grid of lines (total charging h, split half and half between the two ends, LineContainer.cpp:
+    13-34) and shunts: per line, TwoSidesContainer_rxh_A::get_fdpf_coeffs (TwoSidesContainer_rxh_A
+    .hpp:818-843): ys_bp = 1/(jx) (XB) or 1/(r+jx) (BX), ys_bpp the other one; Bp gets -Im(ys_bp)
+    on both diagonals and +Im(ys_bp) off them, Bpp -(Im(ys_bpp) + Im(h)/2) on the diagonals and
+    +Im(ys_bpp) off them (fillBp_Bpp, :503-598); a shunt adds q_mvar / sn_mva to its Bpp diagonal
+    (ShuntContainer.cpp:64-95). method: "XB" or "BX"."""
+    f, t = np.asarray(f, int), np.asarray(t, int)
+    r, x, h = np.asarray(r, float), np.asarray(x, float), np.asarray(h, complex)
+    y_jx = (1.0 / (1j * x)).imag
+    y_rx = (1.0 / (r + 1j * x)).imag
+    ybp, ybpp = (y_jx, y_rx) if method == "XB" else (y_rx, y_jx)
+    half = np.imag(h) * 0.5
+    rows = np.r_[f, t, f, t]
+    cols = np.r_[f, t, t, f]
+    Bp = sp.csc_matrix((np.r_[-ybp, -ybp, ybp, ybp], (rows, cols)), shape=(n, n))
+    Bpp = sp.csc_matrix((np.r_[-(ybpp + half), -(ybpp + half), ybpp, ybpp], (rows, cols)), shape=(n, n))
+    if len(shunt_bus):
+        sb = np.asarray(shunt_bus, int)
+        Bpp = Bpp + sp.csc_matrix((np.asarray(shunt_q_mvar, float) / sn_mva, (sb, sb)), shape=(n, n))
+    Bp.sum_duplicates()
+    Bpp.sum_duplicates()
+    Bp.sort_indices()
+    Bpp.sort_indices()
+    return Bp, Bpp
+
+
+def fdpf(Ybus, V0, Sbus, slack_ids, slack_weights, pv, pq, Bp, Bpp, max_iter=10, tol=1e-8):
+    """Fast-decoupled power flow on the oracle (BaseFDPFAlgo<KLU>::compute_pf).
+    Returns (V, iterations, error)."""
+    L = lib()
+    if not getattr(L, "_fdpf_typed", False):
+        L.oracle_fdpf.restype = ctypes.c_int
+        L.oracle_fdpf.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _i32p, ctypes.c_int, _f64p,
+                                  _i32p, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p,
+                                  ctypes.c_int, ctypes.c_double, _i32p]
+        L._fdpf_typed = True
+    Y = _csc(Ybus)
+    n = Y.shape[0]
+    Yp, pYp = _i(Y.indptr)
+    Yi, pYi = _i(Y.indices)
+    Yx, pYx = _c(Y.data)
+    V = np.array(V0, dtype=np.complex128, copy=True)
+    S, pS = _c(Sbus)
+    sl, psl = _i(np.atleast_1d(slack_ids))
+    w, pw = _d(slack_weights)
+    pv_, ppv = _i(pv)
+    pq_, ppq = _i(pq)
+    Bp, Bpp = sp.csc_matrix(Bp), sp.csc_matrix(Bpp)
+    Bp.sort_indices()
+    Bpp.sort_indices()
+    a1, pa1 = _i(Bp.indptr)
+    a2, pa2 = _i(Bp.indices)
+    a3, pa3 = _d(Bp.data)
+    b1, pb1 = _i(Bpp.indptr)
+    b2, pb2 = _i(Bpp.indices)
+    b3, pb3 = _d(Bpp.data)
+    it = np.zeros(1, np.int32)
+    err = L.oracle_fdpf(n, pYp, pYi, pYx, V.ctypes.data_as(_f64p), pS, psl, len(sl), pw, ppv, len(pv_), ppq, len(pq_),
+                        pa1, pa2, pa3, pb1, pb2, pb3, int(max_iter), float(tol), it.ctypes.data_as(_i32p))
+    return V, int(it[0]), int(err)
