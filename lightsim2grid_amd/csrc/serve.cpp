@@ -94,6 +94,34 @@ int ServeLoop::add_grid(const CscC& Y, const CplxVect& Sbus, const CplxVect& V0,
     check_rc(pfgpu_memcpy_h2d(g->ctx, g->dY, Y.x.data(), Y.x.size() * 16), "h2d");
     check_rc(pfgpu_memcpy_h2d(g->ctx, g->dS0, Sbus.data(), (size_t)n * 16), "h2d");
     check_rc(pfgpu_memcpy_h2d(g->ctx, g->dV0, V0.data(), (size_t)n * 16), "h2d");
+    // warm-up: a launch of one instance per compute unit at the resident injections, in each
+    // precision form a launch of this loop may take (the FP32 form re-solves stalled rows in FP64),
+    // so that the first requests do not pay for the launch shape's lane schedule, the kernel's
+    // first load and the growth of the engine's workspace to every resident workgroup
+    {
+        int ncu = 0, lds = 0;
+        size_t fr = 0;
+        check_rc(pfgpu_ctx_device_info(g->ctx, &ncu, &lds, &fr), "pfgpu_ctx_device_info");
+        const int M = std::max(1, std::min(max_inst_, ncu));
+        double* wV = nullptr;
+        alloc(&wV, (size_t)M * n * 16);
+        void *wIt = nullptr, *wSt = nullptr;
+        check_rc(pfgpu_dev_alloc(g->ctx, (size_t)M * 4, &wIt), "pfgpu_dev_alloc");
+        check_rc(pfgpu_dev_alloc(g->ctx, (size_t)M * 4, &wSt), "pfgpu_dev_alloc");
+        std::vector<int> forms{precision_};
+        if (precision_ == PFGPU_FP32) forms.push_back(PFGPU_FP64);
+        int rc = 0;
+        for (int prec : forms) {
+            rc = pfgpu_solve_batch_device(g->ctx, g->grid, M, g->dY, 0, nullptr, nullptr, nullptr, g->dS0, 0, g->dV0, 0,
+                                          wV, max_iter, tol, prec, 1, static_cast<int*>(wIt), static_cast<int*>(wSt),
+                                          nullptr);
+            if (rc != 0) break;
+        }
+        pfgpu_dev_free(g->ctx, wV);
+        pfgpu_dev_free(g->ctx, wIt);
+        pfgpu_dev_free(g->ctx, wSt);
+        check_rc(rc, "pfgpu_solve_batch_device (warm-up)");
+    }
     std::lock_guard<std::mutex> lk(mu_);
     if (stop_) throw std::runtime_error("ServeLoop is closed");
     const int gi = (int)grids_.size();

@@ -13,6 +13,8 @@
 //
 // Device memory: each grid's launch buffers grow with the largest launch it ran and are kept
 // (counted in in_use() until close); a launch is admitted when what it would add fits the budget.
+// add_grid runs a warm-up launch (one instance per compute unit, in each precision form a launch
+// may take), so that the first requests do not pay for the launch shape's set-up.
 #pragma once
 
 #include <atomic>
