@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X vector FLOP peaks by factor precision (bytes per value): FP32 157.3, FP64 78.6 TFLOP/s
+VEC_PEAK_TFLOPS = {4: 157.3, 8: 78.6}
+# LDS: 128 bytes per clock per compute unit at 2.4 GHz
+LDS_PEAK_GBS_PER_CU = 128 * 2.4
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_nr_case2000_n1_hbm_traffic.md")
 PUBLISHED_14BUS_MS = 0.0112  # NR (KLU) time in algo, l2rpn_case14_sandbox, 1 core (docs/benchmarks.rst:212)
 
@@ -97,6 +101,22 @@ def algorithmic_bytes(info, iters, w):
     n, N = info["n_bus"], info["n_jac"]
     per_it = w * (info["nnz_jac"] + 3 * info["nnz_lu"]) + 64 * n + 16 * N
     return float(np.sum(iters)) * per_it
+
+
+def algorithmic_flops(info, iters):
+    """FLOPs of solves with `iters` iterations: sum of it * [refactor flops (the plan's count:
+    multiply-adds as 2, divisions 1) + 2 nnzLU (the two triangular solves) + 16 nnzY (two complex
+    SpMVs)]."""
+    per_it = info["flops"] + 2 * info["nnz_lu"] + 16 * info["nnz_ybus"]
+    return float(np.sum(iters)) * per_it
+
+
+def flop_rate(info, iters, seconds, w):
+    """Achieved FLOP rate of solves against MI355X's vector peak of their factor precision."""
+    fl = algorithmic_flops(info, iters)
+    peak = VEC_PEAK_TFLOPS[w]
+    return {"flops": fl, "achieved_tflops": fl / seconds / 1e12, "peak_tflops": peak,
+            "frac": fl / seconds / 1e12 / peak, "peak": "fp%d vector" % (8 * w)}
 
 
 def parity(V, iters, status, Vr, itr, convr):
@@ -251,6 +271,14 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
     gen_p = Sb[:, gen_bus].real * (gen_bus != g.slack)
     load_p, load_q = -Sb[:, g.pq].real, -Sb[:, g.pq].imag
     out = {"workload": "case118_synth (1 slack, 53 PV), %d hourly steps, warm start from the previous step" % T}
+    from lightsim2grid_amd import abi
+    d = g.solver_inputs()
+    ctx = abi.Context(0)
+    ncu = ctx.device_info()["n_cu"]
+    gr = abi.Grid(ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    info = gr.info
+    gr.close()
+    ctx.close()
     for chain, lname in ((0, "single_chain"), (8, "chains_of_8")):
         res, gpu_V = {}, {}
         for st, nm in ((L.SolverType.AMDGPUMixed, "fp32"), (L.SolverType.AMDGPU, "fp64")):
@@ -268,13 +296,23 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
             from oracle import oracle
             d = g.solver_inputs()
             nch = 1 if chain == 0 else (T + chain - 1) // chain
-            Vr, _, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"],
-                                                              d["pv"], d["pq"], g.v0(), 10, 1e-8, False, nch,
-                                                              1 if nch == 1 else nthreads)
+            Vr, itr, _, conv, status, wall = oracle.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"],
+                                                                d["pv"], d["pq"], g.v0(), 10, 1e-8, False, nch,
+                                                                1 if nch == 1 else nthreads)
             res["cpu_baseline"] = {"value": float(np.sum(conv)) / wall, "cores": 1 if nch == 1 else nthreads,
                                    "kind": "port", "wall_s": wall}
             # every step against the oracle on the same chain layout (TimeSeriesCPP keeps no
             # per-step iteration counts: flags and voltages only)
+            # LDS roof of the solver time: the factors live in LDS, so the algorithmic bytes (at the
+            # oracle's iteration counts) are priced against the LDS bandwidth of the compute units in
+            # use (one for the single chain, every unit for the chains side by side)
+            cus = 1 if nch == 1 else min(ncu, nch)
+            for nm, w in (("fp32", 4), ("fp64", 8)):
+                b = algorithmic_bytes(info, itr[conv], w)
+                t = res[nm]["solver_time_s"]
+                res[nm]["lds_roof"] = {"achieved_gbs": b / t / 1e9, "peak_gbs": LDS_PEAK_GBS_PER_CU * cus,
+                                       "frac": b / t / 1e9 / (LDS_PEAK_GBS_PER_CU * cus), "compute_units": cus,
+                                       "iterations": "the oracle's on the same layout (the FP32 form may differ by 1)"}
             res["parity"] = {}
             for nm, Vg in gpu_V.items():
                 ok = np.all(np.isfinite(Vg), axis=1) & (np.abs(Vg).sum(axis=1) > 0)
@@ -320,10 +358,17 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
         st = gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
         wall = time.perf_counter() - t0
         status = dSt.to_host()
+        iters = dIt.to_host()
         conv = int(np.sum(status == 0))
+        w = 4 if prec == abi.FP32 else 8
+        info = gr.info
         out[nm] = {"wall_s": wall, "kernel_ms": 1e3 * st["kernel_s"], "converged": conv, "solves_per_s": conv / wall,
                    "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"],
-                   "fallback_fp64": st["fallback_fp64"], "resident_workgroups": st["resident_workgroups"]}
+                   "fallback_fp64": st["fallback_fp64"], "resident_workgroups": st["resident_workgroups"],
+                   "iters_mean": float(np.mean(iters[status == 0])),
+                   "achieved_gbs": algorithmic_bytes(info, iters[status == 0], w) / st["kernel_s"] / 1e9,
+                   "hbm_frac": algorithmic_bytes(info, iters[status == 0], w) / st["kernel_s"] / 1e9 / HBM_PEAK_GBS,
+                   "flop_rate": flop_rate(info, iters[status == 0], st["kernel_s"], w)}
         # one solve: the default launch (a wide launch: every workgroup on the one instance, level
         # by level) and the batched kernel's one workgroup per instance
         one = (dptr[:2], dpos[:dptr[1]], dval[:dptr[1]])
@@ -467,6 +512,7 @@ def main():
             converged=int(conv.sum()), ninst=ninst, wall_s=wall, ms_per_step=1e3 * wall / args.steps,
             solves_per_s=int(conv.sum()) * args.steps / wall, kernel_ms=1e3 * kavg,
             iters_mean=float(np.mean(iters[conv])), achieved_gbs=bytes_conv / kavg / 1e9, bytes_per_launch=bytes_conv,
+            flop_rate=flop_rate(info, iters[conv], kavg, w),
             lds_slots=st["lds_slots"], instances_per_wg=st["instances_per_wg"], block_size=st["block_size"],
             resident_workgroups=st["resident_workgroups"], fallback_fp64_per_step=fb / args.steps,
             status_hist=np.bincount(status, minlength=9).tolist(), autotune=tuned.get(name),
@@ -548,7 +594,7 @@ def main():
         "parity": par,
         "fp64_form": {"value": r64["solves_per_s"], "ms_per_step": r64["ms_per_step"], "kernel_ms": r64["kernel_ms"],
                       "achieved_gbs": r64["achieved_gbs"], "iters_mean": r64["iters_mean"],
-                      "converged": r64["converged"]},
+                      "converged": r64["converged"], "flop_rate": r64["flop_rate"]},
         "fp32_over_fp64": r32["solves_per_s"] / r64["solves_per_s"],
         "details": {"fp32": r32, "fp64": r64},
         "other_configs": extras,
