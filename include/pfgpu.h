@@ -375,7 +375,8 @@ int pfgpu_server_create(size_t hbm_budget, int max_instances_per_launch, int pre
 /* Stops the workers; requests still queued complete with an error. */
 int pfgpu_server_destroy(pfgpu_server* srv);
 /* Loads a grid (pattern analysis, schedules) and keeps its Ybus values, base Sbus and warm-start
- * V0 resident; grid_id receives its id. */
+ * V0 resident; grid_id receives its id. Runs a warm-up launch (one instance per compute unit at
+ * the resident injections, in each precision form a launch may take) before it returns. */
 int pfgpu_server_add_grid(pfgpu_server* srv, int n, const int* Yp, const int* Yi, const double* Yx, const double* Sbus,
                           const double* V0, const int* slack_ids, int nslack, const double* slack_weights, const int* pv,
                           int npv, const int* pq, int npq, int max_iter, double tol, int* grid_id);
