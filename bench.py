@@ -352,7 +352,11 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
     out = {"workload": "case10000_synth (distributed slack over the 2500 PV generators, loading 0.25): "
                        "%d non-bridge outages x %d load snapshots = %d solves" % (len(outs), n_snap, ninst),
            "grid_load_s": t_load, "n_jac": gr.info["n_jac"], "nnz_lu": gr.info["nnz_lu"]}
+    # the launch autotuner on the first 2048 solves of the batch (untimed), as for the 2000-bus sweep
+    kt = min(2048, ninst)
+    dd_t = (ctx.to_device(dptr[:kt + 1]), ctx.to_device(dpos[:dptr[kt]]), ctx.to_device(dval[:dptr[kt]]))
     for prec, nm in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+        tuned = gr.autotune_device(dY, 0, dS, g.n, dV0, 0, dV, kt, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd_t)
         gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
         t0 = time.perf_counter()
         st = gr.solve_device(dY, 0, dS, g.n, dV0, 0, dV, ninst, dIt, dSt, 10, 1e-8, precision=prec, deltas=dd)
@@ -365,7 +369,7 @@ def large_grid_run(with_cpu: bool, nthreads: int, n_out: int = 5000, n_snap: int
         out[nm] = {"wall_s": wall, "kernel_ms": 1e3 * st["kernel_s"], "converged": conv, "solves_per_s": conv / wall,
                    "instances_per_wg": st["instances_per_wg"], "lds_slots": st["lds_slots"],
                    "fallback_fp64": st["fallback_fp64"], "resident_workgroups": st["resident_workgroups"],
-                   "iters_mean": float(np.mean(iters[status == 0])),
+                   "iters_mean": float(np.mean(iters[status == 0])), "autotune": list(tuned),
                    "achieved_gbs": algorithmic_bytes(info, iters[status == 0], w) / st["kernel_s"] / 1e9,
                    "hbm_frac": algorithmic_bytes(info, iters[status == 0], w) / st["kernel_s"] / 1e9 / HBM_PEAK_GBS,
                    "flop_rate": flop_rate(info, iters[status == 0], st["kernel_s"], w)}
