@@ -270,7 +270,8 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
     gen_bus = np.r_[g.slack, g.pv]
     gen_p = Sb[:, gen_bus].real * (gen_bus != g.slack)
     load_p, load_q = -Sb[:, g.pq].real, -Sb[:, g.pq].imag
-    out = {"workload": "case118_synth (1 slack, 53 PV), %d hourly steps, warm start from the previous step" % T}
+    out = {"workload": "case118_synth (1 slack, 53 PV), %d hourly steps, warm start from the previous step "
+                       "(the single chain in two batched passes; chains of 8 in one launch)" % T}
     from lightsim2grid_amd import abi
     d = g.solver_inputs()
     ctx = abi.Context(0)
@@ -304,9 +305,10 @@ def time_series_run(with_cpu: bool, nthreads: int, T: int = 8760) -> dict:
             # every step against the oracle on the same chain layout (TimeSeriesCPP keeps no
             # per-step iteration counts: flags and voltages only)
             # LDS roof of the solver time: the factors live in LDS, so the algorithmic bytes (at the
-            # oracle's iteration counts) are priced against the LDS bandwidth of the compute units in
-            # use (one for the single chain, every unit for the chains side by side)
-            cus = 1 if nch == 1 else min(ncu, nch)
+            # oracle's iteration counts: one solve per step) are priced against the LDS bandwidth of
+            # every compute unit (the single chain runs in two batched passes over all the steps,
+            # the chains side by side in one launch)
+            cus = ncu
             for nm, w in (("fp32", 4), ("fp64", 8)):
                 b = algorithmic_bytes(info, itr[conv], w)
                 t = res[nm]["solver_time_s"]

@@ -243,7 +243,15 @@ int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst
  * single chain). The head of chain c starts from V0 (row c if v0_stride == n, the shared row if
  * 0); every later step starts from the previous step's converged V. A chain stops at its first
  * step that does not converge; its later steps get PFGPU_STATUS_NOT_RUN, 0 iterations and zero
- * voltages. All chains run in one launch, the steps of a chain one after the other inside it.
+ * voltages. Several chains run in one launch, the steps of a chain one after the other inside it.
+ * A single chain of 4 steps or more runs in batched passes instead: pass 1 solves every step from
+ * the head, pass 2 solves step k from pass 1's step k - 1. Up to the first step whose two passes
+ * did not converge to the same voltages (within 1e-6 pu), each pass-2 step starts where the chain's
+ * own step would, to within what two converged solves of one step differ, so it takes the same
+ * iterations to the same solution (within the tolerance); the rest is solved again from that
+ * step, and after 8 such rounds as a sequential chain. Results equal the step-by-step chain's in
+ * converged flags and iterations, and in voltages to within the tolerance; the step-by-step chain
+ * runs instead when PFGPU_TS_SEQUENTIAL=1 is set in the environment.
  *   Yx      shared Ybus values [nnz_ybus complex]
  *   Sbus    [nrows x n complex]; V [nrows x n complex] out; iters / status [nrows] out
  */
@@ -257,7 +265,8 @@ int pfgpu_solve_chains_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int c
                               int* d_status, pfgpu_stats* stats);
 
 /* Per-row PFGPU_FLAG_* of the last solve call on the context (of its first pass: the FP32
- * launch for PFGPU_FP32); nrows must not exceed that call's rows. */
+ * launch for PFGPU_FP32; for a single chain in batched passes, of the pass that gave each row);
+ * nrows must not exceed that call's rows. */
 int pfgpu_ctx_last_flags(pfgpu_ctx* ctx, int* flags, long long nrows);
 
 /*

@@ -69,6 +69,9 @@ struct pfgpu_ctx {
     // host-buffer entry points: device copies of the caller's arrays (grow-only)
     void* stage = nullptr;
     size_t stage_bytes = 0;
+    // two-pass single chains: the first pass's rows, the second pass's starts, flags (grow-only)
+    void* chain = nullptr;
+    size_t chain_bytes = 0;
     long long last_rows = 0;            // rows of the last solve call
     const int* last_flags = nullptr;    // and their flags (device)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

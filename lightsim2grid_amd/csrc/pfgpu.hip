@@ -1386,6 +1386,57 @@ __global__ void __launch_bounds__(1024) k_collect_fallback(const int* status, co
     if (tid == 0) *count = base;
 }
 
+// Two-pass single chains (see solve_single_chain). Starts of the second pass: row 0 from the
+// chain's head, row k from the first pass's row k - 1.
+__global__ void k_chain_starts(const double* head, const double* V1, double* V0p, long long rows, int n) {
+    const long long total = rows * n;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long r = k / n;
+        const int b = (int)(k - r * n);
+        const double* src = r == 0 ? head + 2 * b : V1 + 2 * ((r - 1) * n + b);
+        V0p[2 * k] = src[0];
+        V0p[2 * k + 1] = src[1];
+    }
+}
+
+// agree[r] = 1 when row r converged in both passes to the same voltages (max |V1 - V2| <= eps),
+// one workgroup of 256 lanes per row
+__global__ void __launch_bounds__(256) k_rows_agree(const double* V1, const double* V2, const int* st1,
+                                                    const int* st2, long long rows, int n, double eps, int* agree) {
+    __shared__ int bad;
+    for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+        if (threadIdx.x == 0) bad = (st1[r] != 0 || st2[r] != 0) ? 1 : 0;
+        __syncthreads();
+        int mine = 0;
+        for (int b = threadIdx.x; b < n; b += blockDim.x) {
+            const double dr = V1[2 * (r * n + b)] - V2[2 * (r * n + b)];
+            const double di = V1[2 * (r * n + b) + 1] - V2[2 * (r * n + b) + 1];
+            mine |= !(dr * dr + di * di <= eps * eps);  // a NaN disagrees
+        }
+        if (mine) atomicOr(&bad, 1);
+        __syncthreads();
+        if (threadIdx.x == 0) agree[r] = bad ? 0 : 1;
+        __syncthreads();
+    }
+}
+
+// Rows [r0, r1) of a chain after its first failure: not run, 0 iterations, zero voltages and flags.
+__global__ void k_chain_not_run(double* V, int* iters, int* status, int* flags, long long r0, long long r1, int n) {
+    const long long total = (r1 - r0) * n;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        V[2 * (r0 * n + k)] = 0.0;
+        V[2 * (r0 * n + k) + 1] = 0.0;
+        if (k % n == 0) {
+            const long long r = r0 + k / n;
+            iters[r] = 0;
+            status[r] = kStatusNotRun;
+            if (flags) flags[r] = 0;
+        }
+    }
+}
+
 // Work-queue order for the next launch of a batch of the same size: the instances by decreasing
 // iteration count of this launch (longest first), ties in index order. Taking the longest solves
 // first shortens the end of a launch, where a few long solves (diverging outages run max_iter
@@ -2255,6 +2306,135 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
 
 }  // namespace
 
+// A single warm-started chain (TimeSeries.cpp:80-103) in batched passes instead of one step after
+// the other. Pass 1 solves every step from the chain's head. Pass 2 solves step k from pass 1's
+// step k - 1 (step 0 from the head). Step 1 of pass 2 is then the sequential chain's step 1 exactly,
+// and by induction step k is the chain's step k as long as every earlier step converged in both
+// passes to the same voltages (to within kChainAgree): its start differs from the chain's by what
+// two converged solves of one step differ, far below the tolerance, so it takes the same NR
+// iterations to the same solution. At the first step that fails in pass 2 (with a valid start) the
+// chain stops, as the reference's loop does; at the first step whose passes disagree (pass 1
+// diverged from the head, or reached another solution), the rest is solved again the same way with
+// that step's pass-2 voltages as the head. After kChainRounds rounds, or for a short rest, the rest
+// runs as a sequential chain. Per-row flags are gathered for pfgpu_ctx_last_flags.
+constexpr long long kChainTwoPassMin = 4;  // shorter chains run sequentially
+constexpr int kChainRounds = 8;
+constexpr double kChainAgree = 1e-6;       // pu: two converged solves of one step agree
+
+bool chain_two_pass_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("PFGPU_TS_SEQUENTIAL");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
+void add_stats(pfgpu_stats* st, const pfgpu_stats& s) {
+    if (!st) return;
+    st->kernel_s += s.kernel_s;
+    st->fallback_fp64 += s.fallback_fp64;
+    st->repivot_rows += s.repivot_rows;
+    st->graph_launches += s.graph_launches;
+    for (int k = 0; k < 6; ++k) st->phase_s[k] += s.phase_s[k];
+    st->instances_per_wg = s.instances_per_wg;
+    st->lds_slots = s.lds_slots;
+    st->workgroups = s.workgroups;
+    st->block_size = s.block_size;
+    st->resident_workgroups = s.resident_workgroups;
+}
+
+void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgpu_stats* st) {
+    const int n = gr->P.n;
+    const long long T = q.nrows;
+    const size_t row = (size_t)n * 2;  // doubles per voltage row
+    // workspace: V1 and the pass-2 starts [T][n] complex, pass-1 iterations / statuses, agreement,
+    // the chain's flags [T] ints
+    const size_t vbytes = align_up((size_t)T * n * 16), ibytes = align_up((size_t)T * 4);
+    char* w = static_cast<char*>(grow_buffer(ctx->chain, ctx->chain_bytes, 2 * vbytes + 4 * ibytes));
+    double* V1 = reinterpret_cast<double*>(w);
+    double* V0p = reinterpret_cast<double*>(w + vbytes);
+    int* it1 = reinterpret_cast<int*>(w + 2 * vbytes);
+    int* st1 = reinterpret_cast<int*>(w + 2 * vbytes + ibytes);
+    int* agree = reinterpret_cast<int*>(w + 2 * vbytes + 2 * ibytes);
+    int* flagsT = reinterpret_cast<int*>(w + 2 * vbytes + 3 * ibytes);
+    hipStream_t s = ctx->stream;
+    const double* head = q.V0;  // the chain's head row
+    long long r0 = 0;
+    int rounds = 0;
+    auto keep_flags = [&](long long at, long long rows) {
+        if (ctx->last_flags && rows > 0)
+            HIP_OK(hipMemcpyAsync(flagsT + at, ctx->last_flags, (size_t)rows * 4, hipMemcpyDeviceToDevice, s));
+    };
+    auto grid_of = [](long long total) { return (int)std::min<long long>(std::max<long long>((total + 255) / 256, 1), 4096); };
+    while (r0 < T) {
+        const long long R = T - r0;
+        if (R < kChainTwoPassMin || rounds >= kChainRounds) {
+            // the rest as a sequential chain from the head
+            SolveArgs c = q;
+            c.ninst = 1; c.chain = (int)R; c.nrows = R;
+            c.S = q.S + r0 * row; c.V0 = head; c.v0_stride = 0;
+            c.V = q.V + r0 * row; c.iters = q.iters + r0; c.status = q.status + r0;
+            pfgpu_stats sc{};
+            solve_device(ctx, gr, c, st ? &sc : nullptr);
+            add_stats(st, sc);
+            keep_flags(r0, R);
+            break;
+        }
+        ++rounds;
+        // pass 1: every remaining step from the head
+        SolveArgs p1 = q;
+        p1.ninst = (int)R; p1.chain = 1; p1.nrows = R; p1.Yx_stride = 0;  // every step on the chain's Ybus
+        p1.S = q.S + r0 * row; p1.sbus_stride = n; p1.V0 = head; p1.v0_stride = 0;
+        p1.V = V1; p1.iters = it1; p1.status = st1;
+        pfgpu_stats s1{};
+        solve_device(ctx, gr, p1, st ? &s1 : nullptr);
+        add_stats(st, s1);
+        // pass 2: step k from pass 1's step k - 1, step 0 from the head
+        hipLaunchKernelGGL(k_chain_starts, dim3(grid_of(R * n)), dim3(256), 0, s, head, V1, V0p, R, n);
+        HIP_OK(hipGetLastError());
+        SolveArgs p2 = p1;
+        p2.V0 = V0p; p2.v0_stride = n;
+        p2.V = q.V + r0 * row; p2.iters = q.iters + r0; p2.status = q.status + r0;
+        pfgpu_stats s2{};
+        solve_device(ctx, gr, p2, st ? &s2 : nullptr);
+        add_stats(st, s2);
+        keep_flags(r0, R);
+        hipLaunchKernelGGL(k_rows_agree, dim3((unsigned)std::min<long long>(R, 4096)), dim3(256), 0, s, V1, p2.V, st1,
+                           p2.status, R, n, kChainAgree, agree);
+        HIP_OK(hipGetLastError());
+        std::vector<int> st2h((size_t)R), agh((size_t)R);
+        HIP_OK(hipMemcpyAsync(st2h.data(), p2.status, (size_t)R * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(agh.data(), agree, (size_t)R * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        long long k = 0;
+        while (k < R && st2h[k] == 0 && agh[k]) ++k;
+        if (k == R) break;  // every step agrees: pass 2 is the chain
+        if (st2h[k] != 0) {
+            // the chain's first failure (its start is valid): the later steps are not run
+            if (k + 1 < R) {
+                hipLaunchKernelGGL(k_chain_not_run, dim3(grid_of((R - k - 1) * n)), dim3(256), 0, s, q.V, q.iters,
+                                   q.status, flagsT, r0 + k + 1, T, n);
+                HIP_OK(hipGetLastError());
+            }
+            break;
+        }
+        // step k converged in pass 2 but its passes disagree: steps after it start again from it
+        head = q.V + (r0 + k) * row;
+        r0 += k + 1;
+    }
+    ctx->last_rows = T;
+    ctx->last_flags = flagsT;
+}
+
+// The solve of a call: a single chain of enough steps in two batched passes, anything else as is.
+void solve_call(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgpu_stats* st) {
+    if (q.ninst == 1 && q.chain > 1 && q.nrows >= kChainTwoPassMin && !q.Jx_last && !q.dptr &&
+        !gr->dc && chain_two_pass_enabled())
+        solve_single_chain(ctx, gr, q, st);
+    else
+        solve_device(ctx, gr, q, st);
+}
+
 namespace {
 // B' and B'' of a fast-decoupled grid (n x n, CSC)
 struct FdpfB {
@@ -2305,7 +2485,7 @@ int pfgpu_ctx_destroy(pfgpu_ctx* ctx) {
         if (!ctx) return 0;
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-        for (void* p : {ctx->ws, ctx->aux, ctx->io, ctx->stage})
+        for (void* p : {ctx->ws, ctx->aux, ctx->io, ctx->stage, ctx->chain})
             if (p) (void)hipFree(p);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -2796,7 +2976,7 @@ int pfgpu_solve_chains_device(pfgpu_ctx* ctx, pfgpu_grid* grid, int nrows, int c
         q.Yx = d_Yx; q.Yx_stride = Yx_stride; q.S = d_Sbus; q.sbus_stride = grid->P.n; q.V0 = d_V0;
         q.v0_stride = v0_stride; q.V = d_V; q.max_iter = max_iter; q.tol = tol; q.precision = precision;
         q.G_req = instances_per_wg; q.iters = d_iters; q.status = d_status;
-        solve_device(ctx, grid, q, stats);
+        solve_call(ctx, grid, q, stats);
         if (stats) stats->total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return 0;
     });
@@ -2861,7 +3041,7 @@ int solve_host(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst, int chain, long long
     q.max_iter = max_iter; q.tol = tol; q.precision = precision; q.G_req = instances_per_wg;
     q.iters = reinterpret_cast<int*>(b + o_it); q.status = reinterpret_cast<int*>(b + o_st);
     q.Jx_last = Jx_last ? reinterpret_cast<double*>(b + o_j) : nullptr;
-    solve_device(ctx, grid, q, &st);
+    solve_call(ctx, grid, q, &st);
     auto t2 = clk::now();
     HIP_OK(hipMemcpyAsync(V, b + o_v, nv, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(iters, b + o_it, (size_t)nrows * 4, hipMemcpyDeviceToHost, s));

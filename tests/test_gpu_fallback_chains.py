@@ -5,7 +5,9 @@
   klu_factor / klu_refactor report them (src/linear_solvers/KLUSolver.cpp:38-57);
 * one solver reused across changed pv / pq sets (BaseNRAlgo.hpp:155-166);
 * time-series chains inside one launch (TimeSeries.cpp:80-103), the reference's single chain
-  and several chains side by side, including a chain that fails.
+  and several chains side by side, including a chain that fails;
+* the single chain's two batched passes when steps solved from the head diverge where the
+  warm-started chain converges (restarts, the sequential rest, a failure after restarts).
 """
 import sys
 
@@ -210,6 +212,46 @@ def test_time_series_chain_stops_at_first_failure(gpu_ctx, oracle_mod, prec):
     r2 = gr.solve_chains(gr.ybus_values(d["Ybus"]), Sb, g.v0(), chain_len=12, precision=prec)
     assert r2["status"][9] != 0 and np.all(r2["status"][10:12] == abi.STATUS_NOT_RUN)
     assert np.all(r2["status"][12:] == 0)
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("T,amp,max_iter", [(40, 0.6, 3), (40, 0.3, 2), (80, 1.0, 2)])
+def test_single_chain_restarts_match_sequential_chain(gpu_ctx, oracle_mod, prec, T, amp, max_iter):
+    """A load ramp (Sbus x (1 + amp t / T)) from the converged base case with a small max_iter: the
+    steps solved from the head diverge after the first few while the warm-started chain converges
+    (40, 0.6, 3: one restart; 40, 0.3, 2: restarts, then the sequential rest; 80, 1.0, 2: the
+    chain itself fails at step 58 after restarts). Every step equals the oracle's sequential chain:
+    flags, iterations (FP64: equal; FP32: within 1), voltages, and the not-run rows."""
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make("case118_synth")
+    d = g.solver_inputs()
+    o = oracle_mod.OracleNR(False)
+    assert o.compute_pf(d["Ybus"], d["V0"], d["Sbus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"], 10, 1e-8)
+    Vb = o.get_V()
+    S0 = np.asarray(d["Sbus"])
+    Sb = np.stack([S0 * (1 + amp * t / T) for t in range(T)])
+    _say("two-pass restarts", T, amp, max_iter, prec)
+    Vr, itr, _, convr, status, _ = oracle_mod.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
+                                                          d["pq"], Vb, max_iter, 1e-8, False, 1)
+    # from the head alone, later steps diverge (the case the restarts are for)
+    _, _, _, conv_head, _, _ = oracle_mod.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
+                                                      d["pq"], Vb, max_iter, 1e-8, False, T)
+    assert not conv_head[3 * T // 4:].any()
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    r = gr.solve_chains(gr.ybus_values(d["Ybus"]), Sb, Vb, chain_len=None, max_iter=max_iter, precision=prec)
+    conv = r["status"] == 0
+    assert np.array_equal(conv[convr], np.ones(int(convr.sum()), bool))
+    if convr.all():
+        assert conv.all()
+    else:
+        f = int(np.argmin(convr))  # the oracle chain's first failure
+        assert conv[:f].all() and not conv[f]
+        assert np.all(r["status"][f + 1:] == abi.STATUS_NOT_RUN) and np.all(r["iters"][f + 1:] == 0)
+        assert np.all(r["V"][f + 1:] == 0)
+    tol_it = 0 if prec == FP64 else 1
+    assert np.max(np.abs(r["iters"][convr] - itr[convr])) <= tol_it
+    assert np.max(np.abs(r["V"][convr] - Vr[convr])) <= (1e-8 if prec == FP64 else 5e-8)
+    gr.close()
 
 
 def test_phase_timers_cover_the_kernel(gpu_ctx):
