@@ -444,6 +444,8 @@ def main():
     ap.add_argument("--single-solve", metavar="GRID", default=None,
                     help="only time single solves of GRID from its V0 (default launch, then one workgroup) "
                          "and print one JSON line")
+    ap.add_argument("--time-series", action="store_true",
+                    help="only run the 118-bus time-series leg (GPU only) and print one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the N-1 sweep would shard by "
@@ -452,6 +454,9 @@ def main():
 
     from lightsim2grid_amd import abi, gridgen
 
+    if args.time_series:
+        print(json.dumps(time_series_run(False, 0)))
+        return
     if args.single_solve:
         print(json.dumps(single_solves(args.single_solve)))
         return

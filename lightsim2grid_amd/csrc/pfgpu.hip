@@ -1438,40 +1438,31 @@ __global__ void k_chain_not_run(double* V, int* iters, int* status, int* flags, 
 }
 
 // Work-queue order for the next launch of a batch of the same size: the instances by decreasing
-// iteration count of this launch (longest first), ties in index order. Taking the longest solves
+// iteration count of this launch (longest first; ties in any order). Taking the longest solves
 // first shortens the end of a launch, where a few long solves (diverging outages run max_iter
 // iterations) would otherwise start late and leave the other workgroups idle. One workgroup of
-// 1024 lanes, one compaction pass per iteration count.
+// 1024 lanes: a counting sort (a histogram of the counts in LDS, offsets by decreasing count, then
+// every instance placed at its count's next offset), two passes over the counts.
 __global__ void __launch_bounds__(1024) k_order_by_iters(const int* iters, long long n, int nkeys, int* perm) {
-    __shared__ int wsum[16];
-    __shared__ int base;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) base = 0;
+    __shared__ int cnt[64], cur[64];
+    const int tid = threadIdx.x;
+    auto key = [&](long long u) {
+        const int it = iters[u];
+        return it >= nkeys - 1 ? nkeys - 1 : (it < 0 ? 0 : it);
+    };
+    if (tid < 64) cnt[tid] = 0;
     __syncthreads();
-    for (int k = nkeys - 1; k >= 0; --k) {
-        for (long long u0 = 0; u0 < n; u0 += 1024) {
-            const long long u = u0 + tid;
-            bool take = false;
-            if (u < n) {
-                const int it = iters[u];
-                take = (it >= nkeys - 1 ? nkeys - 1 : (it < 0 ? 0 : it)) == k;
-            }
-            const unsigned long long m = __ballot(take);
-            const int before = __popcll(m & ((1ull << lane) - 1ull));
-            if (lane == 0) wsum[wave] = __popcll(m);
-            __syncthreads();
-            int off = base;
-            for (int w = 0; w < wave; ++w) off += wsum[w];
-            if (take) perm[off + before] = (int)u;
-            __syncthreads();
-            if (tid == 0) {
-                int t = 0;
-                for (int w = 0; w < 16; ++w) t += wsum[w];
-                base += t;
-            }
-            __syncthreads();
+    for (long long u = tid; u < n; u += 1024) atomicAdd(&cnt[key(u)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int off = 0;
+        for (int k = nkeys - 1; k >= 0; --k) {
+            cur[k] = off;
+            off += cnt[k];
         }
     }
+    __syncthreads();
+    for (long long u = tid; u < n; u += 1024) perm[atomicAdd(&cur[key(u)], 1)] = (int)u;
 }
 
 // Branch flows at the origin side, one (instance, branch) per thread.
