@@ -2307,10 +2307,13 @@ void solve_device(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q0, pfgpu_sta
 // chain stops, as the reference's loop does; at the first step whose passes disagree (pass 1
 // diverged from the head, or reached another solution), the rest is solved again the same way with
 // that step's pass-2 voltages as the head. After kChainRounds rounds, or for a short rest, the rest
-// runs as a sequential chain. Per-row flags are gathered for pfgpu_ctx_last_flags.
+// runs as a sequential chain. A long chain of a large grid is taken in windows of steps whose pass
+// buffers fit a quarter of the context's HBM budget (up to kChainWindowBytes), each window's head
+// being the previous window's last step. Per-row flags are gathered for pfgpu_ctx_last_flags.
 constexpr long long kChainTwoPassMin = 4;  // shorter chains run sequentially
 constexpr int kChainRounds = 8;
 constexpr double kChainAgree = 1e-6;       // pu: two converged solves of one step agree
+constexpr size_t kChainWindowBytes = (size_t)1 << 30;
 
 bool chain_two_pass_enabled() {
     static const bool on = [] {
@@ -2338,10 +2341,14 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
     const int n = gr->P.n;
     const long long T = q.nrows;
     const size_t row = (size_t)n * 2;  // doubles per voltage row
-    // workspace: V1 and the pass-2 starts [T][n] complex, pass-1 iterations / statuses, agreement,
-    // the chain's flags [T] ints
-    const size_t vbytes = align_up((size_t)T * n * 16), ibytes = align_up((size_t)T * 4);
-    char* w = static_cast<char*>(grow_buffer(ctx->chain, ctx->chain_bytes, 2 * vbytes + 4 * ibytes));
+    // steps per window: V1 and the pass-2 starts take 32 n bytes per step
+    const size_t cap = std::min(kChainWindowBytes, ctx->budget / 4);
+    const long long W = std::max<long long>(kChainTwoPassMin, std::min<long long>(T, (long long)(cap / ((size_t)n * 32 + 16))));
+    // workspace: V1 and the pass-2 starts [W][n] complex, pass-1 iterations / statuses, agreement
+    // [W] ints, the chain's flags [T] ints
+    const size_t vbytes = align_up((size_t)W * n * 16), ibytes = align_up((size_t)W * 4);
+    const size_t fbytes = align_up((size_t)T * 4);
+    char* w = static_cast<char*>(grow_buffer(ctx->chain, ctx->chain_bytes, 2 * vbytes + 3 * ibytes + fbytes));
     double* V1 = reinterpret_cast<double*>(w);
     double* V0p = reinterpret_cast<double*>(w + vbytes);
     int* it1 = reinterpret_cast<int*>(w + 2 * vbytes);
@@ -2358,8 +2365,8 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
     };
     auto grid_of = [](long long total) { return (int)std::min<long long>(std::max<long long>((total + 255) / 256, 1), 4096); };
     while (r0 < T) {
-        const long long R = T - r0;
-        if (R < kChainTwoPassMin || rounds >= kChainRounds) {
+        if (T - r0 < kChainTwoPassMin || rounds >= kChainRounds) {
+            const long long R = T - r0;
             // the rest as a sequential chain from the head
             SolveArgs c = q;
             c.ninst = 1; c.chain = (int)R; c.nrows = R;
@@ -2371,8 +2378,8 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
             keep_flags(r0, R);
             break;
         }
-        ++rounds;
-        // pass 1: every remaining step from the head
+        const long long R = std::min(T - r0, W);  // this window's steps
+        // pass 1: every step of the window from the head
         SolveArgs p1 = q;
         p1.ninst = (int)R; p1.chain = 1; p1.nrows = R; p1.Yx_stride = 0;  // every step on the chain's Ybus
         p1.S = q.S + r0 * row; p1.sbus_stride = n; p1.V0 = head; p1.v0_stride = 0;
@@ -2399,11 +2406,17 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
         HIP_OK(hipStreamSynchronize(s));
         long long k = 0;
         while (k < R && st2h[k] == 0 && agh[k]) ++k;
-        if (k == R) break;  // every step agrees: pass 2 is the chain
+        if (k == R) {
+            // every step of the window agrees: pass 2 is the chain; the next window starts from its last
+            // step
+            head = q.V + (r0 + R - 1) * row;
+            r0 += R;
+            continue;
+        }
         if (st2h[k] != 0) {
             // the chain's first failure (its start is valid): the later steps are not run
-            if (k + 1 < R) {
-                hipLaunchKernelGGL(k_chain_not_run, dim3(grid_of((R - k - 1) * n)), dim3(256), 0, s, q.V, q.iters,
+            if (r0 + k + 1 < T) {
+                hipLaunchKernelGGL(k_chain_not_run, dim3(grid_of((T - r0 - k - 1) * n)), dim3(256), 0, s, q.V, q.iters,
                                    q.status, flagsT, r0 + k + 1, T, n);
                 HIP_OK(hipGetLastError());
             }
@@ -2412,6 +2425,7 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
         // step k converged in pass 2 but its passes disagree: steps after it start again from it
         head = q.V + (r0 + k) * row;
         r0 += k + 1;
+        ++rounds;
     }
     ctx->last_rows = T;
     ctx->last_flags = flagsT;

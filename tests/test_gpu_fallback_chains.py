@@ -214,14 +214,17 @@ def test_time_series_chain_stops_at_first_failure(gpu_ctx, oracle_mod, prec):
     assert np.all(r2["status"][12:] == 0)
 
 
+@pytest.mark.parametrize("window", [False, True])
 @pytest.mark.parametrize("prec", [FP64, FP32])
 @pytest.mark.parametrize("T,amp,max_iter", [(40, 0.6, 3), (40, 0.3, 2), (80, 1.0, 2)])
-def test_single_chain_restarts_match_sequential_chain(gpu_ctx, oracle_mod, prec, T, amp, max_iter):
+def test_single_chain_restarts_match_sequential_chain(gpu_ctx, oracle_mod, prec, T, amp, max_iter, window):
     """A load ramp (Sbus x (1 + amp t / T)) from the converged base case with a small max_iter: the
     steps solved from the head diverge after the first few while the warm-started chain converges
     (40, 0.6, 3: one restart; 40, 0.3, 2: restarts, then the sequential rest; 80, 1.0, 2: the
-    chain itself fails at step 58 after restarts). Every step equals the oracle's sequential chain:
-    flags, iterations (FP64: equal; FP32: within 1), voltages, and the not-run rows."""
+    chain itself fails at step 58 after restarts). window: a context whose HBM budget holds the
+    pass buffers of 19 steps (a quarter of it), so the chain is taken in windows. Every step equals the oracle's
+    sequential chain: flags, iterations (FP64: equal; FP32: within 1), voltages, and the not-run
+    rows."""
     from lightsim2grid_amd import abi, gridgen
     g = gridgen.make("case118_synth")
     d = g.solver_inputs()
@@ -237,7 +240,8 @@ def test_single_chain_restarts_match_sequential_chain(gpu_ctx, oracle_mod, prec,
     _, _, _, conv_head, _, _ = oracle_mod.time_series(d["Ybus"], Sb, d["slack_ids"], d["slack_weights"], d["pv"],
                                                       d["pq"], Vb, max_iter, 1e-8, False, T)
     assert not conv_head[3 * T // 4:].any()
-    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    ctx = abi.Context(0, 4 * 19 * (32 * g.n + 16)) if window else gpu_ctx
+    gr = abi.Grid(ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
     r = gr.solve_chains(gr.ybus_values(d["Ybus"]), Sb, Vb, chain_len=None, max_iter=max_iter, precision=prec)
     conv = r["status"] == 0
     assert np.array_equal(conv[convr], np.ones(int(convr.sum()), bool))
@@ -252,6 +256,8 @@ def test_single_chain_restarts_match_sequential_chain(gpu_ctx, oracle_mod, prec,
     assert np.max(np.abs(r["iters"][convr] - itr[convr])) <= tol_it
     assert np.max(np.abs(r["V"][convr] - Vr[convr])) <= (1e-8 if prec == FP64 else 5e-8)
     gr.close()
+    if window:
+        ctx.close()
 
 
 def test_phase_timers_cover_the_kernel(gpu_ctx):
