@@ -249,7 +249,9 @@ int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst
  * did not converge to the same voltages (within 1e-6 pu), each pass-2 step starts where the chain's
  * own step would, to within what two converged solves of one step differ, so it takes the same
  * iterations to the same solution (within the tolerance); the rest is solved again from that
- * step, and after 8 such rounds as a sequential chain. Results equal the step-by-step chain's in
+ * step, and after 8 such rounds as a sequential chain (a long chain of a large grid is taken in
+ * windows whose pass buffers fit a quarter of the context's HBM budget). Results equal the
+ * step-by-step chain's in
  * converged flags and iterations, and in voltages to within the tolerance; the step-by-step chain
  * runs instead when PFGPU_TS_SEQUENTIAL=1 is set in the environment.
  *   Yx      shared Ybus values [nnz_ybus complex]
