@@ -251,9 +251,9 @@ int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst
  * iterations to the same solution (within the tolerance); the rest is solved again from that
  * step, and after 8 such rounds as a sequential chain (a long chain of a large grid is taken in
  * windows whose pass buffers fit a quarter of the context's HBM budget). Results equal the
- * step-by-step chain's in
- * converged flags and iterations, and in voltages to within the tolerance; the step-by-step chain
- * runs instead when PFGPU_TS_SEQUENTIAL=1 is set in the environment.
+ * step-by-step chain's in converged flags and iterations, and in voltages to within the
+ * tolerance; the step-by-step chain runs instead when PFGPU_TS_SEQUENTIAL=1 is set in the
+ * environment.
  *   Yx      shared Ybus values [nnz_ybus complex]
  *   Sbus    [nrows x n complex]; V [nrows x n complex] out; iters / status [nrows] out
  */
