@@ -244,8 +244,9 @@ int pfgpu_solve_batch_device_noalloc(pfgpu_ctx* ctx, pfgpu_grid* grid, int ninst
  * 0); every later step starts from the previous step's converged V. A chain stops at its first
  * step that does not converge; its later steps get PFGPU_STATUS_NOT_RUN, 0 iterations and zero
  * voltages. Several chains run in one launch, the steps of a chain one after the other inside it.
- * A single chain of 4 steps or more runs in batched passes instead: pass 1 solves every step from
- * the head, pass 2 solves step k from pass 1's step k - 1. Up to the first step whose two passes
+ * A single chain of 4 steps or more (tol <= 1e-6) runs in batched passes instead: pass 1 solves
+ * every step from the head, pass 2 solves step k from pass 1's step k - 1. Up to the first step
+ * whose two passes
  * did not converge to the same voltages (within 1e-6 pu), each pass-2 step starts where the chain's
  * own step would, to within what two converged solves of one step differ, so it takes the same
  * iterations to the same solution (within the tolerance); the rest is solved again from that

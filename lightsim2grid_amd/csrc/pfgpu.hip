@@ -2432,9 +2432,12 @@ void solve_single_chain(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgp
 }
 
 // The solve of a call: a single chain of enough steps in two batched passes, anything else as is.
+// The passes need a tolerance tight enough that two converged solves of one step are equivalent
+// starts for the next one (kChainTwoPassTol); with a looser one the chain runs step by step.
+constexpr double kChainTwoPassTol = 1e-6;
 void solve_call(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, pfgpu_stats* st) {
     if (q.ninst == 1 && q.chain > 1 && q.nrows >= kChainTwoPassMin && !q.Jx_last && !q.dptr &&
-        !gr->dc && chain_two_pass_enabled())
+        q.tol <= kChainTwoPassTol && !gr->dc && chain_two_pass_enabled())
         solve_single_chain(ctx, gr, q, st);
     else
         solve_device(ctx, gr, q, st);
