@@ -624,7 +624,7 @@ void build_jfill_table(DevicePlan& P) {
 // by the mismatch of the previous iteration, read by the update) keep their slots;
 // every other value takes a physical slot freed by a value whose last read came at
 // an earlier time (greedy interval colouring, optimal for interval graphs).
-// Physical slots are then ordered tile first, then by decreasing hotness.
+// Physical slots are then ordered tile first, then by decreasing accesses per iteration.
 // ---------------------------------------------------------------------------
 void recycle_slots(DevicePlan& P) {
     // one extra slot: the shared zero (structural zeros, and the operand of padding items)
@@ -661,10 +661,12 @@ void recycle_slots(DevicePlan& P) {
     fixed[zslot] = 1;
     // intervals
     std::vector<int> start(NV, 1 << 30), end(NV, -1), hot(NV, -1);
+    std::vector<long long> nacc(NV, 0);  // reads and writes of each value per iteration
     auto acc = [&](int s, int t, bool w) {
         end[s] = std::max(end[s], t);
         if (w) start[s] = std::min(start[s], t);
         if (t < T_END) hot[s] = std::max(hot[s], t);
+        ++nacc[s];
     };
     for (int q = 0; q < P.nJ; ++q) acc(P.j_slot[q] & ((1 << 30) - 1), 0, true);
     for (int s : P.zero_slots) acc(s, 0, true);
@@ -716,16 +718,23 @@ void recycle_slots(DevicePlan& P) {
         }
         freeq.push({end[s], phys[s]});
     }
-    // physical hotness and final order: tile first, then decreasing hotness
+    // final order: tile first, then by decreasing accesses per iteration (ties: decreasing
+    // hotness), so that a launch whose LDS holds only the first slots sends the fewest accesses
+    // to the spilled ones
     std::vector<int> phot(nphys, -1);
-    for (int s = 0; s < NV; ++s) phot[phys[s]] = std::max(phot[phys[s]], hot[s]);
+    std::vector<long long> pacc(nphys, 0);
+    for (int s = 0; s < NV; ++s) {
+        phot[phys[s]] = std::max(phot[phys[s]], hot[s]);
+        pacc[phys[s]] += nacc[s];
+    }
     for (int s = 0; s < NV; ++s)  // the zero slot's aliases are read-only zeros at all times
         if (zero[s]) { start[s] = -1; end[s] = T_END; }
     std::vector<int> pord, fin(nphys, -1);
     for (int s = 0; s < tile; ++s) fin[phys[s]] = s;
     for (int ph = 0; ph < nphys; ++ph)
         if (fin[ph] < 0) pord.push_back(ph);
-    std::stable_sort(pord.begin(), pord.end(), [&](int a, int b) { return phot[a] > phot[b]; });
+    std::stable_sort(pord.begin(), pord.end(),
+                     [&](int a, int b) { return pacc[a] != pacc[b] ? pacc[a] > pacc[b] : phot[a] > phot[b]; });
     for (size_t i = 0; i < pord.size(); ++i) fin[pord[i]] = tile + (int)i;
     std::vector<int> mp(NV);
     for (int s = 0; s < NV; ++s) mp[s] = fin[phys[s]];

@@ -111,10 +111,10 @@ struct Schedule {
 
 Schedule build_schedule(const SymbolicLU& S);
 
-// Flattened arrays uploaded to the device. Value slots are renumbered by
-// "hotness" (the highest level at which a slot is read or written comes
-// first), so that a launch can keep the first `nlds` slots of every instance
-// in LDS and spill the rest to a per-instance global buffer.
+// Flattened arrays uploaded to the device. Value slots are renumbered by their
+// accesses per NR iteration (the most read and written first, after the dense
+// tile), so that a launch can keep the first `nlds` slots of every instance in
+// LDS and spill the rest to a per-instance global buffer.
 struct DevicePlan {
     int n = 0, N = 0, off = 0, nnzY = 0, nJ = 0, nvals = 0, slack0 = 0;
     // dense tail: slots [0, m*(m+1)) hold the m x m block row-major with row stride m + 1,
