@@ -1679,7 +1679,7 @@ size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 // Launch heuristic: the largest G whose value arrays fit entirely in LDS while
 // keeping at least two workgroups per CU; otherwise G = 1 with the hottest
 // slots in LDS and the rest spilled to global memory, or, for a large batch of
-// a plan that spills even at G = 1, G = 4 (FP32) / 2 (FP64).
+// a plan of which more than half spills even at G = 1, G = 4 (FP32) / 2 (FP64).
 LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int ninst, int fsize, int G_req,
                            int NT_req = 0) {
     // per workgroup: static LDS of the variant + value slots + the level table
@@ -1705,11 +1705,12 @@ LaunchChoice choose_launch(const pfgpu_ctx* ctx, const pfgpu::DevicePlan& P, int
                 if (need <= avail(G, 1024) && ninst >= 4 * G * std::max(1, ctx->n_cu)) { c.G = G; break; }
             }
         }
-        // a plan that spills even at one instance per workgroup: large batches take several
-        // instances per workgroup (4 with FP32 factors, 2 with FP64), which hide the spilled slots'
-        // memory latency better (the autotuner's choice on the 10 000-bus N-1 sweep in both forms)
+        // a plan of which more than half spills even at one instance per workgroup: large batches
+        // take several instances per workgroup (4 with FP32 factors, 2 with FP64), which hide the
+        // spilled slots' memory latency better (the autotuner's choice on the 10 000-bus N-1 sweep in
+        // both forms; on the 2000-bus FP64 plan, a third of which spills, it keeps 1)
         const int Gs = fsize == 4 ? 4 : 2;
-        if (c.G == 1 && (size_t)P.nvals * fsize > avail(1, 1024) && ninst >= 2LL * Gs * std::max(1, ctx->n_cu))
+        if (c.G == 1 && (size_t)P.nvals * fsize > 2 * avail(1, 1024) && ninst >= 2LL * Gs * std::max(1, ctx->n_cu))
             c.G = Gs;
     }
     // the dense tail must be LDS-resident
