@@ -1492,6 +1492,24 @@ __global__ void k_branch_flows(int ninst, int n, const double* __restrict__ V, i
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// A graph capture on one thread's stream and a call that synchronises with the device
+// (hipFree, hipMalloc, a copy on the legacy stream) on another thread must not overlap: HIP fails
+// the call with hipErrorStreamCaptureImplicit (it would wait on the capturing stream). Several
+// contexts run on threads of their own (the serving loop), so every such call of this library and
+// every capture hold this lock.
+std::recursive_mutex& capture_mu() {
+    static std::recursive_mutex m;
+    return m;
+}
+hipError_t dev_malloc(void** p, size_t bytes) {
+    std::lock_guard<std::recursive_mutex> lk(capture_mu());
+    return hipMalloc(p, bytes);
+}
+hipError_t dev_free(void* p) {
+    std::lock_guard<std::recursive_mutex> lk(capture_mu());
+    return hipFree(p);
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -1500,18 +1518,19 @@ struct DevBuf {
         release();
         n = v.size();
         if (n == 0) return;
+        std::lock_guard<std::recursive_mutex> lk(capture_mu());
         HIP_OK(hipMalloc(&p, n * sizeof(T)));
         HIP_OK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)dev_free(p);
         p = nullptr;
         n = 0;
     }
     T* get_or_alloc(size_t count) {
         if (n < count) {
             release();
-            HIP_OK(hipMalloc(&p, count * sizeof(T)));
+            HIP_OK(dev_malloc((void**)&p, count * sizeof(T)));
             n = count;
         }
         return p;
@@ -1770,10 +1789,10 @@ void ensure_streams(pfgpu_grid* gr, int G, int NT, int fsize, bool lean) {
 
 void* grow_buffer(void*& p, size_t& cap, size_t bytes) {
     if (bytes > cap) {
-        if (p) HIP_OK(hipFree(p));
+        if (p) HIP_OK(dev_free(p));
         p = nullptr;
         cap = 0;
-        HIP_OK(hipMalloc(&p, bytes));
+        HIP_OK(dev_malloc(&p, bytes));
         cap = bytes;
     }
     return p;
@@ -2109,6 +2128,7 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
         if (!hit && std::find(gr->seen.begin(), gr->seen.end(), key) != gr->seen.end()) {
             // second launch of this sequence: capture it
             hipGraph_t g = nullptr;
+            std::lock_guard<std::recursive_mutex> lk(capture_mu());  // no device-wide call of another thread meanwhile
             HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
                 issue();
@@ -2158,7 +2178,8 @@ void launch_pass(pfgpu_ctx* ctx, pfgpu_grid* gr, const SolveArgs& q, int precisi
         st->resident_workgroups = nres;
         st->graph_launches += replayed ? 1 : 0;
         unsigned long long ph[8];
-        HIP_OK(hipMemcpy(ph, phase, sizeof(ph), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpyAsync(ph, phase, sizeof(ph), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
         for (int k = 0; k < 6; ++k) st->phase_s[k] += (double)ph[k] / ctx->wall_hz;
     }
 }
@@ -2470,6 +2491,7 @@ int pfgpu_ctx_create(int device, size_t hbm_budget_bytes, pfgpu_ctx** out) {
         HIP_OK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) throw ApiError(PFGPU_E_INVALID, "no such device");
         HIP_OK(hipSetDevice(device));
+        std::lock_guard<std::recursive_mutex> lk(capture_mu());
         scratch_init(device);
         auto c = std::make_unique<pfgpu_ctx>();
         c->device = device;
@@ -2494,8 +2516,9 @@ int pfgpu_ctx_destroy(pfgpu_ctx* ctx) {
         if (!ctx) return 0;
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        std::lock_guard<std::recursive_mutex> lk(capture_mu());
         for (void* p : {ctx->ws, ctx->aux, ctx->io, ctx->stage, ctx->chain})
-            if (p) (void)hipFree(p);
+            if (p) (void)dev_free(p);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -2837,7 +2860,7 @@ int pfgpu_dev_alloc(pfgpu_ctx* ctx, size_t bytes, void** dptr) {
         if (!ctx || !dptr) throw ApiError(PFGPU_E_INVALID, "NULL argument");
         HIP_OK(hipSetDevice(ctx->device));
         *dptr = nullptr;
-        HIP_OK(hipMalloc(dptr, std::max<size_t>(bytes, 1)));
+        HIP_OK(dev_malloc(dptr, std::max<size_t>(bytes, 1)));
         return 0;
     });
 }
@@ -2845,7 +2868,7 @@ int pfgpu_dev_free(pfgpu_ctx* ctx, void* dptr) {
     return guarded([&]() -> int {
         if (!ctx) throw ApiError(PFGPU_E_INVALID, "NULL ctx");
         HIP_OK(hipSetDevice(ctx->device));
-        if (dptr) HIP_OK(hipFree(dptr));
+        if (dptr) HIP_OK(dev_free(dptr));
         return 0;
     });
 }
@@ -2885,7 +2908,8 @@ int pfgpu_ctx_last_flags(pfgpu_ctx* ctx, int* flags, long long nrows) {
         HIP_OK(hipSetDevice(ctx->device));
         if (nrows > ctx->last_rows) throw ApiError(PFGPU_E_INVALID, "more rows than the last solve call had");
         if (nrows == 0) return 0;
-        HIP_OK(hipMemcpy(flags, ctx->last_flags, (size_t)nrows * 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpyAsync(flags, ctx->last_flags, (size_t)nrows * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
 }

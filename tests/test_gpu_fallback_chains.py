@@ -322,3 +322,57 @@ def test_graph_replays_equal_eager_launches(gpu_ctx):
             # eager work of another shape between the replays
             gr.solve_chains(Yx, Sb[: 3 + rep], d["V0"], chain_len=None, precision=prec)
         assert replays >= 4
+
+
+def test_graph_capture_beside_allocations_of_another_thread(gpu_ctx):
+    """One thread's solves capture HIP graphs (each launch sequence on its second call) while a
+    second thread, on a context of its own, allocates and frees device memory, as the serving
+    loop's workers do. HIP refuses a free that would wait on a capturing stream; the library keeps
+    its captures and such calls apart, so neither thread sees an error and every captured solve
+    gives the eager result bit for bit."""
+    import threading
+    from lightsim2grid_amd import abi, gridgen
+    g = gridgen.make("case118_synth")
+    d = g.solver_inputs()
+    _say("graph capture beside another thread's allocations")
+    gr = abi.Grid(gpu_ctx, d["Ybus"], d["slack_ids"], d["slack_weights"], d["pv"], d["pq"])
+    Yx = gr.ybus_values(d["Ybus"])
+    n, M = g.n, 48
+    dY, dS, dV0 = gpu_ctx.to_device(Yx), gpu_ctx.to_device(np.ascontiguousarray(d["Sbus"])), gpu_ctx.to_device(d["V0"])
+    dV = gpu_ctx.empty((M, n), np.complex128)
+    dIt, dSt = gpu_ctx.empty(M, np.int32), gpu_ctx.empty(M, np.int32)
+    other = abi.Context(0)
+    stop, errors, churns = threading.Event(), [], [0]
+
+    def churn():
+        try:
+            while not stop.is_set():
+                bufs = [other.empty(1 << 16, np.uint8) for _ in range(4)]
+                for b in bufs:
+                    abi._check(other._L.pfgpu_dev_free(other.h, b.ptr))
+                    b.ptr = None
+                churns[0] += 1
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    # a first solve of the fresh grid (its key differs from later calls of the same shape)
+    gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, M, dIt, dSt, 10, 1e-8, precision=FP64)
+    t = threading.Thread(target=churn)
+    t.start()
+    captured = 0
+    try:
+        for m in range(1, M):
+            outs = []
+            for _ in range(2):  # eager, then captured and replayed
+                st = gr.solve_device(dY, 0, dS, 0, dV0, 0, dV, m, dIt, dSt, 10, 1e-8, precision=FP64)
+                outs.append((dV.to_host()[:m], dIt.to_host()[:m], dSt.to_host()[:m]))
+            captured += st["graph_launches"]
+            for a, b in zip(outs[0], outs[1]):
+                assert np.array_equal(a, b)
+            assert (outs[0][2] == 0).all()
+    finally:
+        stop.set()
+        t.join()
+        other.close()
+    assert not errors, errors
+    assert captured == M - 1 and churns[0] > 0
