@@ -320,6 +320,18 @@ int pfgpu_batch_flows_device(pfgpu_ctx* ctx, long long nrows, int n, const doubl
                              double* d_amps, double* d_p);
 
 /*
+ * The same for rows solved by the DC solver (the !is_ac branches of compute_amps_flows /
+ * compute_active_power_flows, BaseBatchSolverSynch.hpp:96-135 and :151-186): d_yff / d_yft are the
+ * branches' DC coefficients (1/x and -1/x, ydc_11 / ydc_12), and
+ *   p = (Re(yff) arg(Ef) + Re(yft) arg(Et)) sn_mva,  amps = |p| / (sqrt(3) (|Ef| vn_kv)).
+ * A row of zero voltages gives p = 0 and NaN amps, as in the reference.
+ */
+int pfgpu_batch_flows_dc_device(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src,
+                                int nbr, const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
+                                const double* d_vn_kv_from, double sn_mva, const int* d_clean_ptr,
+                                const int* d_clean_br, double* d_amps, double* d_p);
+
+/*
  * Time-series Sbus rows on the device (TimeSeries::compute_Vs, TimeSeries.cpp:45-53):
  *   Sbus[t, b] = sum gen_p[t, g] + sum sgen_p[t, s] - sum load_p[t, l] - j sum load_q[t, l]
  * over the elements at solver bus b (CSR lists ptr[ns + 1] / element ids, in element order),

@@ -106,6 +106,8 @@ SIGNATURES = {
                                                  _vp, _vp, ctypes.c_double, _vp, _vp]),
     "pfgpu_batch_flows_device": (ctypes.c_int, [_vp, ctypes.c_longlong, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp,
                                                 _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "pfgpu_batch_flows_dc_device": (ctypes.c_int, [_vp, ctypes.c_longlong, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                                   _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "pfgpu_sbus_rows_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp,
                                               ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_double,
                                               _vp]),

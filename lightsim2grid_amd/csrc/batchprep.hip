@@ -32,7 +32,7 @@ __global__ void k_batch_flows(long long nrows, int n, const double* __restrict__
                               int nbr, const int* __restrict__ f, const int* __restrict__ t,
                               const double* __restrict__ yff, const double* __restrict__ yft,
                               const double* __restrict__ vn, double sn_mva, const int* __restrict__ cptr,
-                              const int* __restrict__ cbr, double* __restrict__ amps, double* __restrict__ p) {
+                              const int* __restrict__ cbr, double* __restrict__ amps, double* __restrict__ p, int dc) {
     const long long total = nrows * nbr;
     const double sqrt3 = 1.7320508075688772;
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
@@ -52,14 +52,20 @@ __global__ void k_batch_flows(long long nrows, int n, const double* __restrict__
                 eti = Vr[2 * t[b] + 1];
             }
             const double gr = yff[2 * b], gi = yff[2 * b + 1], hr = yft[2 * b], hi = yft[2 * b + 1];
-            // I = yff Ef + yft Et
-            const double ir = (gr * efr - gi * efi) + (hr * etr - hi * eti);
-            const double ii = (gr * efi + gi * efr) + (hr * eti + hi * etr);
-            // S = Ef conj(I)
-            const double sr = efr * ir + efi * ii;
-            const double si = efi * ir - efr * ii;
-            a = (hypot(sr, si) * sn_mva) / (sqrt3 * (hypot(efr, efi) * vn[b]));
-            pw = sr * sn_mva;
+            if (dc) {
+                // P = (Re(yff) arg(Ef) + Re(yft) arg(Et)) sn_mva, A = |P| / (sqrt(3) |Ef| vn_kv)
+                pw = (gr * atan2(efi, efr) + hr * atan2(eti, etr)) * sn_mva;
+                a = fabs(pw) / (sqrt3 * (hypot(efr, efi) * vn[b]));
+            } else {
+                // I = yff Ef + yft Et
+                const double ir = (gr * efr - gi * efi) + (hr * etr - hi * eti);
+                const double ii = (gr * efi + gi * efr) + (hr * eti + hi * etr);
+                // S = Ef conj(I)
+                const double sr = efr * ir + efi * ii;
+                const double si = efi * ir - efr * ii;
+                a = (hypot(sr, si) * sn_mva) / (sqrt3 * (hypot(efr, efi) * vn[b]));
+                pw = sr * sn_mva;
+            }
         }
         if (cptr)
             for (int q = cptr[r]; q < cptr[r + 1]; ++q)
@@ -249,14 +255,9 @@ size_t bfs_lds(int n) { return (size_t)(((n + 31) / 32 + 3) & ~3) * 4 + (size_t)
 
 int grid_of(long long work, int block) { return (int)std::max<long long>(1, std::min<long long>((work + block - 1) / block, 65536)); }
 
-}  // namespace
-
-extern "C" {
-
-int pfgpu_batch_flows_device(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src, int nbr,
-                             const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
-                             const double* d_vn_kv_from, double sn_mva, const int* d_clean_ptr, const int* d_clean_br,
-                             double* d_amps, double* d_p) {
+int batch_flows(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src, int nbr,
+                const int* d_f, const int* d_t, const double* d_yff, const double* d_yft, const double* d_vn_kv_from,
+                double sn_mva, const int* d_clean_ptr, const int* d_clean_br, double* d_amps, double* d_p, int dc) {
     return guarded([&]() -> int {
         if (!ctx || nrows < 0 || n <= 0 || nbr < 0 || (d_clean_ptr == nullptr) != (d_clean_br == nullptr) ||
             (nrows > 0 && nbr > 0 && (!d_V || !d_f || !d_t || !d_yff || !d_yft || !d_vn_kv_from)))
@@ -266,11 +267,30 @@ int pfgpu_batch_flows_device(pfgpu_ctx* ctx, long long nrows, int n, const doubl
         const long long total = nrows * nbr;
         if (total == 0 || (!d_amps && !d_p)) return 0;
         hipLaunchKernelGGL(k_batch_flows, dim3(grid_of(total, 256)), dim3(256), 0, ctx->stream, nrows, n, d_V, d_row_src,
-                           nbr, d_f, d_t, d_yff, d_yft, d_vn_kv_from, sn_mva, d_clean_ptr, d_clean_br, d_amps, d_p);
+                           nbr, d_f, d_t, d_yff, d_yft, d_vn_kv_from, sn_mva, d_clean_ptr, d_clean_br, d_amps, d_p, dc);
         HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
+}
+}  // namespace
+
+extern "C" {
+
+int pfgpu_batch_flows_device(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src, int nbr,
+                             const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
+                             const double* d_vn_kv_from, double sn_mva, const int* d_clean_ptr, const int* d_clean_br,
+                             double* d_amps, double* d_p) {
+    return batch_flows(ctx, nrows, n, d_V, d_row_src, nbr, d_f, d_t, d_yff, d_yft, d_vn_kv_from, sn_mva, d_clean_ptr,
+                       d_clean_br, d_amps, d_p, 0);
+}
+
+int pfgpu_batch_flows_dc_device(pfgpu_ctx* ctx, long long nrows, int n, const double* d_V, const int* d_row_src,
+                                int nbr, const int* d_f, const int* d_t, const double* d_yff, const double* d_yft,
+                                const double* d_vn_kv_from, double sn_mva, const int* d_clean_ptr,
+                                const int* d_clean_br, double* d_amps, double* d_p) {
+    return batch_flows(ctx, nrows, n, d_V, d_row_src, nbr, d_f, d_t, d_yff, d_yft, d_vn_kv_from, sn_mva, d_clean_ptr,
+                       d_clean_br, d_amps, d_p, 1);
 }
 
 int pfgpu_sbus_rows_device(pfgpu_ctx* ctx, int T, int ns, const int* d_gen_ptr, const int* d_gen_el, int ngen,

@@ -548,8 +548,10 @@ void BaseBatchSolverSynch::keep_device_rows(int ns, const IntVect& row_src) {
 
 // Amps / active power at the origin side of every line for every result row
 // (BaseBatchSolverSynch.hpp:86-192), on the device from the voltages the last compute left there:
-// I = yff Ef + yft Et, S = Ef conj(I); A = |S| sn_mva / (sqrt(3) |Ef| vn_kv(from)), P = Re(S) sn_mva.
-// Rows that did not converge have zero voltages (NaN amps); lines out of service give 0.
+// I = yff Ef + yft Et, S = Ef conj(I); A = |S| sn_mva / (sqrt(3) |Ef| vn_kv(from)), P = Re(S) sn_mva;
+// after a DC compute P = (Re(yff) arg Ef + Re(yft) arg Et) sn_mva with the DC coefficients and
+// A = |P| / (sqrt(3) |Ef| vn_kv(from)). Rows that did not converge have zero voltages (NaN amps);
+// lines out of service give 0.
 void BaseBatchSolverSynch::compute_flows_from_Vs(bool amps) {
     if (voltages_.empty() || ns_ == 0)
         throw std::runtime_error("BaseMultiplePowerflow::compute_flows_from_Vs: cannot compute the flows as the "
@@ -562,7 +564,8 @@ void BaseBatchSolverSynch::compute_flows_from_Vs(bool amps) {
     for (int k = 0; k < nl; ++k) {
         if (!grid_model_.line_status(k)) continue;
         cplx ytf, ytt;
-        grid_model_.branch_admittance(k, yff[k], yft[k], ytf, ytt);
+        if (dc_rows_) grid_model_.branch_admittance_dc(k, yff[k], yft[k], ytf, ytt);  // ydc_11, ydc_12
+        else grid_model_.branch_admittance(k, yff[k], yft[k], ytf, ytt);
         f[k] = id_me_to_solver_[grid_model_.line_from(k)];
         t[k] = id_me_to_solver_[grid_model_.line_to(k)];
         vn[k] = grid_model_.bus_vn_kv(grid_model_.line_from(k));
@@ -580,10 +583,10 @@ void BaseBatchSolverSynch::compute_flows_from_Vs(bool amps) {
     RealVect& out = amps ? amps_ : pflows_;
     out.assign((size_t)nrows_ * nl, 0.0);
     double* dout = static_cast<double*>(d_tmp_[7].get(std::max<size_t>(out.size(), 1) * 8));
-    check_rc(pfgpu_batch_flows_device(engine_ctx(), nrows_, ns_, d_V_.as<double>(), d_row_src_.as<int>(), nl, df, dt,
-                                      dyff, dyft, dvn, grid_model_.get_sn_mva(), dcp, dcb, amps ? dout : nullptr,
-                                      amps ? nullptr : dout),
-             "pfgpu_batch_flows_device");
+    auto flows = dc_rows_ ? pfgpu_batch_flows_dc_device : pfgpu_batch_flows_device;
+    check_rc(flows(engine_ctx(), nrows_, ns_, d_V_.as<double>(), d_row_src_.as<int>(), nl, df, dt, dyff, dyft, dvn,
+                   grid_model_.get_sn_mva(), dcp, dcb, amps ? dout : nullptr, amps ? nullptr : dout),
+             dc_rows_ ? "pfgpu_batch_flows_dc_device" : "pfgpu_batch_flows_device");
     if (!out.empty()) check_rc(pfgpu_memcpy_d2h(engine_ctx(), out.data(), dout, out.size() * 8), "d2h");
     if (amps) timer_amps_ = since(t0);
 }
@@ -640,14 +643,15 @@ int csc_pos(const CscC& Y, int r, int c) {
 
 // ContingencyAnalysis::init_li_coeffs (ContingencyAnalysis.cpp:47-110): the Ybus positions and values
 // of each line's four coefficients, -1 for a line out of service
-void ContingencyAnalysis::branch_tables(const SolverInput& in, IntVect& pos, CplxVect& y) const {
+void ContingencyAnalysis::branch_tables(const SolverInput& in, bool dc, IntVect& pos, CplxVect& y) const {
     const int nl = grid_model_.nb_powerline();
     pos.assign(4 * (size_t)nl, -1);
     y.assign(4 * (size_t)nl, cplx(0, 0));
     for (int k = 0; k < nl; ++k) {
         if (!grid_model_.line_status(k)) continue;
         cplx yff, yft, ytf, ytt;
-        grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
+        if (dc) grid_model_.branch_admittance_dc(k, yff, yft, ytf, ytt);  // ydc_11 ... ydc_22
+        else grid_model_.branch_admittance(k, yff, yft, ytf, ytt);
         const int f = in.id_me_to_solver[grid_model_.line_from(k)], t = in.id_me_to_solver[grid_model_.line_to(k)];
         pos[4 * k] = csc_pos(in.Ybus, f, f);
         pos[4 * k + 1] = csc_pos(in.Ybus, t, t);
@@ -686,10 +690,12 @@ void add_cont(ContLists& L, const std::set<int>& cont, const IntVect& pos) {
 }  // namespace
 
 IntVect ContingencyAnalysis::is_grid_connected_after_contingency() {
-    SolverInput in = grid_model_.pre_process_solver(CplxVect(grid_model_.total_bus(), cplx(1, 0)));
+    // on the Ybus of the solver in use (ContingencyAnalysis.cpp:133-148): the DC one for a DC solver
+    const bool dc = solver_.current().is_dc();
+    SolverInput in = grid_model_.pre_process_solver(CplxVect(grid_model_.total_bus(), cplx(1, 0)), !dc);
     IntVect pos;
     CplxVect y;
-    branch_tables(in, pos, y);
+    branch_tables(in, dc, pos, y);
     ContLists L;
     for (const auto& cont : defaults_) add_cont(L, cont, pos);
     const int ncont = (int)defaults_.size();
@@ -719,14 +725,17 @@ IntVect ContingencyAnalysis::is_grid_connected_after_contingency() {
 
 // ContingencyAnalysis::compute (ContingencyAnalysis.cpp:166-233), on the device: the contingency Ybus
 // edits and the connectivity check of every contingency, then one batched solve of the connected
-// ones; the voltages stay on the device for the flows.
+// ones; the voltages stay on the device for the flows. With the DC solver (ac_solver_used() false)
+// the DC Ybus and coefficients are used, and every contingency is solved: remove_from_Ybus reports
+// the DC Ybus as always invertible (:113-131) and the solver's own failure decides.
 void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double tol) {
     auto t_all = clk::now();
     auto t_pre = clk::now();
     nb_solved_ = 0;
     timer_solver_ = timer_modif_ybus_ = 0;
     const double sn = grid_model_.get_sn_mva();
-    SolverInput in = grid_model_.pre_process_solver(Vinit);
+    const bool dc = solver_.current().is_dc();
+    SolverInput in = grid_model_.pre_process_solver(Vinit, !dc);
     const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
     const int ncont = (int)defaults_.size();
     nrows_ = ncont;
@@ -736,7 +745,7 @@ void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double to
     amps_.clear();
     pflows_.clear();
     GpuNRSolver& s = solver_.current();
-    if (s.is_dc()) throw std::runtime_error("ContingencyAnalysis: the DC solver is not available in the batch classes of this build");
+    dc_rows_ = dc;
     s.reset();
     // base case ("powerflow in n", BaseBatchSolverSynch.hpp:296-313): tol as given
     CplxVect Vb = in.V;
@@ -749,17 +758,17 @@ void ContingencyAnalysis::compute(const CplxVect& Vinit, int max_iter, double to
     auto t_y = clk::now();
     IntVect pos;
     CplxVect y;
-    branch_tables(in, pos, y);
+    branch_tables(in, dc, pos, y);
     ContLists all;
     for (const auto& cont : defaults_) add_cont(all, cont, pos);
     const int n = in.Ybus.ncol;
     pfgpu_ctx* ctx = engine_ctx();
-    IntVect conn(ncont, 0);
+    IntVect conn(ncont, dc ? 1 : 0);
     DevMem m[9];
     double* dYx = reinterpret_cast<double*>(d_Y_.upload(in.Ybus.x));
     int* dpos_br = m[2].upload(pos);
     double* dy = reinterpret_cast<double*>(m[3].upload(y));
-    if (ncont > 0) {
+    if (ncont > 0 && !dc) {
         int* dcp = m[0].upload(all.ptr);
         int* dcb = m[1].upload(all.br);
         int* ddp = m[4].upload(all.dptr);
@@ -840,7 +849,9 @@ void TimeSeries::clear() {
 // TimeSeries::compute_Vs (TimeSeries.cpp:14-109) on the device: the Sbus rows are built there from
 // the injections, then the warm-started step loop runs inside one launch (pfgpu_solve_chains):
 // by default as the reference's single chain; with set_chain_length(L), chains of L steps side by
-// side, each chain head starting from the initial V.
+// side, each chain head starting from the initial V. With the DC solver every step is solved from
+// the initial V side by side: a DC solve sets the angles from B theta = P alone and keeps the
+// magnitudes of its initial V, so a step's result does not depend on the step before it.
 int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p, const RealVect& load_p,
                            const RealVect& load_q, const CplxVect& Vinit, int max_iter, double tol) {
     auto t_all = clk::now();
@@ -853,7 +864,8 @@ int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p,
         (int)load_q.size() != T * nl)
         throw std::runtime_error("TimeSeries::compute_Vs: injection matrices must be [T x nb_element]");
     const double sn = grid_model_.get_sn_mva();
-    SolverInput in = grid_model_.pre_process_solver(Vinit);
+    const bool dc = solver_.current().is_dc();
+    SolverInput in = grid_model_.pre_process_solver(Vinit, !dc);
     const int nb = grid_model_.total_bus(), ns = (int)in.id_solver_to_me.size();
     nbus_solver_ = ns;
     id_me_to_solver_ = in.id_me_to_solver;
@@ -887,7 +899,7 @@ int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p,
     amps_.clear();
     pflows_.clear();
     GpuNRSolver& s = solver_.current();
-    if (s.is_dc()) throw std::runtime_error("TimeSeries: the DC solver is not available in the batch classes of this build");
+    dc_rows_ = dc;
     s.reset();
     CplxVect Vb = in.V;
     const bool base_conv = s.compute_pf(in.Ybus, Vb, in.Sbus, in.slack_ids, in.slack_weights, in.pv, in.pq, max_iter, tol);
@@ -895,7 +907,7 @@ int TimeSeries::compute_Vs(int T, const RealVect& gen_p, const RealVect& sgen_p,
     timer_pre_proc_ = since(t_pre);
     if (!base_conv) { status_ = 0; timer_total_ = since(t_all); return -1; }
     const CplxVect Vstart = init_from_n_ ? s.get_V() : in.V;
-    const int L = std::max(1, std::min(chain_len_ > 0 ? chain_len_ : T, std::max(T, 1)));
+    const int L = dc ? 1 : std::max(1, std::min(chain_len_ > 0 ? chain_len_ : T, std::max(T, 1)));
     auto t_s = clk::now();
     double* dYx = reinterpret_cast<double*>(d_Y_.upload(in.Ybus.x));
     double* dV0 = reinterpret_cast<double*>(d_V0_.upload(Vstart));

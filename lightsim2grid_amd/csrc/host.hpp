@@ -321,7 +321,8 @@ public:
     int nb_bus_total() const { return grid_model_.total_bus(); }
     int nb_branch() const { return grid_model_.nb_powerline(); }
     // Flows of every result row on the device, from the solved voltages kept there
-    // (pfgpu_batch_flows_device); clean lists the branches zeroed per row (contingencies).
+    // (pfgpu_batch_flows_device, or pfgpu_batch_flows_dc_device after a DC compute); clean lists
+    // the branches zeroed per row (contingencies).
     void compute_flows_from_Vs(bool amps);
 
 protected:
@@ -339,6 +340,7 @@ protected:
     RealVect amps_, pflows_;
     IntVect id_me_to_solver_;  // of the last compute
     int ns_ = 0;               // solver buses of the last compute
+    bool dc_rows_ = false;     // the last compute used the DC solver (ac_solver_used() false)
     DevMem d_V_, d_row_src_, d_iters_, d_status_, d_S_, d_Y_, d_V0_, d_tmp_[8];
 };
 
@@ -365,7 +367,8 @@ private:
     // ContingencyAnalysis::clean_flows: each row zeroes the finite flows of its own branches
     void clean_lists(IntVect& ptr, IntVect& br) const override;
     // per-branch Ybus positions (solver CSC: ff, tt, ft, tf; -1 out of service) and stamps
-    void branch_tables(const SolverInput& in, IntVect& pos, CplxVect& y) const;
+    // (the AC coefficients, or the DC ones for the DC solver)
+    void branch_tables(const SolverInput& in, bool dc, IntVect& pos, CplxVect& y) const;
     std::set<std::set<int>> defaults_;
     double timer_modif_ybus_ = 0;
 };
