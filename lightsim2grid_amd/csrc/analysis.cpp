@@ -624,7 +624,8 @@ void build_jfill_table(DevicePlan& P) {
 // by the mismatch of the previous iteration, read by the update) keep their slots;
 // every other value takes a physical slot freed by a value whose last read came at
 // an earlier time (greedy interval colouring, optimal for interval graphs).
-// Physical slots are then ordered tile first, then by decreasing accesses per iteration.
+// Physical slots are then ordered tile first, then the right-hand side / solution slots,
+// then by decreasing accesses per iteration.
 // ---------------------------------------------------------------------------
 void recycle_slots(DevicePlan& P) {
     // one extra slot: the shared zero (structural zeros, and the operand of padding items)
@@ -718,9 +719,11 @@ void recycle_slots(DevicePlan& P) {
         }
         freeq.push({end[s], phys[s]});
     }
-    // final order: tile first, then by decreasing accesses per iteration (ties: decreasing
-    // hotness), so that a launch whose LDS holds only the first slots sends the fewest accesses
-    // to the spilled ones
+    // final order: tile first, then the right-hand side / solution slots, then by decreasing
+    // accesses per iteration (ties: decreasing hotness), so that a launch whose LDS holds only the
+    // first slots sends the fewest accesses to the spilled ones. The solution slots come first
+    // because the backward levels write and read them: one of them spilled puts a memory round
+    // trip between two of a level's barriers.
     std::vector<int> phot(nphys, -1);
     std::vector<long long> pacc(nphys, 0);
     for (int s = 0; s < NV; ++s) {
@@ -733,8 +736,16 @@ void recycle_slots(DevicePlan& P) {
     for (int s = 0; s < tile; ++s) fin[phys[s]] = s;
     for (int ph = 0; ph < nphys; ++ph)
         if (fin[ph] < 0) pord.push_back(ph);
-    std::stable_sort(pord.begin(), pord.end(),
-                     [&](int a, int b) { return pacc[a] != pacc[b] ? pacc[a] > pacc[b] : phot[a] > phot[b]; });
+    std::vector<char> prhs(nphys, 0);
+    for (int b = 0; b < P.n; ++b) {
+        if (P.fslot_p[b] >= 0) prhs[phys[P.fslot_p[b]]] = 1;
+        if (P.fslot_q[b] >= 0) prhs[phys[P.fslot_q[b]]] = 1;
+    }
+    for (int x : P.ux_slot) prhs[phys[x]] = 1;
+    std::stable_sort(pord.begin(), pord.end(), [&](int a, int b) {
+        if (prhs[a] != prhs[b]) return prhs[a] > prhs[b];
+        return pacc[a] != pacc[b] ? pacc[a] > pacc[b] : phot[a] > phot[b];
+    });
     for (size_t i = 0; i < pord.size(); ++i) fin[pord[i]] = tile + (int)i;
     std::vector<int> mp(NV);
     for (int s = 0; s < NV; ++s) mp[s] = fin[phys[s]];
