@@ -952,63 +952,118 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         tmark(1);
         // ---- dense tail: the m x m tile of every instance ---------------------------------
         // Tile at slots [0, m*(m+1)): row-major with row stride m + 1, the right-hand side
-        // held as column m. Right-looking LU with diagonal pivots, two columns per pass (one
-        // barrier per pass): pass k eliminates columns k and k+1 from the trailing block
-        // (rows >= k+2, columns >= k+2, the vector included) with the rank-2 update
-        //   T(i,j) -= L(i,k) U(k,j) + L(i,k+1) U(k+1,j),
-        //   L(i,k) = T(i,k) / d_k,  U(k+1,j) = T(k+1,j) - L(k+1,k) U(k,j),
-        //   L(i,k+1) = (T(i,k+1) - L(i,k) U(k,k+1)) / d_k+1,
-        // each lane recomputing the L and U factors of its entry from the tile. Row k+1 and
-        // the two reciprocal pivots are stored one pass later (so that no pass writes what
-        // it reads); L itself is not stored, nothing after the forward sweep reads it.
+        // held as column m. Right-looking LU with diagonal pivots, four columns per pass (one
+        // barrier per pass): pass k eliminates columns k .. k+c-1 (c = min(4, m - k)) from the
+        // trailing block (rows >= k+c, columns >= k+c, the vector included) with the rank-c update
+        //   T(i,j) -= sum_q L(i,k+q) U(k+q,j),
+        //   U(k+q,j) = T(k+q,j) - sum_{r<q} L(k+q,k+r) U(k+r,j),
+        //   L(i,k+q) = (T(i,k+q) - sum_{r<q} L(i,k+r) U(k+r,k+q)) / d_k+q,
+        // every lane computing the c x c diagonal block's factors and the L and U factors of its
+        // entry from the tile. Rows k+1 .. k+c-1 of U and the c reciprocal pivots are stored one
+        // pass later (so that no pass writes what it reads); L itself is not stored, nothing
+        // after the forward sweep reads it.
         // Then the backward sweep on the first wave alone (its lanes hold s < 64 / G of
         // every instance; LDS accesses of one wave complete in issue order, so its passes
         // need no barrier): x_k = y_k / d_k, y_i -= U(i,k) x_k for i < k.
         if (a.m > 0) {
             const int m = a.m, ld = m + 1;
             auto T = [&](int i, int j) -> int { return (i * ld + j) * G + g; };  // LDS index
-            FT rp0 = (FT)0, rp1 = (FT)0, lp = (FT)0;  // the previous pass's 1/d_k, 1/d_k+1, L(k+1,k)
+            // the previous pass's block: its columns, reciprocal pivots and L factors
+            int pc = 0;
+            FT pr0 = (FT)0, pr1 = (FT)0, pr2 = (FT)0, pr3 = (FT)0;
+            FT pl10 = (FT)0, pl20 = (FT)0, pl21 = (FT)0, pl30 = (FT)0, pl31 = (FT)0, pl32 = (FT)0;
             bool bad = false;
-            for (int k = 0;; k += 2) {
+            for (int k = 0;; k += 4) {
                 if (act) {
-                    FT rd0 = (FT)0, rd1 = (FT)0, l10 = (FT)0;
-                    if (k < m) {
+                    const int c = min(4, m - k);  // columns of this pass's block (<= 0: none)
+                    FT r0 = (FT)0, r1 = (FT)0, r2 = (FT)0, r3 = (FT)0;
+                    FT l10 = (FT)0, l20 = (FT)0, l21 = (FT)0, l30 = (FT)0, l31 = (FT)0, l32 = (FT)0;
+                    if (c > 0) {
+                        // the block's LU, in the same order in every lane
+                        FT u01 = (FT)0, u02 = (FT)0, u03 = (FT)0, u12 = (FT)0, u13 = (FT)0, u23 = (FT)0;
                         const FT d0 = Xl[T(k, k)];
                         bad |= bad_pivot(d0);
-                        rd0 = pivot_recip(d0);
-                        if (k + 1 < m) {
-                            const FT u01 = Xl[T(k, k + 1)];
-                            l10 = Xl[T(k + 1, k)] * rd0;
+                        r0 = pivot_recip(d0);
+                        if (c > 1) {
+                            u01 = Xl[T(k, k + 1)];
+                            l10 = Xl[T(k + 1, k)] * r0;
                             const FT d1 = Xl[T(k + 1, k + 1)] - l10 * u01;
                             bad |= bad_pivot(d1);
-                            rd1 = pivot_recip(d1);
-                            // entries (i, j), i in [k+2, m), j in [k+2, m]: w rows of w + 1
-                            const int w = m - k - 2, wc = w + 1;
-                            // row = floor((idx + 1/2) / wc): the fractional part of the exact
-                            // quotient stays >= 1/(2 wc) from an integer, far above the float error
-                            const float inv = __builtin_amdgcn_rcpf((float)wc);
-                            for (int idx = s; idx < w * wc; idx += nslot) {
-                                const int r = (int)(((float)idx + 0.5f) * inv);
-                                const int i = k + 2 + r, j = k + 2 + (idx - r * wc);
-                                const FT li0 = Xl[T(i, k)] * rd0;
-                                const FT li1 = (Xl[T(i, k + 1)] - li0 * u01) * rd1;
-                                const FT u0j = Xl[T(k, j)];
+                            r1 = pivot_recip(d1);
+                        }
+                        if (c > 2) {
+                            u02 = Xl[T(k, k + 2)];
+                            u12 = Xl[T(k + 1, k + 2)] - l10 * u02;
+                            l20 = Xl[T(k + 2, k)] * r0;
+                            l21 = (Xl[T(k + 2, k + 1)] - l20 * u01) * r1;
+                            const FT d2 = Xl[T(k + 2, k + 2)] - l20 * u02 - l21 * u12;
+                            bad |= bad_pivot(d2);
+                            r2 = pivot_recip(d2);
+                        }
+                        if (c > 3) {
+                            u03 = Xl[T(k, k + 3)];
+                            u13 = Xl[T(k + 1, k + 3)] - l10 * u03;
+                            u23 = Xl[T(k + 2, k + 3)] - l20 * u03 - l21 * u13;
+                            l30 = Xl[T(k + 3, k)] * r0;
+                            l31 = (Xl[T(k + 3, k + 1)] - l30 * u01) * r1;
+                            l32 = (Xl[T(k + 3, k + 2)] - l30 * u02 - l31 * u12) * r2;
+                            const FT d3 = Xl[T(k + 3, k + 3)] - l30 * u03 - l31 * u13 - l32 * u23;
+                            bad |= bad_pivot(d3);
+                            r3 = pivot_recip(d3);
+                        }
+                        // entries (i, j), i in [k+c, m), j in [k+c, m]: w rows of w + 1
+                        const int w = m - k - c, wc = w + 1;
+                        // row = floor((idx + 1/2) / wc): the fractional part of the exact
+                        // quotient stays >= 1/(2 wc) from an integer, far above the float error
+                        const float inv = __builtin_amdgcn_rcpf((float)wc);
+                        for (int idx = s; idx < w * wc; idx += nslot) {
+                            const int rr = (int)(((float)idx + 0.5f) * inv);
+                            const int i = k + c + rr, j = k + c + (idx - rr * wc);
+                            const FT li0 = Xl[T(i, k)] * r0;
+                            const FT u0j = Xl[T(k, j)];
+                            FT upd = li0 * u0j;
+                            if (c > 1) {
+                                const FT li1 = (Xl[T(i, k + 1)] - li0 * u01) * r1;
                                 const FT u1j = Xl[T(k + 1, j)] - l10 * u0j;
-                                Xl[T(i, j)] -= li0 * u0j + li1 * u1j;
+                                upd += li1 * u1j;
+                                if (c > 2) {
+                                    const FT li2 = (Xl[T(i, k + 2)] - li0 * u02 - li1 * u12) * r2;
+                                    const FT u2j = Xl[T(k + 2, j)] - l20 * u0j - l21 * u1j;
+                                    upd += li2 * u2j;
+                                    if (c > 3) {
+                                        const FT li3 = (Xl[T(i, k + 3)] - li0 * u03 - li1 * u13 - li2 * u23) * r3;
+                                        const FT u3j = Xl[T(k + 3, j)] - l30 * u0j - l31 * u1j - l32 * u2j;
+                                        upd += li3 * u3j;
+                                    }
+                                }
+                            }
+                            Xl[T(i, j)] -= upd;
+                        }
+                    }
+                    if (k >= 4 && pc > 1) {  // rows p+1 .. p+pc-1 of U of the previous pass's block
+                        const int p = k - 4;
+                        for (int j = p + 2 + s; j <= m; j += nslot) {
+                            const FT u0j = Xl[T(p, j)];
+                            const FT u1j = Xl[T(p + 1, j)] - pl10 * u0j;
+                            Xl[T(p + 1, j)] = u1j;
+                            if (pc > 2 && j > p + 2) {
+                                const FT u2j = Xl[T(p + 2, j)] - pl20 * u0j - pl21 * u1j;
+                                Xl[T(p + 2, j)] = u2j;
+                                if (pc > 3 && j > p + 3)
+                                    Xl[T(p + 3, j)] = Xl[T(p + 3, j)] - pl30 * u0j - pl31 * u1j - pl32 * u2j;
                             }
                         }
                     }
-                    if (k >= 2) {  // store row k-1 of U and the pivots k-2, k-1 of the previous pass
-                        const int p = k - 2;
-                        if (p + 1 < m) {
-                            for (int j = p + 2 + s; j <= m; j += nslot) Xl[T(p + 1, j)] -= lp * Xl[T(p, j)];
-                            if (s == 0) Xl[T(p + 1, p + 1)] = rp1;
-                        }
-                        if (s == 0) Xl[T(p, p)] = rp0;
+                    if (k >= 4 && pc > 0 && s == 0) {  // and its reciprocal pivots
+                        const int p = k - 4;
+                        Xl[T(p, p)] = pr0;
+                        if (pc > 1) Xl[T(p + 1, p + 1)] = pr1;
+                        if (pc > 2) Xl[T(p + 2, p + 2)] = pr2;
+                        if (pc > 3) Xl[T(p + 3, p + 3)] = pr3;
                     }
-                    rp0 = rd0;
-                    rp1 = rd1;
-                    lp = l10;
+                    pc = max(c, 0);
+                    pr0 = r0; pr1 = r1; pr2 = r2; pr3 = r3;
+                    pl10 = l10; pl20 = l20; pl21 = l21; pl30 = l30; pl31 = l31; pl32 = l32;
                 }
                 __syncthreads();
                 if (k >= m) break;
@@ -1158,7 +1213,22 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             int pl = 0, pk = 0, pK = lK, pU = lU, pT = lvk(0, 1) >> 24;  // process cursor
             const int wave_s0 = (threadIdx.x & ~63) / G;  // first instance slot of this wave
             FT acc = (FT)0;
-            auto process = [&](const int4 (&cur_raw)[kSC], int cur_v, int4 (&nxt)[kSC], int& nxt_v) {
+            // U entries and pivots are final once the factor phase is done (no value starts in the
+            // backward phase, so no recycled slot changes): the U operands of the next group and
+            // the 1/d of its tasks are read while the current group is worked on, and a spilled
+            // one's memory latency overlaps that work instead of following its item's arrival.
+            auto operands = [&](const int4 (&raw)[kSC], int v, FT (&pu)[kSC], FT (&pd)[kSC]) {
+#pragma unroll
+                for (int u = 0; u < kSC; ++u) {
+                    const int4 q = (v >> u) & 1 ? raw[u] : no_item();
+                    const FT xu = XLo(max(q.x, 0)), xd = XLo(max(q.w, 0));
+                    pu[u] = q.x >= 0 ? xu : (FT)0;
+                    pd[u] = (q.z & kEnd) ? xd : (FT)0;
+                }
+            };
+            auto process = [&](const int4 (&cur_raw)[kSC], int cur_v, const FT (&pu)[kSC], const FT (&pd)[kSC],
+                               int4 (&nxt)[kSC], int& nxt_v, const int4 (&pend)[kSC], int pend_v, FT (&pend_u)[kSC],
+                               FT (&pend_d)[kSC]) {
                 int4 cur[kSC];
 #pragma unroll
                 for (int u = 0; u < kSC; ++u) cur[u] = (cur_v >> u) & 1 ? cur_raw[u] : no_item();
@@ -1170,14 +1240,16 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 for (int u = 0; u < kSC; ++u) {
                     if (!wave_on) break;
                     const int4 q = cur[u];
-                    prod[u] = (FT)0;
-                    if (max((unsigned)q.x, (unsigned)q.y) < (unsigned)nlds) prod[u] = Xl[q.x * G + g] * Xl[q.y * G + g];  // all operands in LDS
-                    else if (q.x >= 0) prod[u] = XL(q.x) * XL(q.y);
+                    FT xv = (FT)0;  // x' is written by earlier backward levels: read after their barriers
+                    if ((unsigned)q.y < (unsigned)nlds) xv = Xl[q.y * G + g];
+                    else if (q.y >= 0) xv = Xgb[(size_t)(q.y - nlds) * G + g];
+                    prod[u] = pu[u] * xv;
                     last |= q.z < 0;
                 }
                 // the lane's last item: later chunks of the load cursor's level are skipped
                 if (last && pl == ll) more = false;
                 prefetch(nxt, nxt_v);
+                operands(pend, pend_v, pend_u, pend_d);
                 if (act && wave_on) {
 #pragma unroll
                     for (int u = 0; u < kSC; ++u) {
@@ -1191,7 +1263,7 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                         const bool lead = (s & ((1 << lt) - 1)) == 0;
                             if (lead) {
                                 const int slot = e & ((1 << 25) - 1);
-                                const FT rd = XL(cur[u].w);
+                                const FT rd = pd[u];
                                 if (slot < nlds) {
                                     Xl[slot * G + g] = (Xl[slot * G + g] - sum) * rd;
                                 } else {
@@ -1216,14 +1288,16 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
             };
             int4 A[kSC], B[kSC], C[kSC];
             int Av = 0, Bv = 0, Cv = 0;
+            FT Au[kSC], Ad[kSC], Bu[kSC], Bd[kSC], Cu[kSC], Cd[kSC];
             prefetch(A, Av);
             prefetch(B, Bv);
+            operands(A, Av, Au, Ad);
             while (pl < a.nblev) {
-                process(A, Av, C, Cv);
+                process(A, Av, Au, Ad, C, Cv, B, Bv, Bu, Bd);
                 if (pl >= a.nblev) break;
-                process(B, Bv, A, Av);
+                process(B, Bv, Bu, Bd, A, Av, C, Cv, Cu, Cd);
                 if (pl >= a.nblev) break;
-                process(C, Cv, B, Bv);
+                process(C, Cv, Cu, Cd, B, Bv, A, Av, Au, Ad);
             }
             }
         }
