@@ -361,6 +361,16 @@ __device__ __forceinline__ FT team_sum(FT acc, int lt, int maxlt) {
 // the FP64 form an IEEE division.
 __device__ __forceinline__ float pivot_recip(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ double pivot_recip(double v) { return 1.0 / v; }
+// v of lane `lane` (uniform over the wave) in every lane
+__device__ __forceinline__ float lane_read(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ __forceinline__ double lane_read(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 // zero or non-finite pivot (klu_refactor failure)
 template <typename FT>
 __device__ __forceinline__ bool bad_pivot(FT v) { return !(fabs(v) > (FT)0 && fabs(v) < (FT)INFINITY); }
@@ -964,7 +974,8 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
         // after the forward sweep reads it.
         // Then the backward sweep on the first wave alone (its lanes hold s < 64 / G of
         // every instance; LDS accesses of one wave complete in issue order, so its passes
-        // need no barrier): x_k = y_k / d_k, y_i -= U(i,k) x_k for i < k.
+        // need no barrier): x_k = y_k / d_k, y_i -= U(i,k) x_k for i < k; with one instance per
+        // workgroup, in registers.
         if (a.m > 0) {
             const int m = a.m, ld = m + 1;
             auto T = [&](int i, int j) -> int { return (i * ld + j) * G + g; };  // LDS index
@@ -1069,7 +1080,39 @@ __global__ void __launch_bounds__(NT) k_nr(const KArgs a) {
                 if (k >= m) break;
             }
             if (bad && s == 0) s_piv[g] = 1;
-            if (threadIdx.x < 64 && act) {
+            if (G == 1 && m <= 128) {
+                // One instance per workgroup: lane s of the first wave holds y_s and y_s+64 in
+                // registers, and x_k = y_k / d_k reaches every lane by a lane read of its owner,
+                // so a step is a few dependent register operations. U(i, k) and 1/d_k are not
+                // written by the sweep: each step's reads are issued one step ahead.
+                if (threadIdx.x < 64 && act) {
+                    FT y0 = s < m ? Xl[T(s, m)] : (FT)0, y1 = s + 64 < m ? Xl[T(s + 64, m)] : (FT)0;
+                    // column k's reads, clamped into the tile (a lane's U entries of rows i >= k
+                    // are read but not used)
+                    auto col = [&](int k, FT& u0, FT& u1, FT& dk) {
+                        const int c = max(k, 0);
+                        dk = Xl[T(c, c)];
+                        u0 = Xl[T(min(s, m - 1), c)];
+                        u1 = Xl[T(min(s + 64, m - 1), c)];
+                    };
+                    FT u0, u1, dk;
+                    col(m - 1, u0, u1, dk);
+                    for (int k = m - 1; k >= 0; --k) {
+                        FT nu0, nu1, ndk;
+                        col(k - 1, nu0, nu1, ndk);
+                        const FT xk = lane_read(k < 64 ? y0 : y1, k & 63) * dk;
+                        if (s == (k & 63)) {
+                            if (k < 64) y0 = xk;
+                            else y1 = xk;
+                        }
+                        if (s < k) y0 -= u0 * xk;  // rows i < k only: a non-finite x_k leaves the others
+                        if (s + 64 < k) y1 -= u1 * xk;
+                        u0 = nu0; u1 = nu1; dk = ndk;
+                    }
+                    if (s < m) Xl[T(s, m)] = y0;
+                    if (s + 64 < m) Xl[T(s + 64, m)] = y1;
+                }
+            } else if (threadIdx.x < 64 && act) {
                 constexpr int ws = 64 / G;  // lanes of one instance in the first wave
                 for (int k = m - 1; k >= 0; --k) {
                     const FT xk = Xl[T(k, m)] * Xl[T(k, k)];
